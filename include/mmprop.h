@@ -1,0 +1,195 @@
+/*
+ * mmprop.h -- C ABI of the MI355X (gfx950) Lawson-Runge-Kutta multimode
+ * propagation library (libmmprop_hip.so).
+ *
+ * The reference exposes this hot path only as a C++ class API
+ * (include/MultimodePropagation.hpp:16-94, include/MultimodePropagation_GPU.cuh:10-85).
+ * Every entry point below names the reference member it replaces. Plain
+ * pointers and sizes only; no C++ exception crosses this boundary. All
+ * arrays are host arrays owned by the caller; the library copies what it
+ * needs. Complex arrays are interleaved doubles: element (p, j) of an
+ * M x NT complex container lives at [2*(p*NT + j)] (re) and [+1] (im).
+ *
+ * Threading: one solver handle is used by one host thread at a time.
+ * Different handles may be driven from different threads; each context owns
+ * its own HIP stream.
+ */
+#ifndef MMPROP_H
+#define MMPROP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MMPROP_ABI_VERSION 1
+
+typedef enum mmp_status {
+  MMP_OK = 0,
+  MMP_ERR_INVALID_ARGUMENT = 1, /* malformed parameters (sizes, indices, NULLs, max|Q| == 0, ...) */
+  MMP_ERR_UNSUPPORTED = 2,      /* valid for the reference, not supported here (e.g. NT not a power of two) */
+  MMP_ERR_OUT_OF_MEMORY = 3,    /* device or host allocation failed */
+  MMP_ERR_DEVICE = 4,           /* a HIP call failed */
+  MMP_ERR_NO_DEVICE = 5,        /* no usable gfx950 device / kernels not loadable */
+  MMP_ERR_NONFINITE = 6,        /* non-finite value in inputs or in the propagated field */
+  MMP_ERR_STATE = 7             /* call out of order (e.g. result requested before any step) */
+} mmp_status;
+
+/* Precision of the propagation work (FFTs, Kerr/Raman sums). */
+enum {
+  MMP_PRECISION_FP64 = 0 /* plain FP64 solver: the drop-in for the reference */
+};
+
+/* Which (a, b) pair a Raman/Kerr slot j holds (V_j = U_a * conj(U_b)).
+ * REFERENCE_CPU reproduces src/MultimodePropagation_CPU.cpp:126-133,196-204:
+ * a = j % M, b = j / M, j being the slot counter. PHYSICAL decodes the stored
+ * pair code l + m*M (src/MultimodePropagation.cpp:139-155): a = l, b = m. */
+enum {
+  MMP_PAIRS_REFERENCE_CPU = 0,
+  MMP_PAIRS_PHYSICAL = 1
+};
+
+/* Option flags. */
+enum {
+  MMP_FLAG_NONE = 0,
+  MMP_FLAG_PROFILE = 1u << 0,     /* record per-kernel HIP event times (see mmp_kernel_times) */
+  MMP_FLAG_NO_GRAPH = 1u << 1     /* launch step kernels one by one instead of replaying a HIP graph */
+};
+
+/*
+ * The reference's SimulationParameters (include/MultimodePropagation.hpp:16-33,
+ * called InputParameters in README.md:71-86), flattened.
+ *
+ * coupling_coefficients: Q^(p) for p < n_modes is the COO list
+ *   entries [coupling_offsets[p], coupling_offsets[p+1]) of coupling_klm (k,l,m
+ *   triplets) and coupling_values, in insertion order
+ *   (src/Sparse3DArray.cpp:16-19). The order matters: it fixes the pair-slot
+ *   order (src/MultimodePropagation.cpp:139-155).
+ * raman: 1 selects the RamanON class (raman_response required), 0 the RamanOFF
+ *   class (raman_proportion and raman_response ignored), mirroring the class
+ *   choice of matlab/mm_propagation.cpp:59.
+ */
+typedef struct mmp_params {
+  uint32_t n_modes;                       /* M */
+  double fiber_length;                    /* m */
+  uint32_t n_beta;                        /* dispersion orders per mode, >= 3 */
+  const double* dispersion_coefficients;  /* [M][n_beta], ps^l/m */
+  const uint32_t* coupling_offsets;       /* [M+1] */
+  const uint32_t* coupling_klm;           /* [nnz][3] */
+  const double* coupling_values;          /* [nnz], m^-2 */
+  uint32_t nt;                            /* NT */
+  double time_window;                     /* ps */
+  double pulse_width;                     /* T0, ps */
+  const double* initial_fields;           /* [M][NT] complex, W^(1/2) */
+  uint32_t n_steps;
+  uint32_t method_order;                  /* 8 -> 11 stages, 6 -> 7 stages, else RK4 */
+  double nonlinearity_const;              /* gamma, W^-1 m */
+  double raman_proportion;                /* fR */
+  const double* raman_response;           /* [NT], h_R sampled on t_j = j*dt */
+  uint32_t raman;                         /* 0 = RamanOFF, 1 = RamanON */
+} mmp_params;
+
+typedef struct mmp_options {
+  uint32_t precision;    /* MMP_PRECISION_* */
+  uint32_t pair_decode;  /* MMP_PAIRS_* */
+  uint32_t flags;        /* MMP_FLAG_* */
+  uint32_t reserved;
+} mmp_options;
+
+/* Scalars the reference computes in its constructor
+ * (src/MultimodePropagation.cpp:31-63, 139-155) plus the solver state. */
+typedef struct mmp_info {
+  uint32_t n_modes, nt, stages, n_pairs, nnz, n_conv_lines, n1, n2;
+  uint32_t steps_taken;      /* steps since the last restart (z = 0) */
+  uint32_t default_steps;    /* floor(z_final / delta_z), src/MultimodePropagation.cpp:213-216 */
+  double t_final, delta_t, z_final, delta_z, z;
+  double conversion_factor, sign_gamma, ld, max_beta2, max_q;
+  size_t device_bytes;       /* device memory held by the solver */
+} mmp_info;
+
+typedef struct mmp_ctx mmp_ctx;
+typedef struct mmp_solver mmp_solver;
+
+/* Status text for a status code; never NULL. */
+const char* mmp_status_string(mmp_status status);
+
+/* Detail of the last failed call on this host thread ("" if none). */
+const char* mmp_last_error(void);
+
+/* ABI version compiled into the library (MMPROP_ABI_VERSION). */
+int mmp_abi_version(void);
+
+/* Device context: one HIP stream on one device. No reference counterpart:
+ * the reference hard-codes device 0 and the default stream
+ * (src/MultimodePropagation_GPU.cu:17). */
+mmp_status mmp_ctx_create(int device, mmp_ctx** out);
+void mmp_ctx_destroy(mmp_ctx* ctx);
+
+/* Replaces the constructors MultimodePropagationGPU_RamanON/OFF(const
+ * SimulationParameters&) (include/MultimodePropagation_GPU.cuh:184,196;
+ * src/MultimodePropagation.cpp:4-12): validates and copies the inputs, makes
+ * them dimensionless, builds the pair table and uploads everything once.
+ * options may be NULL (FP64, REFERENCE_CPU pairs, graphs on). */
+mmp_status mmp_solver_create(mmp_ctx* ctx, const mmp_params* params,
+                             const mmp_options* options, mmp_solver** out);
+void mmp_solver_destroy(mmp_solver* solver);
+
+/* Replaces computeLawsonRK(nz) (include/MultimodePropagation_GPU.cuh:143;
+ * semantics of src/MultimodePropagation_CPU.cpp:24-34): restarts at z = 0
+ * with freshly built Lawson operators, keeps the current field, then takes
+ * nz steps. */
+mmp_status mmp_compute_lawson_rk(mmp_solver* solver, uint32_t nz);
+
+/* Replaces computeLawsonRK() (src/MultimodePropagation.cpp:213-216):
+ * mmp_compute_lawson_rk with nz = floor(z_final / delta_z). */
+mmp_status mmp_compute_lawson_rk_default(mmp_solver* solver);
+
+/* Resumable stepping (no reference counterpart): continues from the current
+ * z without restarting. mmp_compute_lawson_rk(s, a) followed by
+ * mmp_advance(s, b) equals mmp_compute_lawson_rk(s, a + b). */
+mmp_status mmp_advance(mmp_solver* solver, uint32_t n_steps);
+
+/* Replaces getResult() (src/MultimodePropagation_CPU.cpp:88-104): the
+ * physical field A_p = F^-1[exp(L_p z) F psi_p] / conversion_factor, written
+ * to fields_out [M][NT] complex. */
+mmp_status mmp_get_result(mmp_solver* solver, double* fields_out);
+
+/* Dimensionless interaction-picture field psi [M][NT] complex and the
+ * accumulated z (the reference's protected psi and z members). Either
+ * pointer may be NULL. */
+mmp_status mmp_get_state(mmp_solver* solver, double* psi_out, double* z_out);
+
+/* Replaces the dimensionless field and restarts at z = 0 (the reference
+ * keeps psi across computeLawsonRK calls; this sets it explicitly).
+ * psi is [M][NT] complex, in dimensionless units. */
+mmp_status mmp_set_state(mmp_solver* solver, const double* psi);
+
+/* Per-mode energy of the physical field, E_p = sum_j |A_p,j|^2 * (T_w / NT)
+ * in W*ps, computed on the device in FP64. energy_out is [M]. */
+mmp_status mmp_energy(mmp_solver* solver, double* energy_out);
+
+/* Solver scalars and state. */
+mmp_status mmp_solver_info(mmp_solver* solver, mmp_info* info_out);
+
+/* With MMP_FLAG_PROFILE: the names and summed HIP-event durations (ms) and
+ * launch counts of the kernels issued since the last call (then reset).
+ * Writes up to capacity entries; *count_out receives the number available. */
+typedef struct mmp_kernel_time {
+  char name[48];
+  double total_ms;
+  uint64_t launches;
+  double bytes_per_launch;  /* algorithmic HBM bytes of one launch */
+} mmp_kernel_time;
+mmp_status mmp_kernel_times(mmp_solver* solver, mmp_kernel_time* out,
+                            uint32_t capacity, uint32_t* count_out);
+
+/* Block until all work queued on the context's stream is complete. */
+mmp_status mmp_ctx_synchronize(mmp_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MMPROP_H */

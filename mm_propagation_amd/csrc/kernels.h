@@ -1,0 +1,104 @@
+// kernels.h -- launch interface of the gfx950 propagation kernels.
+//
+// Data layout in HBM (complex FP64 = 16 B, NT = N1 * N2, row-major [N1][N2]):
+//   time domain   x[q][n1*N2 + n2]  (natural sample order t = n1*N2 + n2)
+//   spectral      S[q][k1*N2 + k2]  holds X[k1 + N1*k2]  (four-step order)
+// A forward transform is a column pass (length-N1 DFT down each column n2,
+// then twiddle W_N^(n2*k1)) followed by a row pass (length-N2 DFT along each
+// row k1); the inverse runs the row pass first. Spectral state never leaves
+// the four-step order, so no transposes are needed.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fft_line.h"
+
+namespace mmp {
+
+constexpr int kMaxStages = 11;
+
+// Conv-line kinds: how a pair slot reads its Raman-convolved product.
+enum SlotKind : int { kDirect = 0, kConj = 1, kRe = 2, kIm = 3 };
+
+struct Geometry {
+  int N, N1, N2, M;
+};
+
+// Twiddle tables on the device.
+struct Twiddles {
+  const cd* rowtw;   // W_N2^j, j < N2
+  const cd* coltw;   // W_N1^j, j < N1
+  const cd* fourtw;  // W_N^e, e < N
+};
+
+// Per-step device state updated on the stream (z by repeated FP64 addition).
+struct StepState {
+  double z;
+  uint32_t steps;
+  uint32_t pad;
+};
+
+// Arguments of the row "stage head" kernel for stage i of the tableau
+// (i == s finishes the step).
+struct HeadArgs {
+  const cd* ysig;      // column-forwarded sigma of stage i-1 (unused when has_prev == 0)
+  cd* psi_hat;         // spectral state
+  cd* acc;             // spectral RK accumulator
+  cd* stash;           // [s][M][N] spectral N_j kept for later stages
+  cd* yout;            // row-inverse output for stage i (unused when final)
+  const double* imL;   // Im L_p(k), four-step order [M][N]
+  const StepState* st;
+  double ch_prev, ch_cur;  // c_{i-1} h, c_i h
+  double bh_prev;          // b_{i-1} h (0 when b_{i-1} == 0)
+  double sgn;              // sign of gamma
+  int has_prev;            // 1 when i > 0
+  int prev_stage;          // i - 1
+  int prev_is_first;       // 1 when i - 1 == 0 (acc starts from psi_hat)
+  int stash_prev;          // 1 when N_{i-1} is read by a later stage
+  int final_stage;         // 1 when i == s: psi_hat = acc, no row-inverse output
+  int n_a;                 // number of nonzero a_ij, j < i, in increasing j
+  int a_idx[kMaxStages];   // j (N_{i-1} comes from registers, earlier N_j from stash)
+  double a_h[kMaxStages];  // a_ij h
+};
+
+struct GatherArgs {
+  const uint32_t* off;     // [M+1] nonzeros of Q^(p)
+  const uint32_t* k;       // [nnz]
+  const uint32_t* slot;    // [nnz]
+  const double* c1;        // [nnz] Q (1 - fR)  (or Q with Raman off)
+  const double* c2;        // [nnz] Q fR T0
+  const uint32_t* slot_a;  // [P]
+  const uint32_t* slot_b;  // [P]
+  const uint32_t* slot_line;  // [P] Raman conv line
+  const uint32_t* slot_kind;  // [P]
+  int raman;
+};
+
+struct PairLineArgs {
+  const uint32_t* line_a;  // [n_lines]
+  const uint32_t* line_b;  // [n_lines]; for self lines the second mode or 0xffffffff
+  const uint32_t* line_self;  // [n_lines] 1 = packed self pairs, 0 = cross pair a, b
+};
+
+// Launchers (return hipSuccess or the launch error).
+hipError_t launch_row_head(const Geometry& g, const Twiddles& tw, const HeadArgs& a, hipStream_t s);
+hipError_t launch_col_inverse(const Geometry& g, const Twiddles& tw, const cd* in, cd* out, int lines, double scale,
+                              double divisor, hipStream_t s);
+hipError_t launch_col_forward(const Geometry& g, const Twiddles& tw, const cd* in, cd* out, int lines,
+                              hipStream_t s);
+hipError_t launch_col_forward_real(const Geometry& g, const Twiddles& tw, const double* in, cd* out,
+                                   hipStream_t s);
+hipError_t launch_col_pairs_forward(const Geometry& g, const Twiddles& tw, const cd* U, cd* out, int lines,
+                                    const PairLineArgs& pl, hipStream_t s);
+hipError_t launch_row_forward(const Geometry& g, const Twiddles& tw, cd* data, int lines, hipStream_t s);
+hipError_t launch_row_inverse_phase(const Geometry& g, const Twiddles& tw, const cd* in, cd* out,
+                                    const double* imL, const StepState* st, int use_phase, hipStream_t s);
+hipError_t launch_row_raman(const Geometry& g, const Twiddles& tw, cd* data, int lines, const cd* hspec,
+                            double hscale, hipStream_t s);
+hipError_t launch_gather(const Geometry& g, const cd* U, const cd* VR, cd* sigma, const GatherArgs& a,
+                         hipStream_t s);
+hipError_t launch_advance_z(StepState* st, double h, hipStream_t s);
+hipError_t launch_energy(const Geometry& g, const cd* fields, double* out, double dt, hipStream_t s);
+
+}  // namespace mmp

@@ -1,0 +1,305 @@
+// setup.cpp -- host-side FP64 preparation (see setup.h).
+//
+// The scalar and operator arithmetic follows the reference's expressions and
+// operation order (src/MultimodePropagation.cpp:31-199) so that the prepared
+// psi0 and Im L are bit-identical to the reference's; that file is built with
+// FMA contraction off for this reason.
+#include "setup.h"
+
+#include <cmath>
+
+// Keep a*b+c as two roundings, as the reference's x86-64 build does.
+#pragma clang fp contract(off)
+#include <complex>
+#include <cstring>
+
+namespace mmp {
+
+Tableau make_tableau(uint32_t method_order) {
+  Tableau t;
+  const double r = std::sqrt(21.);
+  if (method_order == 8) {
+    // 11 stages, order 8 (src/RungeKutta.cpp:125-175); the reference's 12th c entry is never read.
+    t.s = 11;
+    auto& a = t.a;
+    a[1][0] = 1 / 2.;
+    a[2][0] = 1 / 4.; a[2][1] = 1 / 4.;
+    a[3][0] = 1 / 7.; a[3][1] = (-7. + 3. * r) / 98.; a[3][2] = (21. - 5. * r) / 49.;
+    a[4][0] = (11. - r) / 84.; a[4][2] = (18. - 4. * r) / 63.; a[4][3] = (21. + r) / 252.;
+    a[5][0] = (5. - r) / 48.; a[5][2] = (9. - r) / 36.; a[5][3] = (-231. - 14. * r) / 360.; a[5][4] = (63. + 7. * r) / 80.;
+    a[6][0] = (10. + r) / 42.; a[6][2] = (-432. - 92. * r) / 315.; a[6][3] = (633. + 145. * r) / 90.;
+    a[6][4] = (-504. - 115. * r) / 70.; a[6][5] = (63. + 13. * r) / 35.;
+    a[7][0] = 1 / 14.; a[7][4] = (14. + 3. * r) / 126.; a[7][5] = (13. + 3. * r) / 63.; a[7][6] = 1 / 9.;
+    a[8][0] = 1 / 32.; a[8][4] = (91. + 21. * r) / 576.; a[8][5] = 11 / 72.; a[8][6] = (-385. + 75. * r) / 1152.;
+    a[8][7] = (63. - 13. * r) / 128.;
+    a[9][0] = 1 / 14.; a[9][4] = 1 / 9.; a[9][5] = (-733. + 147. * r) / 2205.; a[9][6] = (515. - 111. * r) / 504.;
+    a[9][7] = (-51. + 11. * r) / 56.; a[9][8] = (132. - 28. * r) / 245.;
+    a[10][4] = (-42. - 7. * r) / 18.; a[10][5] = (-18. - 28. * r) / 45.; a[10][6] = (-273. + 53. * r) / 72.;
+    a[10][7] = (301. - 53. * r) / 72.; a[10][8] = (28. + 28. * r) / 45.; a[10][9] = (49. + 7. * r) / 18.;
+    const double b[11] = {1 / 20., 0., 0., 0., 0., 0., 0., 49 / 180., 16 / 45., 49 / 180., 1 / 20.};
+    const double c[11] = {0., 1 / 2., 1 / 2., (7. - r) / 14., (7. - r) / 14., 1 / 2., (7. + r) / 14.,
+                          (7. + r) / 14., 1 / 2., (7. - r) / 14., 1.};
+    std::memcpy(t.b, b, sizeof b);
+    std::memcpy(t.c, c, sizeof c);
+  } else if (method_order == 6) {
+    // 7 stages, order 6, nu = 1 (src/RungeKutta.cpp:24-72 called with nu = 1)
+    const double nu = 1.;
+    t.s = 7;
+    auto& a = t.a;
+    a[1][0] = nu;
+    a[2][0] = (4. * nu - 1.) / (8. * nu); a[2][1] = 1. / (8. * nu);
+    a[3][0] = (10. * nu - 2.) / (27. * nu); a[3][1] = 2. / (27. * nu); a[3][2] = (8. * nu) / (27. * nu);
+    a[4][0] = (56. - 77. * nu + (8. - 17. * nu) * r) / (392. * nu); a[4][1] = (-8. * (7. + r)) / (392. * nu);
+    a[4][2] = (48. * (7. + r) * nu) / (392. * nu); a[4][3] = (-3. * (21. + r) * nu) / (392. * nu);
+    a[5][0] = (-5. * (287. * nu - 56. - (59. * nu - 8.) * r)) / (1960. * nu); a[5][1] = (-40. * (7. - r)) / (1960. * nu);
+    a[5][2] = (320. * r * nu) / (1960. * nu); a[5][3] = (3. * (21. - 121. * r) * nu) / (1960. * nu);
+    a[5][4] = (392. * (6. - r) * nu) / (1960. * nu);
+    a[6][0] = (15. * ((30. * nu - 8.) - 7. * nu * r)) / (180. * nu); a[6][1] = 120. / (180. * nu);
+    a[6][2] = (-40. * (5. + 7. * r) * nu) / (180. * nu); a[6][3] = (63. * (2. + 3. * r) * nu) / (180. * nu);
+    a[6][4] = (-14. * (49. - 9. * r) * nu) / (180. * nu); a[6][5] = (70. * (7. + r) * nu) / (180. * nu);
+    const double b[7] = {9 / 180., 0., 64 / 180., 0., 49 / 180., 49 / 180., 9 / 180.};
+    const double c[7] = {0., nu, 1 / 2., 2 / 3., (7. + r) / 14., (7. - r) / 14., 1.};
+    std::memcpy(t.b, b, sizeof b);
+    std::memcpy(t.c, c, sizeof c);
+  } else {
+    // classic RK4 (src/RungeKutta.cpp:3-21)
+    t.s = 4;
+    t.a[1][0] = 1 / 2.;
+    t.a[2][1] = 1 / 2.;
+    t.a[3][2] = 1.;
+    const double b[4] = {1 / 6., 1 / 3., 1 / 3., 1 / 6.};
+    const double c[4] = {0., 1 / 2., 1 / 2., 1.};
+    std::memcpy(t.b, b, sizeof b);
+    std::memcpy(t.c, c, sizeof c);
+  }
+  return t;
+}
+
+bool split_length(uint32_t nt, uint32_t& n1, uint32_t& n2) {
+  if (nt < 64 || nt > 65536 || (nt & (nt - 1)) != 0) return false;
+  uint32_t lg = 0;
+  while ((1u << lg) < nt) lg++;
+  n1 = 1u << (lg / 2);
+  n2 = nt / n1;
+  return true;
+}
+
+std::vector<double> twiddle_table(uint32_t n) {
+  std::vector<double> w(2 * (size_t)n);
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  for (uint32_t j = 0; j < n; j++) {
+    const long double ang = -two_pi * (long double)j / (long double)n;
+    w[2 * j] = (double)cosl(ang);
+    w[2 * j + 1] = (double)sinl(ang);
+  }
+  return w;
+}
+
+static int factorial_int(int a) {
+  int r = 1;
+  for (int i = 1; i <= a; i++) r *= i;
+  return r;
+}
+
+static bool all_finite(const double* x, size_t n) {
+  for (size_t i = 0; i < n; i++)
+    if (!std::isfinite(x[i])) return false;
+  return true;
+}
+
+mmp_status prepare_problem(const mmp_params* in, const mmp_options& opt, Problem& P, std::string& err) {
+  if (!in) { err = "params is NULL"; return MMP_ERR_INVALID_ARGUMENT; }
+  const uint32_t M = in->n_modes, nt = in->nt, nb = in->n_beta;
+  if (M == 0) { err = "n_modes must be >= 1"; return MMP_ERR_INVALID_ARGUMENT; }
+  if (M > 64) { err = "n_modes above 64 is not supported"; return MMP_ERR_UNSUPPORTED; }
+  if (nt == 0) { err = "nt must be >= 1"; return MMP_ERR_INVALID_ARGUMENT; }
+  if (nb < 3) {
+    err = "dispersion_coefficients needs at least 3 orders per mode (beta_2 is read)";
+    return MMP_ERR_INVALID_ARGUMENT;
+  }
+  if (!in->dispersion_coefficients || !in->coupling_offsets || !in->initial_fields) {
+    err = "dispersion_coefficients, coupling_offsets and initial_fields are required";
+    return MMP_ERR_INVALID_ARGUMENT;
+  }
+  if (in->n_steps == 0) { err = "n_steps must be >= 1"; return MMP_ERR_INVALID_ARGUMENT; }
+  if (in->raman && !in->raman_response) { err = "raman_response is required with Raman on"; return MMP_ERR_INVALID_ARGUMENT; }
+  if (opt.precision != MMP_PRECISION_FP64) { err = "unknown precision"; return MMP_ERR_UNSUPPORTED; }
+  if (opt.pair_decode != MMP_PAIRS_REFERENCE_CPU && opt.pair_decode != MMP_PAIRS_PHYSICAL) {
+    err = "unknown pair_decode";
+    return MMP_ERR_INVALID_ARGUMENT;
+  }
+  if (in->coupling_offsets[0] != 0) { err = "coupling_offsets[0] must be 0"; return MMP_ERR_INVALID_ARGUMENT; }
+  for (uint32_t p = 0; p < M; p++)
+    if (in->coupling_offsets[p + 1] < in->coupling_offsets[p]) {
+      err = "coupling_offsets must be non-decreasing";
+      return MMP_ERR_INVALID_ARGUMENT;
+    }
+  const uint32_t nnz = in->coupling_offsets[M];
+  if (nnz && (!in->coupling_klm || !in->coupling_values)) { err = "coupling arrays missing"; return MMP_ERR_INVALID_ARGUMENT; }
+  for (size_t i = 0; i < 3 * (size_t)nnz; i++)
+    if (in->coupling_klm[i] >= M) { err = "coupling index (k, l or m) >= n_modes"; return MMP_ERR_INVALID_ARGUMENT; }
+  if (!all_finite(in->dispersion_coefficients, (size_t)M * nb) || (nnz && !all_finite(in->coupling_values, nnz)) ||
+      !all_finite(in->initial_fields, 2 * (size_t)M * nt) || (in->raman && !all_finite(in->raman_response, nt)) ||
+      !std::isfinite(in->fiber_length) || !std::isfinite(in->time_window) || !std::isfinite(in->pulse_width) ||
+      !std::isfinite(in->nonlinearity_const) || !std::isfinite(in->raman_proportion)) {
+    err = "non-finite input";
+    return MMP_ERR_NONFINITE;
+  }
+  if (!split_length(nt, P.N1, P.N2)) {
+    err = "nt must be a power of two in [64, 65536]";
+    return MMP_ERR_UNSUPPORTED;
+  }
+  P.M = M;
+  P.N = nt;
+  P.n_beta = nb;
+  P.nnz = nnz;
+  P.n_steps = in->n_steps;
+  P.rk = make_tableau(in->method_order);
+  P.raman = in->raman ? 1 : 0;
+  P.fR = in->raman_proportion;
+
+  // setDimensionlessVariables (src/MultimodePropagation.cpp:31-56)
+  P.T0 = in->pulse_width;
+  const double gamma = in->nonlinearity_const;
+  P.sign_gamma = gamma >= 0 ? 1 : -1;
+  double max_beta2 = 0.;
+  for (uint32_t p = 0; p < M; p++) {
+    const double v = std::abs(in->dispersion_coefficients[(size_t)p * nb + 2]);
+    if (v > max_beta2) max_beta2 = v;
+  }
+  double max_q = 0.;
+  for (uint32_t i = 0; i < nnz; i++) {
+    const double v = std::abs(in->coupling_values[i]);
+    if (v > max_q) max_q = v;
+  }
+  if (max_beta2 == 0.) { err = "max |beta_2| is zero (the dispersion length is undefined)"; return MMP_ERR_INVALID_ARGUMENT; }
+  if (max_q == 0.) { err = "max |Q| is zero (the coupling normalisation is undefined)"; return MMP_ERR_INVALID_ARGUMENT; }
+  if (P.T0 <= 0. || in->time_window <= 0. || in->fiber_length <= 0.) {
+    err = "pulse_width, time_window and fiber_length must be positive";
+    return MMP_ERR_INVALID_ARGUMENT;
+  }
+  P.max_beta2 = max_beta2;
+  P.max_q = max_q;
+  const double LD = (P.T0 * P.T0) / max_beta2;
+  P.ld = LD;
+  P.beta_tilde.resize((size_t)M * nb);
+  std::vector<double> cf(nb);
+  for (uint32_t i = 0; i < nb; i++) cf[i] = 1. / std::pow(P.T0, i) * LD;
+  for (uint32_t p = 0; p < M; p++)
+    for (uint32_t i = 0; i < nb; i++)
+      P.beta_tilde[(size_t)p * nb + i] = in->dispersion_coefficients[(size_t)p * nb + i] * cf[i];
+  P.t_final = in->time_window / P.T0;
+  P.delta_t = (double)P.t_final / nt;
+  P.z_final = in->fiber_length / LD;
+  P.delta_z = P.z_final / in->n_steps;
+  P.conversion_factor = std::sqrt(std::abs(gamma) * LD) * std::sqrt(max_q);
+  P.default_steps = (uint32_t)std::floor(P.z_final / P.delta_z);
+
+  // psi = A0 * conversion_factor (src/MultimodePropagation.cpp:59-63)
+  P.psi0.resize(2 * (size_t)M * nt);
+  for (size_t j = 0; j < 2 * (size_t)M * nt; j++) P.psi0[j] = in->initial_fields[j] * P.conversion_factor;
+
+  // Im L_p(alpha_j) (src/MultimodePropagation.cpp:165-187), stored in four-step order.
+  P.imL_fourstep.resize((size_t)M * nt);
+  const std::complex<double> iu(0., 1.);
+  for (uint32_t p = 0; p < M; p++) {
+    const double* bt = &P.beta_tilde[(size_t)p * nb];
+    for (uint32_t j = 0; j < nt; j++) {
+      const int kk = (j <= nt / 2) ? (int)j : (int)j - (int)nt;
+      const double alpha = (2 * M_PI / P.t_final) * kk;
+      std::complex<double> L = iu * (bt[0] - bt[1] * alpha - bt[2] / 2 * (alpha * alpha));
+      for (uint32_t o = 3; o < nb; o++) L += iu * (std::pow(-1, o) * bt[o] / factorial_int((int)o) * std::pow(alpha, o));
+      // natural index j = k1 + N1*k2  ->  four-step slot k1*N2 + k2
+      const uint32_t k1 = j % P.N1, k2 = j / P.N1;
+      P.imL_fourstep[(size_t)p * nt + (size_t)k1 * P.N2 + k2] = L.imag();
+    }
+  }
+
+  if (P.raman) P.raman_response.assign(in->raman_response, in->raman_response + nt);
+
+  // unique pair codes in first-occurrence order (src/MultimodePropagation.cpp:139-155)
+  P.slot.resize(nnz);
+  for (uint32_t e = 0; e < nnz; e++) {
+    const uint32_t lm = in->coupling_klm[3 * e + 1] + in->coupling_klm[3 * e + 2] * M;
+    uint32_t j = 0;
+    while (j < P.pair_codes.size() && P.pair_codes[j] != lm) j++;
+    if (j == P.pair_codes.size()) P.pair_codes.push_back(lm);
+    P.slot[e] = j;  // src/MultimodePropagation_CPU.cpp:149-150 lookup
+  }
+  P.P = (uint32_t)P.pair_codes.size();
+  // decoded pair of each slot: REFERENCE_CPU decodes the slot counter
+  // (src/MultimodePropagation_CPU.cpp:128-129, 198-199), PHYSICAL the stored code.
+  P.slot_a.resize(P.P);
+  P.slot_b.resize(P.P);
+  for (uint32_t j = 0; j < P.P; j++) {
+    const uint32_t code = (opt.pair_decode == MMP_PAIRS_PHYSICAL) ? P.pair_codes[j] : j;
+    P.slot_a[j] = code % M;
+    P.slot_b[j] = code / M;
+  }
+  // Raman convolution lines: one per unordered cross pair (the reversed pair is its
+  // conjugate since h_R is real) and one per two self pairs (|U_a|^2 + i |U_a'|^2).
+  P.slot_line.assign(P.P, 0);
+  P.slot_kind.assign(P.P, 0);
+  if (P.raman) {
+    std::vector<int> cross_line((size_t)M * M, -1);
+    std::vector<uint32_t> self_modes;
+    std::vector<int> self_pos(M, -1);
+    for (uint32_t j = 0; j < P.P; j++) {
+      const uint32_t a = P.slot_a[j], b = P.slot_b[j];
+      if (a == b) {
+        if (self_pos[a] < 0) {
+          self_pos[a] = (int)self_modes.size();
+          self_modes.push_back(a);
+        }
+      } else {
+        const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+        if (cross_line[lo * M + hi] < 0) {
+          cross_line[lo * M + hi] = (int)P.line_a.size();
+          P.line_a.push_back(lo);
+          P.line_b.push_back(hi);
+          P.line_self.push_back(0);
+        }
+        P.slot_line[j] = (uint32_t)cross_line[lo * M + hi];
+        P.slot_kind[j] = (a == lo) ? 0u : 1u;  // direct / conjugate
+      }
+    }
+    const uint32_t first_self_line = (uint32_t)P.line_a.size();
+    for (size_t s = 0; s < self_modes.size(); s += 2) {
+      P.line_a.push_back(self_modes[s]);
+      P.line_b.push_back(s + 1 < self_modes.size() ? self_modes[s + 1] : 0xffffffffu);
+      P.line_self.push_back(1);
+    }
+    for (uint32_t j = 0; j < P.P; j++) {
+      const uint32_t a = P.slot_a[j];
+      if (a != P.slot_b[j]) continue;
+      const int pos = self_pos[a];
+      P.slot_line[j] = first_self_line + (uint32_t)pos / 2;
+      P.slot_kind[j] = (pos % 2 == 0) ? 2u : 3u;  // real / imaginary part
+    }
+  }
+  // Kerr gather coefficients: W = V*(Q(1-fR)) + VR*(Q fR T0) with Raman on
+  // (src/MultimodePropagation_CPU.cpp:223-225), W = V*Q with Raman off (:152-153).
+  P.off.assign(in->coupling_offsets, in->coupling_offsets + M + 1);
+  P.k.resize(nnz);
+  P.c1.resize(nnz);
+  P.c2.resize(nnz);
+  for (uint32_t e = 0; e < nnz; e++) {
+    P.k[e] = in->coupling_klm[3 * e];
+    const double Q = in->coupling_values[e] / max_q;
+    if (P.raman) {
+      P.c1[e] = Q * (1 - P.fR);
+      P.c2[e] = Q * P.fR * P.T0;
+    } else {
+      P.c1[e] = Q;
+      P.c2[e] = 0.;
+    }
+  }
+  // a stage result N_j needs a stash slot when a stage i > j + 1 reads it
+  P.stash_needed.assign(P.rk.s, 0);
+  for (int i = 0; i < P.rk.s; i++)
+    for (int j = 0; j + 1 < i; j++)
+      if (P.rk.a[i][j] != 0.) P.stash_needed[j] = 1;
+  return MMP_OK;
+}
+
+}  // namespace mmp

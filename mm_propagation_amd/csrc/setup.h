@@ -1,0 +1,61 @@
+// setup.h -- host-side FP64 preparation of one propagation problem: the
+// reference constructor's dimensionless scaling, Butcher tableau, linear
+// operator and pair table, plus the tables the device kernels consume.
+#pragma once
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/mmprop.h"
+
+namespace mmp {
+
+struct Tableau {
+  int s = 0;
+  double a[11][11] = {};
+  double b[11] = {};
+  double c[11] = {};
+};
+
+// setRungeKuttaCoefficients (src/MultimodePropagation.cpp:15-27) with the
+// tableaux of src/RungeKutta.cpp:3-72,125-175.
+Tableau make_tableau(uint32_t method_order);
+
+struct Problem {
+  // sizes
+  uint32_t M = 0, N = 0, N1 = 0, N2 = 0, n_beta = 0, nnz = 0, P = 0;
+  Tableau rk;
+  // dimensionless scalars (src/MultimodePropagation.cpp:31-56)
+  double T0 = 0, sign_gamma = 1, max_beta2 = 0, max_q = 0, ld = 0;
+  double t_final = 0, delta_t = 0, z_final = 0, delta_z = 0, conversion_factor = 0;
+  uint32_t n_steps = 0, default_steps = 0;
+  int raman = 0;
+  double fR = 0;
+  std::vector<double> beta_tilde;   // [M][n_beta]
+  std::vector<double> psi0;         // [M][N] complex, conv * A0 (time domain)
+  std::vector<double> imL_fourstep; // [M][N] Im L_p(k), at S[k1*N2+k2] <-> k = k1 + N1*k2
+  std::vector<double> raman_response;  // [N]
+  // pair slots (src/MultimodePropagation.cpp:139-155)
+  std::vector<uint32_t> pair_codes;  // [P] first-occurrence order of l + m*M
+  std::vector<uint32_t> slot_a, slot_b, slot_line, slot_kind;  // [P]
+  // Raman convolution lines
+  std::vector<uint32_t> line_a, line_b, line_self;  // [n_lines]
+  // Kerr gather (CSR by p, insertion order)
+  std::vector<uint32_t> off, k, slot;  // [M+1], [nnz], [nnz]
+  std::vector<double> c1, c2;          // [nnz]
+  // which stage results must be kept for later stages
+  std::vector<int> stash_needed;       // [s]
+};
+
+// Validates and prepares. On failure returns a status and fills err.
+mmp_status prepare_problem(const mmp_params* in, const mmp_options& opt, Problem& out, std::string& err);
+
+// Four-step split NT = N1 * N2 with N1 <= N2 <= 2*N1; false if NT is unsupported.
+bool split_length(uint32_t nt, uint32_t& n1, uint32_t& n2);
+
+// exp(-2*pi*i*j/n), j < n, evaluated in long double and rounded once.
+std::vector<double> twiddle_table(uint32_t n);
+
+}  // namespace mmp

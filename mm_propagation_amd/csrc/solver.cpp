@@ -1,0 +1,651 @@
+// solver.cpp -- the C ABI of include/mmprop.h: contexts, solver setup, device
+// memory, step scheduling (eager or HIP-graph replay) and results.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/mmprop.h"
+#include "kernels.h"
+#include "setup.h"
+
+using namespace mmp;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+mmp_status fail(mmp_status st, const std::string& msg) {
+  g_last_error = msg;
+  return st;
+}
+
+mmp_status hip_fail(hipError_t e, const char* what) {
+  g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return MMP_ERR_OUT_OF_MEMORY;
+  if (e == hipErrorNoBinaryForGpu || e == hipErrorNoDevice || e == hipErrorInvalidDevice) return MMP_ERR_NO_DEVICE;
+  return MMP_ERR_DEVICE;
+}
+
+#define HIP_TRY(expr, what)                        \
+  do {                                             \
+    hipError_t e_ = (expr);                        \
+    if (e_ != hipSuccess) return hip_fail(e_, what); \
+  } while (0)
+
+#define MMP_TRY(expr)                 \
+  do {                                \
+    mmp_status s_ = (expr);           \
+    if (s_ != MMP_OK) return s_;      \
+  } while (0)
+
+constexpr int kGraphSteps = 16;  // steps captured in one replayable HIP graph
+
+}  // namespace
+
+struct mmp_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+};
+
+struct KernelRecord {
+  int name;
+  double bytes;
+  hipEvent_t start, stop;
+};
+
+struct mmp_solver {
+  mmp_ctx* ctx = nullptr;
+  mmp_options opt{};
+  Problem P;
+  double time_window = 0;
+  Geometry g{};
+  Twiddles tw{};
+  std::vector<void*> allocations;
+  size_t device_bytes = 0;
+  cd* psi_hat = nullptr;
+  cd* acc = nullptr;
+  cd* ybuf = nullptr;
+  cd* U = nullptr;
+  cd* conv = nullptr;
+  cd* stash = nullptr;
+  cd* hspec = nullptr;
+  double* imL = nullptr;
+  double* hR = nullptr;
+  double* energy = nullptr;
+  StepState* st = nullptr;
+  int n_lines = 0;
+  std::vector<HeadArgs> heads;  // s + 1 entries
+  GatherArgs gather{};
+  PairLineArgs pairs{};
+  double z_host = 0;
+  uint32_t steps_taken = 0;
+  bool graphs = true;
+  hipGraphExec_t graph_exec = nullptr;
+  hipGraph_t graph = nullptr;
+  // profiling
+  bool profile = false;
+  std::vector<std::string> kernel_names;
+  std::vector<double> kernel_bytes;
+  std::vector<double> kernel_ms;
+  std::vector<uint64_t> kernel_count;
+  std::vector<KernelRecord> pending;
+  std::vector<hipEvent_t> free_events;
+};
+
+namespace {
+
+template <class T>
+mmp_status dalloc(mmp_solver* s, T** out, size_t count) {
+  void* p = nullptr;
+  const size_t bytes = sizeof(T) * (count ? count : 1);
+  HIP_TRY(hipMalloc(&p, bytes), "hipMalloc");
+  s->allocations.push_back(p);
+  s->device_bytes += bytes;
+  *out = static_cast<T*>(p);
+  return MMP_OK;
+}
+
+template <class T>
+mmp_status upload(mmp_solver* s, T* dst, const void* src, size_t bytes) {
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->ctx->stream), "hipMemcpyAsync H2D");
+  return MMP_OK;
+}
+
+int kernel_id(mmp_solver* s, const char* name) {
+  for (size_t i = 0; i < s->kernel_names.size(); i++)
+    if (s->kernel_names[i] == name) return (int)i;
+  s->kernel_names.push_back(name);
+  s->kernel_bytes.push_back(0.);
+  s->kernel_ms.push_back(0.);
+  s->kernel_count.push_back(0);
+  return (int)s->kernel_names.size() - 1;
+}
+
+hipEvent_t take_event(mmp_solver* s) {
+  if (!s->free_events.empty()) {
+    hipEvent_t e = s->free_events.back();
+    s->free_events.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+mmp_status drain_profile(mmp_solver* s) {
+  if (s->pending.empty()) return MMP_OK;
+  HIP_TRY(hipStreamSynchronize(s->ctx->stream), "hipStreamSynchronize");
+  for (auto& r : s->pending) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, r.start, r.stop), "hipEventElapsedTime");
+    s->kernel_ms[r.name] += ms;
+    s->kernel_count[r.name] += 1;
+    s->kernel_bytes[r.name] += r.bytes;
+    s->free_events.push_back(r.start);
+    s->free_events.push_back(r.stop);
+  }
+  s->pending.clear();
+  return MMP_OK;
+}
+
+// Launch `fn` on the solver stream, bracketed by HIP events when profiling.
+template <class F>
+mmp_status run_kernel(mmp_solver* s, const char* name, double bytes, F&& fn) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (s->profile) {
+    e0 = take_event(s);
+    e1 = take_event(s);
+    if (!e0 || !e1) return fail(MMP_ERR_DEVICE, "hipEventCreate failed");
+    HIP_TRY(hipEventRecord(e0, s->ctx->stream), "hipEventRecord");
+  }
+  hipError_t e = fn();
+  if (e != hipSuccess) return hip_fail(e, name);
+  if (s->profile) {
+    HIP_TRY(hipEventRecord(e1, s->ctx->stream), "hipEventRecord");
+    s->pending.push_back({kernel_id(s, name), bytes, e0, e1});
+    if (s->pending.size() >= 2048) MMP_TRY(drain_profile(s));
+  }
+  return MMP_OK;
+}
+
+mmp_status enqueue_step(mmp_solver* s) {
+  const Problem& P = s->P;
+  const Geometry& g = s->g;
+  hipStream_t st = s->ctx->stream;
+  const double cb = 16.0;  // bytes per complex FP64
+  const double MN = (double)P.M * P.N;
+  const double LN = (double)s->n_lines * P.N;
+  const double invN = 1.0 / P.N;
+  for (int i = 0; i <= P.rk.s; i++) {
+    const HeadArgs& h = s->heads[i];
+    const double hb = cb * MN * ((h.has_prev ? 1 : 0) + 1 + (h.has_prev && !h.prev_is_first ? 1 : 0) +
+                                 (h.final_stage ? 0 : 1) + (h.final_stage ? 0 : h.n_a)) +
+                      8.0 * MN * ((h.has_prev ? 1 : 0) + (h.final_stage ? 0 : 1));
+    MMP_TRY(run_kernel(s, i == P.rk.s ? "row_close_step" : "row_stage_head", hb,
+                       [&] { return launch_row_head(g, s->tw, h, st); }));
+    if (i == P.rk.s) break;
+    MMP_TRY(run_kernel(s, "col_inverse_fields", 2 * cb * MN,
+                       [&] { return launch_col_inverse(g, s->tw, s->ybuf, s->U, P.M, invN, 1.0, st); }));
+    if (P.raman) {
+      MMP_TRY(run_kernel(s, "col_pairs_forward", cb * (MN + LN), [&] {
+        return launch_col_pairs_forward(g, s->tw, s->U, s->conv, s->n_lines, s->pairs, st);
+      }));
+      MMP_TRY(run_kernel(s, "row_raman", cb * (2 * LN + P.N),
+                         [&] { return launch_row_raman(g, s->tw, s->conv, s->n_lines, s->hspec, P.delta_t, st); }));
+      MMP_TRY(run_kernel(s, "col_inverse_raman", 2 * cb * LN, [&] {
+        return launch_col_inverse(g, s->tw, s->conv, s->conv, s->n_lines, invN, 1.0, st);
+      }));
+    }
+    MMP_TRY(run_kernel(s, "gather_kerr_raman", cb * (2 * MN + LN),
+                       [&] { return launch_gather(g, s->U, s->conv, s->ybuf, s->gather, st); }));
+    MMP_TRY(run_kernel(s, "col_forward_sigma", 2 * cb * MN,
+                       [&] { return launch_col_forward(g, s->tw, s->ybuf, s->ybuf, P.M, st); }));
+  }
+  MMP_TRY(run_kernel(s, "advance_z", 0.0, [&] { return launch_advance_z(s->st, P.delta_z, st); }));
+  return MMP_OK;
+}
+
+mmp_status build_graph(mmp_solver* s) {
+  hipStream_t st = s->ctx->stream;
+  HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  mmp_status status = MMP_OK;
+  for (int k = 0; k < kGraphSteps && status == MMP_OK; k++) status = enqueue_step(s);
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(st, &graph);
+  if (status != MMP_OK) {
+    if (graph) hipGraphDestroy(graph);
+    return status;
+  }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    hipGraphDestroy(graph);
+    return hip_fail(e, "hipGraphInstantiate");
+  }
+  s->graph = graph;
+  s->graph_exec = exec;
+  return MMP_OK;
+}
+
+mmp_status set_device(mmp_solver* s) {
+  HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
+  return MMP_OK;
+}
+
+mmp_status reset_z(mmp_solver* s) {
+  static const StepState zero{0., 0u, 0u};
+  MMP_TRY(upload(s, s->st, &zero, sizeof zero));
+  s->z_host = 0.;
+  s->steps_taken = 0;
+  return MMP_OK;
+}
+
+// psi_hat = F(psi) for a time-domain field already in s->U
+mmp_status forward_state_from_U(mmp_solver* s) {
+  hipStream_t st = s->ctx->stream;
+  HIP_TRY(launch_col_forward(s->g, s->tw, s->U, s->psi_hat, s->P.M, st), "col_forward");
+  HIP_TRY(launch_row_forward(s->g, s->tw, s->psi_hat, s->P.M, st), "row_forward");
+  return MMP_OK;
+}
+
+// time-domain field of psi_hat into s->U: with_phase -> exp(L z) applied and divided by conv
+mmp_status inverse_state_to_U(mmp_solver* s, bool with_phase) {
+  hipStream_t st = s->ctx->stream;
+  HIP_TRY(launch_row_inverse_phase(s->g, s->tw, s->psi_hat, s->ybuf, s->imL, s->st, with_phase ? 1 : 0, st),
+          "row_inverse_phase");
+  HIP_TRY(launch_col_inverse(s->g, s->tw, s->ybuf, s->U, s->P.M, 1.0 / s->P.N,
+                             with_phase ? s->P.conversion_factor : 1.0, st),
+          "col_inverse");
+  return MMP_OK;
+}
+
+void release(mmp_solver* s) {
+  if (!s) return;
+  if (s->ctx) hipSetDevice(s->ctx->device);
+  if (s->ctx && s->ctx->stream) hipStreamSynchronize(s->ctx->stream);
+  for (auto& r : s->pending) {
+    hipEventDestroy(r.start);
+    hipEventDestroy(r.stop);
+  }
+  for (auto e : s->free_events) hipEventDestroy(e);
+  if (s->graph_exec) hipGraphExecDestroy(s->graph_exec);
+  if (s->graph) hipGraphDestroy(s->graph);
+  for (void* p : s->allocations) hipFree(p);
+  delete s;
+}
+
+mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options* options, mmp_solver** out) {
+  if (!ctx || !out) return fail(MMP_ERR_INVALID_ARGUMENT, "ctx and out are required");
+  *out = nullptr;
+  mmp_options opt{};
+  if (options) opt = *options;
+  mmp_solver* s = new (std::nothrow) mmp_solver();
+  if (!s) return fail(MMP_ERR_OUT_OF_MEMORY, "host allocation failed");
+  s->ctx = ctx;
+  s->opt = opt;
+  std::string err;
+  mmp_status stt = prepare_problem(params, opt, s->P, err);
+  if (stt != MMP_OK) {
+    delete s;
+    return fail(stt, err);
+  }
+  s->time_window = params->time_window;
+  s->profile = (opt.flags & MMP_FLAG_PROFILE) != 0;
+  s->graphs = !s->profile && (opt.flags & MMP_FLAG_NO_GRAPH) == 0;
+  Problem& P = s->P;
+  s->g = Geometry{(int)P.N, (int)P.N1, (int)P.N2, (int)P.M};
+  s->n_lines = (int)P.line_a.size();
+  auto cleanup = [&](mmp_status st2) {
+    release(s);
+    return st2;
+  };
+  if (set_device(s) != MMP_OK) return cleanup(MMP_ERR_DEVICE);
+  const size_t MN = (size_t)P.M * P.N;
+  // device buffers
+  cd *rowtw, *coltw, *fourtw;
+  mmp_status st = MMP_OK;
+  auto A = [&](mmp_status x) {
+    if (st == MMP_OK) st = x;
+  };
+  A(dalloc(s, &s->psi_hat, MN));
+  A(dalloc(s, &s->acc, MN));
+  A(dalloc(s, &s->ybuf, MN));
+  A(dalloc(s, &s->U, MN));
+  A(dalloc(s, &s->imL, MN));
+  A(dalloc(s, &rowtw, P.N2));
+  A(dalloc(s, &coltw, P.N1));
+  A(dalloc(s, &fourtw, P.N));
+  A(dalloc(s, &s->st, 1));
+  A(dalloc(s, &s->energy, P.M));
+  bool any_stash = false;
+  for (int j = 0; j < P.rk.s; j++) any_stash = any_stash || P.stash_needed[j];
+  if (any_stash) A(dalloc(s, &s->stash, (size_t)P.rk.s * MN));
+  if (P.raman) {
+    A(dalloc(s, &s->conv, (size_t)(s->n_lines ? s->n_lines : 1) * P.N));
+    A(dalloc(s, &s->hspec, P.N));
+    A(dalloc(s, &s->hR, P.N));
+  }
+  uint32_t *d_off, *d_k, *d_slot, *d_sa, *d_sb, *d_sl, *d_sk, *d_la, *d_lb, *d_ls;
+  double *d_c1, *d_c2;
+  const size_t nnz1 = P.nnz ? P.nnz : 1, P1 = P.P ? P.P : 1, L1 = s->n_lines ? s->n_lines : 1;
+  A(dalloc(s, &d_off, P.M + 1));
+  A(dalloc(s, &d_k, nnz1));
+  A(dalloc(s, &d_slot, nnz1));
+  A(dalloc(s, &d_c1, nnz1));
+  A(dalloc(s, &d_c2, nnz1));
+  A(dalloc(s, &d_sa, P1));
+  A(dalloc(s, &d_sb, P1));
+  A(dalloc(s, &d_sl, P1));
+  A(dalloc(s, &d_sk, P1));
+  A(dalloc(s, &d_la, L1));
+  A(dalloc(s, &d_lb, L1));
+  A(dalloc(s, &d_ls, L1));
+  if (st != MMP_OK) return cleanup(st);
+  // tables
+  std::vector<double> tr = twiddle_table(P.N2), tc = twiddle_table(P.N1), t4 = twiddle_table(P.N);
+  A(upload(s, rowtw, tr.data(), tr.size() * 8));
+  A(upload(s, coltw, tc.data(), tc.size() * 8));
+  A(upload(s, fourtw, t4.data(), t4.size() * 8));
+  A(upload(s, s->imL, P.imL_fourstep.data(), MN * 8));
+  A(upload(s, d_off, P.off.data(), (P.M + 1) * 4));
+  if (P.nnz) {
+    A(upload(s, d_k, P.k.data(), P.nnz * 4));
+    A(upload(s, d_slot, P.slot.data(), P.nnz * 4));
+    A(upload(s, d_c1, P.c1.data(), P.nnz * 8));
+    A(upload(s, d_c2, P.c2.data(), P.nnz * 8));
+  }
+  if (P.P) {
+    A(upload(s, d_sa, P.slot_a.data(), P.P * 4));
+    A(upload(s, d_sb, P.slot_b.data(), P.P * 4));
+    A(upload(s, d_sl, P.slot_line.data(), P.P * 4));
+    A(upload(s, d_sk, P.slot_kind.data(), P.P * 4));
+  }
+  if (s->n_lines) {
+    A(upload(s, d_la, P.line_a.data(), s->n_lines * 4));
+    A(upload(s, d_lb, P.line_b.data(), s->n_lines * 4));
+    A(upload(s, d_ls, P.line_self.data(), s->n_lines * 4));
+  }
+  if (st != MMP_OK) return cleanup(st);
+  s->tw = Twiddles{rowtw, coltw, fourtw};
+  s->gather = GatherArgs{d_off, d_k, d_slot, d_c1, d_c2, d_sa, d_sb, d_sl, d_sk, P.raman};
+  s->pairs = PairLineArgs{d_la, d_lb, d_ls};
+  // stage heads
+  const double h = P.delta_z;
+  for (int i = 0; i <= P.rk.s; i++) {
+    HeadArgs a{};
+    a.ysig = s->ybuf;
+    a.psi_hat = s->psi_hat;
+    a.acc = s->acc;
+    a.stash = s->stash;
+    a.yout = s->ybuf;
+    a.imL = s->imL;
+    a.st = s->st;
+    a.sgn = P.sign_gamma;
+    a.has_prev = i > 0;
+    a.prev_stage = i - 1;
+    a.prev_is_first = (i == 1);
+    a.final_stage = (i == P.rk.s);
+    if (i > 0) {
+      a.ch_prev = P.rk.c[i - 1] * h;
+      a.bh_prev = (P.rk.b[i - 1] != 0.) ? P.rk.b[i - 1] * h : 0.;
+      a.stash_prev = P.stash_needed[i - 1];
+    }
+    if (i < P.rk.s) {
+      a.ch_cur = P.rk.c[i] * h;
+      for (int j = 0; j < i; j++)
+        if (P.rk.a[i][j] != 0.) {
+          a.a_idx[a.n_a] = j;
+          a.a_h[a.n_a] = P.rk.a[i][j] * h;
+          a.n_a++;
+        }
+    }
+    s->heads.push_back(a);
+  }
+  // initial state and Raman spectrum
+  A(upload(s, s->U, P.psi0.data(), MN * 16));
+  A(forward_state_from_U(s));
+  if (P.raman) {
+    A(upload(s, s->hR, P.raman_response.data(), P.N * 8));
+    hipError_t e = launch_col_forward_real(s->g, s->tw, s->hR, s->hspec, ctx->stream);
+    if (e != hipSuccess) A(hip_fail(e, "col_forward_real"));
+    e = launch_row_forward(s->g, s->tw, s->hspec, 1, ctx->stream);
+    if (e != hipSuccess) A(hip_fail(e, "row_forward"));
+  }
+  A(reset_z(s));
+  if (st == MMP_OK) {
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) A(hip_fail(e, "hipStreamSynchronize"));
+  }
+  if (st != MMP_OK) return cleanup(st);
+  *out = s;
+  return MMP_OK;
+}
+
+mmp_status advance_impl(mmp_solver* s, uint32_t n) {
+  MMP_TRY(set_device(s));
+  uint32_t left = n;
+  if (s->graphs && left >= (uint32_t)kGraphSteps) {
+    if (!s->graph_exec) {
+      mmp_status bs = build_graph(s);
+      if (bs != MMP_OK) {
+        // graphs unavailable: keep launching step by step
+        s->graphs = false;
+        hipGetLastError();
+      }
+    }
+    while (s->graphs && left >= (uint32_t)kGraphSteps) {
+      HIP_TRY(hipGraphLaunch(s->graph_exec, s->ctx->stream), "hipGraphLaunch");
+      left -= kGraphSteps;
+    }
+  }
+  for (uint32_t k = 0; k < left; k++) MMP_TRY(enqueue_step(s));
+  for (uint32_t k = 0; k < n; k++) s->z_host += s->P.delta_z;  // z += h (src/MultimodePropagation_CPU.cpp:32)
+  s->steps_taken += n;
+  return MMP_OK;
+}
+
+mmp_status read_fields(mmp_solver* s, double* out) {
+  HIP_TRY(hipMemcpyAsync(out, s->U, sizeof(cd) * s->P.M * s->P.N, hipMemcpyDeviceToHost, s->ctx->stream),
+          "hipMemcpyAsync D2H");
+  HIP_TRY(hipStreamSynchronize(s->ctx->stream), "hipStreamSynchronize");
+  return MMP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mmp_status_string(mmp_status status) {
+  switch (status) {
+    case MMP_OK: return "MMP_OK";
+    case MMP_ERR_INVALID_ARGUMENT: return "MMP_ERR_INVALID_ARGUMENT";
+    case MMP_ERR_UNSUPPORTED: return "MMP_ERR_UNSUPPORTED";
+    case MMP_ERR_OUT_OF_MEMORY: return "MMP_ERR_OUT_OF_MEMORY";
+    case MMP_ERR_DEVICE: return "MMP_ERR_DEVICE";
+    case MMP_ERR_NO_DEVICE: return "MMP_ERR_NO_DEVICE";
+    case MMP_ERR_NONFINITE: return "MMP_ERR_NONFINITE";
+    case MMP_ERR_STATE: return "MMP_ERR_STATE";
+  }
+  return "MMP_ERR_UNKNOWN";
+}
+
+const char* mmp_last_error(void) { return g_last_error.c_str(); }
+
+int mmp_abi_version(void) { return MMPROP_ABI_VERSION; }
+
+mmp_status mmp_ctx_create(int device, mmp_ctx** out) {
+  if (!out) return fail(MMP_ERR_INVALID_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) return fail(MMP_ERR_NO_DEVICE, "no HIP device available");
+  if (device < 0 || device >= count) return fail(MMP_ERR_INVALID_ARGUMENT, "device index out of range");
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(MMP_ERR_NO_DEVICE, std::string("kernels are built for gfx950, device is ") + prop.gcnArchName);
+  HIP_TRY(hipSetDevice(device), "hipSetDevice");
+  mmp_ctx* c = new (std::nothrow) mmp_ctx();
+  if (!c) return fail(MMP_ERR_OUT_OF_MEMORY, "host allocation failed");
+  c->device = device;
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return hip_fail(e, "hipStreamCreate");
+  }
+  *out = c;
+  return MMP_OK;
+}
+
+void mmp_ctx_destroy(mmp_ctx* ctx) {
+  if (!ctx) return;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) {
+    hipStreamSynchronize(ctx->stream);
+    hipStreamDestroy(ctx->stream);
+  }
+  delete ctx;
+}
+
+mmp_status mmp_ctx_synchronize(mmp_ctx* ctx) {
+  if (!ctx) return fail(MMP_ERR_INVALID_ARGUMENT, "ctx is NULL");
+  HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  return MMP_OK;
+}
+
+mmp_status mmp_solver_create(mmp_ctx* ctx, const mmp_params* params, const mmp_options* options,
+                             mmp_solver** out) {
+  try {
+    return create_impl(ctx, params, options, out);
+  } catch (const std::bad_alloc&) {
+    return fail(MMP_ERR_OUT_OF_MEMORY, "host allocation failed");
+  } catch (...) {
+    return fail(MMP_ERR_DEVICE, "unexpected exception");
+  }
+}
+
+void mmp_solver_destroy(mmp_solver* s) { release(s); }
+
+mmp_status mmp_compute_lawson_rk(mmp_solver* s, uint32_t nz) {
+  if (!s) return fail(MMP_ERR_INVALID_ARGUMENT, "solver is NULL");
+  MMP_TRY(set_device(s));
+  MMP_TRY(reset_z(s));  // initLawsonRK: z = 0, operators rebuilt (src/MultimodePropagation.cpp:158-163)
+  return advance_impl(s, nz);
+}
+
+mmp_status mmp_compute_lawson_rk_default(mmp_solver* s) {
+  if (!s) return fail(MMP_ERR_INVALID_ARGUMENT, "solver is NULL");
+  return mmp_compute_lawson_rk(s, s->P.default_steps);
+}
+
+mmp_status mmp_advance(mmp_solver* s, uint32_t n) {
+  if (!s) return fail(MMP_ERR_INVALID_ARGUMENT, "solver is NULL");
+  return advance_impl(s, n);
+}
+
+mmp_status mmp_get_result(mmp_solver* s, double* out) {
+  if (!s || !out) return fail(MMP_ERR_INVALID_ARGUMENT, "solver and fields_out are required");
+  MMP_TRY(set_device(s));
+  MMP_TRY(inverse_state_to_U(s, true));
+  MMP_TRY(read_fields(s, out));
+  const size_t n = 2 * (size_t)s->P.M * s->P.N;
+  for (size_t i = 0; i < n; i++)
+    if (!std::isfinite(out[i])) return fail(MMP_ERR_NONFINITE, "the propagated field is not finite");
+  return MMP_OK;
+}
+
+mmp_status mmp_get_state(mmp_solver* s, double* psi_out, double* z_out) {
+  if (!s) return fail(MMP_ERR_INVALID_ARGUMENT, "solver is NULL");
+  MMP_TRY(set_device(s));
+  if (z_out) *z_out = s->z_host;
+  if (psi_out) {
+    MMP_TRY(inverse_state_to_U(s, false));
+    MMP_TRY(read_fields(s, psi_out));
+  }
+  return MMP_OK;
+}
+
+mmp_status mmp_set_state(mmp_solver* s, const double* psi) {
+  if (!s || !psi) return fail(MMP_ERR_INVALID_ARGUMENT, "solver and psi are required");
+  const size_t n = 2 * (size_t)s->P.M * s->P.N;
+  for (size_t i = 0; i < n; i++)
+    if (!std::isfinite(psi[i])) return fail(MMP_ERR_NONFINITE, "psi is not finite");
+  MMP_TRY(set_device(s));
+  MMP_TRY(upload(s, s->U, psi, n * 8));
+  MMP_TRY(forward_state_from_U(s));
+  MMP_TRY(reset_z(s));
+  HIP_TRY(hipStreamSynchronize(s->ctx->stream), "hipStreamSynchronize");
+  return MMP_OK;
+}
+
+mmp_status mmp_energy(mmp_solver* s, double* energy_out) {
+  if (!s || !energy_out) return fail(MMP_ERR_INVALID_ARGUMENT, "solver and energy_out are required");
+  MMP_TRY(set_device(s));
+  MMP_TRY(inverse_state_to_U(s, true));
+  HIP_TRY(launch_energy(s->g, s->U, s->energy, s->time_window / s->P.N, s->ctx->stream), "energy");
+  HIP_TRY(hipMemcpyAsync(energy_out, s->energy, sizeof(double) * s->P.M, hipMemcpyDeviceToHost, s->ctx->stream),
+          "hipMemcpyAsync D2H");
+  HIP_TRY(hipStreamSynchronize(s->ctx->stream), "hipStreamSynchronize");
+  return MMP_OK;
+}
+
+mmp_status mmp_solver_info(mmp_solver* s, mmp_info* o) {
+  if (!s || !o) return fail(MMP_ERR_INVALID_ARGUMENT, "solver and info_out are required");
+  const Problem& P = s->P;
+  std::memset(o, 0, sizeof *o);
+  o->n_modes = P.M;
+  o->nt = P.N;
+  o->stages = (uint32_t)P.rk.s;
+  o->n_pairs = P.P;
+  o->nnz = P.nnz;
+  o->n_conv_lines = (uint32_t)s->n_lines;
+  o->n1 = P.N1;
+  o->n2 = P.N2;
+  o->steps_taken = s->steps_taken;
+  o->default_steps = P.default_steps;
+  o->t_final = P.t_final;
+  o->delta_t = P.delta_t;
+  o->z_final = P.z_final;
+  o->delta_z = P.delta_z;
+  o->z = s->z_host;
+  o->conversion_factor = P.conversion_factor;
+  o->sign_gamma = P.sign_gamma;
+  o->ld = P.ld;
+  o->max_beta2 = P.max_beta2;
+  o->max_q = P.max_q;
+  o->device_bytes = s->device_bytes;
+  return MMP_OK;
+}
+
+mmp_status mmp_kernel_times(mmp_solver* s, mmp_kernel_time* out, uint32_t capacity, uint32_t* count_out) {
+  if (!s) return fail(MMP_ERR_INVALID_ARGUMENT, "solver is NULL");
+  MMP_TRY(set_device(s));
+  MMP_TRY(drain_profile(s));
+  const uint32_t n = (uint32_t)s->kernel_names.size();
+  if (count_out) *count_out = n;
+  for (uint32_t i = 0; i < n && i < capacity && out; i++) {
+    std::memset(&out[i], 0, sizeof out[i]);
+    std::snprintf(out[i].name, sizeof out[i].name, "%s", s->kernel_names[i].c_str());
+    out[i].total_ms = s->kernel_ms[i];
+    out[i].launches = s->kernel_count[i];
+    out[i].bytes_per_launch = s->kernel_count[i] ? s->kernel_bytes[i] / (double)s->kernel_count[i] : 0.;
+  }
+  if (out) {
+    for (uint32_t i = 0; i < n; i++) {
+      s->kernel_ms[i] = 0.;
+      s->kernel_count[i] = 0;
+      s->kernel_bytes[i] = 0.;
+    }
+  }
+  return MMP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,699 @@
+/*
+ * oracle.c -- FP64 CPU restatement of MM_Propagation's Lawson-Runge-Kutta CPU
+ * path. TEST INFRASTRUCTURE ONLY (the checker): only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ *
+ * Every function cites the reference lines it restates. Arithmetic follows the
+ * reference's std::complex<double> operation order with FMA contraction off
+ * (built with -ffp-contract=off), so the setup products (L, E, psi0, scalars,
+ * pair table) are bit-identical to the reference's; the FFT is a textbook
+ * radix-2 transform (the reference uses FFTW, absent from this image), so
+ * stepped fields agree with the reference to FFT rounding only.
+ *
+ * Pinning (which check fixes which function) is listed in DESIGN.md; every
+ * function here is covered by at least one check under tests/.
+ */
+#define _GNU_SOURCE
+#include "oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define MAX_STAGES 11
+
+static __thread char g_err[256];
+static void set_err(const char* msg) { snprintf(g_err, sizeof g_err, "%s", msg); }
+const char* orc_last_error(void) { return g_err; }
+
+/* ------------------------------------------------------------------------ */
+/* Complex helpers with std::complex<double> semantics for finite values.    */
+/* ------------------------------------------------------------------------ */
+typedef struct { double re, im; } cx;
+
+static inline cx cx_mul(cx a, cx b) { cx r = {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; return r; }
+static inline cx cx_scale(cx a, double d) { cx r = {a.re * d, a.im * d}; return r; }
+static inline cx cx_add(cx a, cx b) { cx r = {a.re + b.re, a.im + b.im}; return r; }
+static inline cx cx_conj(cx a) { cx r = {a.re, -a.im}; return r; }
+/* std::exp(std::complex<double>) -> glibc cexp (ComplexArray.cpp:227-236). */
+static inline cx cx_exp(cx a) {
+  double complex z = cexp(CMPLX(a.re, a.im));
+  cx r = {creal(z), cimag(z)};
+  return r;
+}
+
+/* ------------------------------------------------------------------------ */
+/* FFT: radix-2, FFTW sign/normalisation conventions                         */
+/* (src/DiscreteFourierTransform.cpp:12,22-38). Twiddles are cos/sin of      */
+/* 2*pi*k/n evaluated in long double and rounded once.                       */
+/* ------------------------------------------------------------------------ */
+typedef struct fft_plan {
+  unsigned n, logn;
+  int pow2;
+  cx* tw;          /* tw[k] = exp(-2*pi*i*k/n), k < n */
+  unsigned* rev;   /* bit reversal permutation */
+} fft_plan;
+
+static int fft_plan_init(fft_plan* p, unsigned n) {
+  memset(p, 0, sizeof *p);
+  p->n = n;
+  p->pow2 = (n >= 1) && ((n & (n - 1)) == 0);
+  p->tw = (cx*)malloc(sizeof(cx) * (n ? n : 1));
+  if (!p->tw) return 0;
+  for (unsigned k = 0; k < n; k++) {
+    long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n;
+    p->tw[k].re = (double)cosl(ang);
+    p->tw[k].im = (double)sinl(ang);
+  }
+  if (p->pow2) {
+    unsigned logn = 0;
+    while ((1u << logn) < n) logn++;
+    p->logn = logn;
+    p->rev = (unsigned*)malloc(sizeof(unsigned) * n);
+    if (!p->rev) return 0;
+    for (unsigned j = 0; j < n; j++) {
+      unsigned r = 0;
+      for (unsigned b = 0; b < logn; b++) r |= ((j >> b) & 1u) << (logn - 1 - b);
+      p->rev[j] = r;
+    }
+  }
+  return 1;
+}
+
+static void fft_plan_free(fft_plan* p) { free(p->tw); free(p->rev); memset(p, 0, sizeof *p); }
+
+static inline cx twiddle(const fft_plan* p, unsigned k, int sign) {
+  cx w = p->tw[k];
+  if (sign > 0) w.im = -w.im;
+  return w;
+}
+
+/* Unnormalised transform, out = sum_j in[j] exp(sign*2*pi*i*j*k/n); in may alias out. */
+static void fft_raw(const fft_plan* p, const cx* in, cx* out, int sign, int variant, cx* scratch) {
+  const unsigned n = p->n;
+  if (!p->pow2) {
+    /* O(n^2) DFT for sizes that are not powers of two. */
+    for (unsigned k = 0; k < n; k++) {
+      cx acc = {0.0, 0.0};
+      for (unsigned j = 0; j < n; j++) {
+        unsigned idx = (unsigned)(((unsigned long long)j * k) % n);
+        acc = cx_add(acc, cx_mul(in[j], twiddle(p, idx, sign)));
+      }
+      scratch[k] = acc;
+    }
+    memcpy(out, scratch, sizeof(cx) * n);
+    return;
+  }
+  if (variant == 0) {
+    /* Decimation in time: bit-reverse, then butterflies of growing span. */
+    if (in == out) {
+      memcpy(scratch, in, sizeof(cx) * n);
+      in = scratch;
+    }
+    for (unsigned j = 0; j < n; j++) out[p->rev[j]] = in[j];
+    for (unsigned half = 1; half < n; half <<= 1) {
+      unsigned stride = n / (2 * half);
+      for (unsigned base = 0; base < n; base += 2 * half) {
+        for (unsigned j = 0; j < half; j++) {
+          cx w = twiddle(p, j * stride, sign);
+          cx u = out[base + j];
+          cx v = cx_mul(out[base + j + half], w);
+          out[base + j] = cx_add(u, v);
+          out[base + j + half].re = u.re - v.re;
+          out[base + j + half].im = u.im - v.im;
+        }
+      }
+    }
+  } else {
+    /* Decimation in frequency: butterflies of shrinking span, then bit-reverse. */
+    memcpy(scratch, in, sizeof(cx) * n);
+    for (unsigned half = n >> 1; half >= 1; half >>= 1) {
+      unsigned stride = n / (2 * half);
+      for (unsigned base = 0; base < n; base += 2 * half) {
+        for (unsigned j = 0; j < half; j++) {
+          cx w = twiddle(p, j * stride, sign);
+          cx u = scratch[base + j];
+          cx v = scratch[base + j + half];
+          scratch[base + j] = cx_add(u, v);
+          cx d = {u.re - v.re, u.im - v.im};
+          scratch[base + j + half] = cx_mul(d, w);
+        }
+      }
+      if (half == 1) break;
+    }
+    for (unsigned j = 0; j < n; j++) out[p->rev[j]] = scratch[j];
+  }
+}
+
+/* Forward_DFT::compute (src/DiscreteFourierTransform.cpp:22-27). */
+static void fft_forward(const fft_plan* p, const cx* in, cx* out, int variant, cx* scratch) {
+  fft_raw(p, in, out, -1, variant, scratch);
+}
+
+/* Inverse_DFT::compute (src/DiscreteFourierTransform.cpp:30-39): backward transform,
+ * then out[i] *= complex(1/n, 0). */
+static void fft_inverse(const fft_plan* p, const cx* in, cx* out, int variant, cx* scratch) {
+  fft_raw(p, in, out, +1, variant, scratch);
+  const cx f = {1.0 / p->n, 0.0};
+  for (unsigned i = 0; i < p->n; i++) out[i] = cx_mul(out[i], f);
+}
+
+void orc_dft(unsigned n, const double* in, double* out, int sign, int variant) {
+  fft_plan p;
+  if (!fft_plan_init(&p, n)) return;
+  cx* scratch = (cx*)malloc(sizeof(cx) * n);
+  if (sign < 0) fft_forward(&p, (const cx*)in, (cx*)out, variant, scratch);
+  else fft_inverse(&p, (const cx*)in, (cx*)out, variant, scratch);
+  free(scratch);
+  fft_plan_free(&p);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Solver state                                                              */
+/* ------------------------------------------------------------------------ */
+struct orc_solver {
+  unsigned M, nt, s, n_beta, nnz, P;
+  double a[MAX_STAGES][MAX_STAGES], b[MAX_STAGES], c[MAX_STAGES];
+  double T0, sign_gamma, t_final, delta_t, z, z_final, delta_z, conv, LD, max_beta2, max_Q;
+  double* beta_tilde;      /* [M][n_beta] */
+  unsigned* q_off;         /* [M+1] */
+  unsigned* q_klm;         /* [nnz][3] */
+  double* q_val;           /* [nnz] (dimensionless) */
+  unsigned* unique_pairs;  /* [P] codes l + m*M in first-occurrence order */
+  unsigned* nnz_slot;      /* [nnz] index into unique_pairs */
+  unsigned* pair_a;        /* [P] */
+  unsigned* pair_b;        /* [P] */
+  cx* L;                   /* [M][nt] */
+  cx* psi;                 /* [M][nt] */
+  cx* E1;                  /* [M][nt] */
+  cx* E;                   /* [s][M][nt] */
+  int raman;
+  double fR;
+  cx* hR_fft;              /* [nt] */
+  fft_plan plan;
+  orc_options opt;
+  int nthreads;
+  /* work arrays */
+  cx* N;                   /* [s][M][nt] */
+  cx* result;              /* [M][nt] */
+  cx* psi_i;               /* [M][nt] */
+  cx* U;                   /* [M][nt] */
+  cx* V;                   /* [P][nt] */
+  cx* VR;                  /* [P][nt] */
+  cx* sigma;               /* [M][nt] */
+  cx* tmp;                 /* [nthreads][2][nt] per-thread FFT buffers */
+};
+
+static cx* cx_alloc(size_t n) { return (cx*)calloc(n ? n : 1, sizeof(cx)); }
+
+/* Butcher tableaux (src/RungeKutta.cpp:3-21 Butcher4, :24-72 Butcher7(nu),
+ * :125-175 Butcher11). Entries are the published formulas evaluated in the same
+ * order, so the doubles match the reference bit for bit. Butcher11 pushes a 12th
+ * c entry that is never read (only c[0..s-1] are used); it is not kept here. */
+static void tableau_rk4(orc_solver* s) {
+  s->s = 4;
+  s->a[1][0] = 1 / 2.;
+  s->a[2][1] = 1 / 2.;
+  s->a[3][2] = 1.;
+  s->b[0] = 1 / 6.; s->b[1] = 1 / 3.; s->b[2] = 1 / 3.; s->b[3] = 1 / 6.;
+  s->c[0] = 0.; s->c[1] = 1 / 2.; s->c[2] = 1 / 2.; s->c[3] = 1.;
+}
+
+static void tableau_7stage(orc_solver* s, double nu) {
+  const double r21 = sqrt(21.);
+  s->s = 7;
+  s->a[1][0] = nu;
+  s->a[2][0] = (4. * nu - 1.) / (8. * nu);
+  s->a[2][1] = 1. / (8. * nu);
+  s->a[3][0] = (10. * nu - 2.) / (27. * nu);
+  s->a[3][1] = 2. / (27. * nu);
+  s->a[3][2] = (8. * nu) / (27. * nu);
+  s->a[4][0] = (56. - 77. * nu + (8. - 17. * nu) * r21) / (392. * nu);
+  s->a[4][1] = (-8. * (7. + r21)) / (392. * nu);
+  s->a[4][2] = (48. * (7. + r21) * nu) / (392. * nu);
+  s->a[4][3] = (-3. * (21. + r21) * nu) / (392. * nu);
+  s->a[5][0] = (-5. * (287. * nu - 56. - (59. * nu - 8.) * r21)) / (1960. * nu);
+  s->a[5][1] = (-40. * (7. - r21)) / (1960. * nu);
+  s->a[5][2] = (320. * r21 * nu) / (1960. * nu);
+  s->a[5][3] = (3. * (21. - 121. * r21) * nu) / (1960. * nu);
+  s->a[5][4] = (392. * (6. - r21) * nu) / (1960. * nu);
+  s->a[6][0] = (15. * ((30. * nu - 8.) - 7. * nu * r21)) / (180. * nu);
+  s->a[6][1] = 120. / (180. * nu);
+  s->a[6][2] = (-40. * (5. + 7. * r21) * nu) / (180. * nu);
+  s->a[6][3] = (63. * (2. + 3. * r21) * nu) / (180. * nu);
+  s->a[6][4] = (-14. * (49. - 9. * r21) * nu) / (180. * nu);
+  s->a[6][5] = (70. * (7. + r21) * nu) / (180. * nu);
+  s->b[0] = 9 / 180.; s->b[1] = 0.; s->b[2] = 64 / 180.; s->b[3] = 0.;
+  s->b[4] = 49 / 180.; s->b[5] = 49 / 180.; s->b[6] = 9 / 180.;
+  s->c[0] = 0.; s->c[1] = nu; s->c[2] = 1 / 2.; s->c[3] = 2 / 3.;
+  s->c[4] = (7. + r21) / 14.; s->c[5] = (7. - r21) / 14.; s->c[6] = 1.;
+}
+
+static void tableau_11stage(orc_solver* s) {
+  const double r = sqrt(21.);
+  s->s = 11;
+  s->a[1][0] = 1 / 2.;
+  s->a[2][0] = 1 / 4.; s->a[2][1] = 1 / 4.;
+  s->a[3][0] = 1 / 7.; s->a[3][1] = (-7. + 3. * r) / 98.; s->a[3][2] = (21. - 5. * r) / 49.;
+  s->a[4][0] = (11. - r) / 84.; s->a[4][2] = (18. - 4. * r) / 63.; s->a[4][3] = (21. + r) / 252.;
+  s->a[5][0] = (5. - r) / 48.; s->a[5][2] = (9. - r) / 36.; s->a[5][3] = (-231. - 14. * r) / 360.;
+  s->a[5][4] = (63. + 7. * r) / 80.;
+  s->a[6][0] = (10. + r) / 42.; s->a[6][2] = (-432. - 92. * r) / 315.; s->a[6][3] = (633. + 145. * r) / 90.;
+  s->a[6][4] = (-504. - 115. * r) / 70.; s->a[6][5] = (63. + 13. * r) / 35.;
+  s->a[7][0] = 1 / 14.; s->a[7][4] = (14. + 3. * r) / 126.; s->a[7][5] = (13. + 3. * r) / 63.;
+  s->a[7][6] = 1 / 9.;
+  s->a[8][0] = 1 / 32.; s->a[8][4] = (91. + 21. * r) / 576.; s->a[8][5] = 11 / 72.;
+  s->a[8][6] = (-385. + 75. * r) / 1152.; s->a[8][7] = (63. - 13. * r) / 128.;
+  s->a[9][0] = 1 / 14.; s->a[9][4] = 1 / 9.; s->a[9][5] = (-733. + 147. * r) / 2205.;
+  s->a[9][6] = (515. - 111. * r) / 504.; s->a[9][7] = (-51. + 11. * r) / 56.;
+  s->a[9][8] = (132. - 28. * r) / 245.;
+  s->a[10][4] = (-42. - 7. * r) / 18.; s->a[10][5] = (-18. - 28. * r) / 45.;
+  s->a[10][6] = (-273. + 53. * r) / 72.; s->a[10][7] = (301. - 53. * r) / 72.;
+  s->a[10][8] = (28. + 28. * r) / 45.; s->a[10][9] = (49. + 7. * r) / 18.;
+  s->b[0] = 1 / 20.; s->b[7] = 49 / 180.; s->b[8] = 16 / 45.; s->b[9] = 49 / 180.; s->b[10] = 1 / 20.;
+  s->c[0] = 0.; s->c[1] = 1 / 2.; s->c[2] = 1 / 2.; s->c[3] = (7. - r) / 14.; s->c[4] = (7. - r) / 14.;
+  s->c[5] = 1 / 2.; s->c[6] = (7. + r) / 14.; s->c[7] = (7. + r) / 14.; s->c[8] = 1 / 2.;
+  s->c[9] = (7. - r) / 14.; s->c[10] = 1.;
+}
+
+/* setRungeKuttaCoefficients (src/MultimodePropagation.cpp:15-27). */
+static void set_tableau(orc_solver* s, unsigned method_order) {
+  memset(s->a, 0, sizeof s->a);
+  memset(s->b, 0, sizeof s->b);
+  memset(s->c, 0, sizeof s->c);
+  if (method_order == 8) tableau_11stage(s);
+  else if (method_order == 6) tableau_7stage(s, 1.);
+  else tableau_rk4(s);
+}
+
+/* factorial (src/MultimodePropagation.cpp:202-210): int arithmetic. */
+static int factorial_int(int a) {
+  int r = 1;
+  for (int i = 1; i <= a; i++) r *= i;
+  return r;
+}
+
+static int validate(const mmp_params* in) {
+  if (!in) { set_err("params is NULL"); return 0; }
+  if (in->n_modes == 0) { set_err("n_modes must be >= 1"); return 0; }
+  if (in->nt == 0) { set_err("nt must be >= 1"); return 0; }
+  if (in->n_beta < 3) { set_err("n_beta must be >= 3"); return 0; }
+  if (!in->dispersion_coefficients || !in->coupling_offsets || !in->initial_fields) {
+    set_err("missing array"); return 0;
+  }
+  if (in->coupling_offsets[0] != 0) { set_err("coupling_offsets[0] != 0"); return 0; }
+  for (unsigned p = 0; p < in->n_modes; p++)
+    if (in->coupling_offsets[p + 1] < in->coupling_offsets[p]) { set_err("coupling_offsets decreasing"); return 0; }
+  unsigned nnz = in->coupling_offsets[in->n_modes];
+  if (nnz && (!in->coupling_klm || !in->coupling_values)) { set_err("missing coupling arrays"); return 0; }
+  for (unsigned i = 0; i < 3 * nnz; i++)
+    if (in->coupling_klm[i] >= in->n_modes) { set_err("coupling index out of range"); return 0; }
+  if (in->raman && !in->raman_response) { set_err("raman_response missing"); return 0; }
+  if (in->n_steps == 0) { set_err("n_steps must be >= 1"); return 0; }
+  return 1;
+}
+
+/* Constructor chain: MultimodePropagation (src/MultimodePropagation.cpp:4-12),
+ * setDimensionlessVariables (:31-56), getMax (:67-96),
+ * setDimensionless{Dispersion,Coupling}Coefficients (:99-136), setUniquePairs
+ * (:139-155), MultimodePropagationCPU::setFFTPlans (src/MultimodePropagation_CPU.cpp:9-21),
+ * MultimodePropagationCPU_RamanON (:170-177). */
+orc_solver* orc_create(const mmp_params* in, const orc_options* opt) {
+  if (!validate(in)) return NULL;
+  orc_solver* s = (orc_solver*)calloc(1, sizeof *s);
+  if (!s) { set_err("out of memory"); return NULL; }
+  if (opt) s->opt = *opt;
+  const unsigned M = in->n_modes, nt = in->nt, nb = in->n_beta;
+  s->M = M; s->nt = nt; s->n_beta = nb;
+  s->T0 = in->pulse_width;
+  set_tableau(s, in->method_order);
+
+  /* setDimensionlessVariables */
+  double gamma = in->nonlinearity_const;
+  s->sign_gamma = gamma >= 0 ? 1 : -1;
+  double max_beta2 = 0.;
+  for (unsigned p = 0; p < M; p++) {
+    double v = fabs(in->dispersion_coefficients[p * nb + 2]);
+    if (v > max_beta2) max_beta2 = v;
+  }
+  const unsigned nnz = in->coupling_offsets[M];
+  s->nnz = nnz;
+  double max_Q = 0.;
+  for (unsigned i = 0; i < nnz; i++) {
+    double v = fabs(in->coupling_values[i]);
+    if (v > max_Q) max_Q = v;
+  }
+  s->max_beta2 = max_beta2;
+  s->max_Q = max_Q;
+  double LD = (s->T0 * s->T0) / max_beta2;
+  s->LD = LD;
+  s->beta_tilde = (double*)malloc(sizeof(double) * M * nb);
+  double* cfac = (double*)malloc(sizeof(double) * nb);
+  for (unsigned i = 0; i < nb; i++) cfac[i] = 1. / pow(s->T0, i) * LD;
+  for (unsigned p = 0; p < M; p++)
+    for (unsigned i = 0; i < nb; i++) s->beta_tilde[p * nb + i] = in->dispersion_coefficients[p * nb + i] * cfac[i];
+  free(cfac);
+  s->q_off = (unsigned*)malloc(sizeof(unsigned) * (M + 1));
+  memcpy(s->q_off, in->coupling_offsets, sizeof(unsigned) * (M + 1));
+  s->q_klm = (unsigned*)malloc(sizeof(unsigned) * 3 * (nnz ? nnz : 1));
+  s->q_val = (double*)malloc(sizeof(double) * (nnz ? nnz : 1));
+  if (nnz) memcpy(s->q_klm, in->coupling_klm, sizeof(unsigned) * 3 * nnz);
+  for (unsigned i = 0; i < nnz; i++) s->q_val[i] = in->coupling_values[i] / max_Q;
+  s->t_final = in->time_window / s->T0;
+  s->delta_t = (double)s->t_final / nt;
+  s->z_final = in->fiber_length / LD;
+  s->delta_z = s->z_final / in->n_steps;
+  s->conv = sqrt(fabs(gamma) * LD) * sqrt(max_Q);
+
+  /* setDimensionlessInitialFields (:59-63): psi = A0 * conversion_factor */
+  s->psi = cx_alloc((size_t)M * nt);
+  const cx* A0 = (const cx*)in->initial_fields;
+  for (size_t j = 0; j < (size_t)M * nt; j++) s->psi[j] = cx_scale(A0[j], s->conv);
+
+  /* setUniquePairs (:139-155): first-occurrence order of l + m*M */
+  s->unique_pairs = (unsigned*)malloc(sizeof(unsigned) * (nnz ? nnz : 1));
+  s->nnz_slot = (unsigned*)malloc(sizeof(unsigned) * (nnz ? nnz : 1));
+  unsigned P = 0;
+  for (unsigned i = 0; i < nnz; i++) {
+    unsigned lm = s->q_klm[3 * i + 1] + s->q_klm[3 * i + 2] * M;
+    unsigned j = 0;
+    while (j < P && s->unique_pairs[j] != lm) j++;
+    if (j == P) s->unique_pairs[P++] = lm;
+  }
+  s->P = P;
+  /* slot lookup of each nonzero (src/MultimodePropagation_CPU.cpp:149-150,220-221) */
+  for (unsigned i = 0; i < nnz; i++) {
+    unsigned lm = s->q_klm[3 * i + 1] + s->q_klm[3 * i + 2] * M;
+    unsigned j = 0;
+    while (s->unique_pairs[j] != lm) j++;
+    s->nnz_slot[i] = j;
+  }
+  /* decoded pair per slot: the reference CPU decodes the slot counter
+   * (src/MultimodePropagation_CPU.cpp:128-129,198-199); PHYSICAL decodes the code. */
+  s->pair_a = (unsigned*)malloc(sizeof(unsigned) * (P ? P : 1));
+  s->pair_b = (unsigned*)malloc(sizeof(unsigned) * (P ? P : 1));
+  for (unsigned j = 0; j < P; j++) {
+    unsigned code = (s->opt.pair_decode == MMP_PAIRS_PHYSICAL) ? s->unique_pairs[j] : j;
+    s->pair_a[j] = code % M;
+    s->pair_b[j] = code / M;
+  }
+  if (s->opt.pair_decode != MMP_PAIRS_PHYSICAL) {
+    for (unsigned j = 0; j < P; j++)
+      if (s->pair_b[j] >= M) {
+        set_err("REFERENCE_CPU pair decode reads a mode index >= M (more than M*M slots)");
+        orc_destroy(s);
+        return NULL;
+      }
+  }
+
+  s->L = cx_alloc((size_t)M * nt);
+  s->E1 = cx_alloc((size_t)M * nt);
+  s->E = cx_alloc((size_t)s->s * M * nt);
+  s->N = cx_alloc((size_t)s->s * M * nt);
+  s->result = cx_alloc((size_t)M * nt);
+  s->psi_i = cx_alloc((size_t)M * nt);
+  s->U = cx_alloc((size_t)M * nt);
+  s->V = cx_alloc((size_t)(P ? P : 1) * nt);
+  s->VR = cx_alloc((size_t)(P ? P : 1) * nt);
+  s->sigma = cx_alloc((size_t)M * nt);
+#ifdef _OPENMP
+  s->nthreads = s->opt.n_threads > 0 ? s->opt.n_threads : omp_get_max_threads();
+#else
+  s->nthreads = 1;
+#endif
+  s->tmp = cx_alloc((size_t)s->nthreads * 3 * nt);
+  if (!fft_plan_init(&s->plan, nt)) { set_err("out of memory"); orc_destroy(s); return NULL; }
+
+  s->raman = in->raman ? 1 : 0;
+  s->fR = in->raman_proportion;
+  s->hR_fft = cx_alloc(nt);
+  if (s->raman) {
+    /* hR_fft = FFT(h_R) (src/MultimodePropagation_CPU.cpp:170-177) */
+    cx* hR = cx_alloc(nt);
+    for (unsigned i = 0; i < nt; i++) { hR[i].re = in->raman_response[i]; hR[i].im = 0.; }
+    fft_forward(&s->plan, hR, s->hR_fft, s->opt.fft_variant, s->tmp);
+    free(hR);
+  }
+  s->z = 0.;
+  return s;
+}
+
+void orc_destroy(orc_solver* s) {
+  if (!s) return;
+  free(s->beta_tilde); free(s->q_off); free(s->q_klm); free(s->q_val);
+  free(s->unique_pairs); free(s->nnz_slot); free(s->pair_a); free(s->pair_b);
+  free(s->L); free(s->psi); free(s->E1); free(s->E); free(s->hR_fft);
+  free(s->N); free(s->result); free(s->psi_i); free(s->U); free(s->V); free(s->VR);
+  free(s->sigma); free(s->tmp);
+  fft_plan_free(&s->plan);
+  free(s);
+}
+
+/* computeLinearOperators (src/MultimodePropagation.cpp:165-187): alpha_j uses
+ * k = j for j <= nt/2 (the Nyquist bin is +nt/2), else j - nt. */
+static void compute_linear_operators(orc_solver* s) {
+  const unsigned M = s->M, nt = s->nt, nb = s->n_beta;
+  const cx i_unit = {0., 1.};
+  for (unsigned p = 0; p < M; p++) {
+    const double* bt = s->beta_tilde + p * nb;
+    for (unsigned j = 0; j < nt; j++) {
+      int k = (j <= nt / 2) ? (int)j : (int)j - (int)nt;
+      double alpha = (2 * M_PI / s->t_final) * k;
+      double poly = bt[0] - bt[1] * alpha - bt[2] / 2 * (alpha * alpha);
+      cx Lpj = cx_scale(i_unit, poly);
+      for (unsigned kk = 3; kk < nb; kk++) {
+        double term = pow(-1, kk) * bt[kk] / factorial_int((int)kk) * pow(alpha, kk);
+        Lpj = cx_add(Lpj, cx_scale(i_unit, term));
+      }
+      s->L[(size_t)p * nt + j] = Lpj;
+    }
+  }
+}
+
+/* initExponentialOperatorsMatrix (src/MultimodePropagation.cpp:190-199). */
+static void init_exponential_operators(orc_solver* s) {
+  const unsigned M = s->M, nt = s->nt;
+  for (unsigned p = 0; p < M; p++) {
+    for (unsigned j = 0; j < nt; j++) {
+      size_t idx = (size_t)p * nt + j;
+      s->E1[idx] = cx_exp(cx_scale(s->L[idx], s->delta_z));
+    }
+    for (unsigned i = 0; i < s->s; i++) {
+      double ch = s->c[i] * s->delta_z;
+      for (unsigned j = 0; j < nt; j++) {
+        size_t idx = (size_t)p * nt + j;
+        s->E[((size_t)i * M + p) * nt + j] = cx_exp(cx_scale(s->L[idx], ch));
+      }
+    }
+  }
+}
+
+/* initLawsonRK (src/MultimodePropagation.cpp:158-163). */
+void orc_init_operators(orc_solver* s) {
+  s->z = 0.;
+  compute_linear_operators(s);
+  init_exponential_operators(s);
+}
+
+static int thread_id(void) {
+#ifdef _OPENMP
+  return omp_get_thread_num();
+#else
+  return 0;
+#endif
+}
+
+/* computeNonlinearity, RamanOFF (src/MultimodePropagation_CPU.cpp:111-166) and
+ * RamanON (:180-238) with convolveWithRamanResponse (:241-251). */
+static void nonlinearity(orc_solver* s, cx* N_i, const cx* E_i, const cx* psi_i) {
+  const unsigned M = s->M, nt = s->nt, P = s->P;
+  const int var = s->opt.fft_variant;
+  const int nth = s->nthreads;
+  cx* U = s->U;
+  /* Step 1: U_p = F^-1[E_i,p * F psi_i,p] */
+#pragma omp parallel for num_threads(nth) schedule(static)
+  for (int p = 0; p < (int)M; p++) {
+    cx* buf = s->tmp + (size_t)thread_id() * 3 * nt;
+    cx* scr = buf + nt;
+    fft_forward(&s->plan, psi_i + (size_t)p * nt, buf, var, scr);
+    const cx* Ep = E_i + (size_t)p * nt;
+    for (unsigned j = 0; j < nt; j++) buf[j] = cx_mul(buf[j], Ep[j]);
+    fft_inverse(&s->plan, buf, U + (size_t)p * nt, var, scr);
+  }
+  /* Step 2: V_j = U_a * conj(U_b) for slot j; RamanON also V_R,j = dt * F^-1[hR_fft * F V_j] */
+#pragma omp parallel for num_threads(s->opt.parallel_pairs ? nth : 1) schedule(static)
+  for (int j = 0; j < (int)P; j++) {
+    const cx* Ua = U + (size_t)s->pair_a[j] * nt;
+    const cx* Ub = U + (size_t)s->pair_b[j] * nt;
+    cx* Vj = s->V + (size_t)j * nt;
+    for (unsigned t = 0; t < nt; t++) Vj[t] = cx_mul(Ua[t], cx_conj(Ub[t]));
+    if (s->raman) {
+      cx* buf = s->tmp + (size_t)thread_id() * 3 * nt;
+      cx* scr = buf + nt;
+      cx* VRj = s->VR + (size_t)j * nt;
+      fft_forward(&s->plan, Vj, buf, var, scr);
+      for (unsigned t = 0; t < nt; t++) buf[t] = cx_mul(buf[t], s->hR_fft[t]);
+      fft_inverse(&s->plan, buf, VRj, var, scr);
+      for (unsigned t = 0; t < nt; t++) VRj[t] = cx_scale(VRj[t], s->delta_t);
+    }
+  }
+  /* Steps 3 and 4 per mode */
+  const double fR = s->fR;
+  const cx i_sgn = {0., s->sign_gamma};
+#pragma omp parallel for num_threads(nth) schedule(static)
+  for (int p = 0; p < (int)M; p++) {
+    cx* sigma = s->sigma + (size_t)p * nt;
+    for (unsigned t = 0; t < nt; t++) { sigma[t].re = 0.; sigma[t].im = 0.; }
+    for (unsigned e = s->q_off[p]; e < s->q_off[p + 1]; e++) {
+      const unsigned k = s->q_klm[3 * e];
+      const double Q = s->q_val[e];
+      const unsigned slot = s->nnz_slot[e];
+      const cx* Vs = s->V + (size_t)slot * nt;
+      const cx* Uk = U + (size_t)k * nt;
+      if (s->raman) {
+        const cx* VRs = s->VR + (size_t)slot * nt;
+        const double c1 = Q * (1 - fR);
+        const double c2 = Q * fR * s->T0;
+        for (unsigned t = 0; t < nt; t++) {
+          cx W = cx_scale(Vs[t], c1);
+          W = cx_add(W, cx_scale(VRs[t], c2));
+          W = cx_mul(W, Uk[t]);
+          sigma[t] = cx_add(sigma[t], W);
+        }
+      } else {
+        for (unsigned t = 0; t < nt; t++) {
+          cx W = cx_scale(Vs[t], Q);
+          W = cx_mul(W, Uk[t]);
+          sigma[t] = cx_add(sigma[t], W);
+        }
+      }
+    }
+    /* Step 4: N_i,p = F^-1[conj(E_i,p) * F sigma_p] * i*sign_gamma */
+    cx* buf = s->tmp + (size_t)thread_id() * 3 * nt;
+    cx* scr = buf + nt;
+    fft_forward(&s->plan, sigma, buf, var, scr);
+    const cx* Ep = E_i + (size_t)p * nt;
+    for (unsigned t = 0; t < nt; t++) buf[t] = cx_mul(buf[t], cx_conj(Ep[t]));
+    cx* Np = N_i + (size_t)p * nt;
+    fft_inverse(&s->plan, buf, Np, var, scr);
+    for (unsigned t = 0; t < nt; t++) Np[t] = cx_mul(Np[t], i_sgn);
+  }
+}
+
+void orc_nonlinearity(orc_solver* s, unsigned stage, const double* psi_i, double* N_out) {
+  nonlinearity(s, (cx*)N_out, s->E + (size_t)stage * s->M * s->nt, (const cx*)psi_i);
+}
+
+/* applyRungeKuttaMethod (src/MultimodePropagation_CPU.cpp:47-85). */
+static void apply_rk(orc_solver* s, double h) {
+  const size_t MN = (size_t)s->M * s->nt;
+  memset(s->N, 0, sizeof(cx) * s->s * MN);
+  memcpy(s->result, s->psi, sizeof(cx) * MN);
+  for (unsigned i = 0; i < s->s; i++) {
+    memcpy(s->psi_i, s->psi, sizeof(cx) * MN);
+    for (unsigned j = 0; j < i; j++) {
+      if (s->a[i][j] != 0.) {
+        const double f = s->a[i][j] * h;
+        const cx* Nj = s->N + (size_t)j * MN;
+        for (size_t t = 0; t < MN; t++) s->psi_i[t] = cx_add(s->psi_i[t], cx_scale(Nj[t], f));
+      }
+    }
+    cx* Ni = s->N + (size_t)i * MN;
+    nonlinearity(s, Ni, s->E + (size_t)i * MN, s->psi_i);
+    if (s->b[i] != 0.) {
+      const double f = s->b[i] * h;
+      for (size_t t = 0; t < MN; t++) s->result[t] = cx_add(s->result[t], cx_scale(Ni[t], f));
+    }
+  }
+  memcpy(s->psi, s->result, sizeof(cx) * MN);
+}
+
+/* computeExponentialOperatorsMatrix (src/MultimodePropagation_CPU.cpp:37-44): E_i *= E1. */
+static void advance_exponentials(orc_solver* s) {
+  const size_t MN = (size_t)s->M * s->nt;
+  for (unsigned i = 0; i < s->s; i++) {
+    cx* Ei = s->E + (size_t)i * MN;
+    for (size_t t = 0; t < MN; t++) Ei[t] = cx_mul(Ei[t], s->E1[t]);
+  }
+}
+
+void orc_advance(orc_solver* s, unsigned nz) {
+  for (unsigned n = 0; n < nz; n++) {
+    apply_rk(s, s->delta_z);
+    advance_exponentials(s);
+    s->z += s->delta_z;
+  }
+}
+
+/* computeLawsonRK(nz) (src/MultimodePropagation_CPU.cpp:24-34). */
+void orc_compute_lawson_rk(orc_solver* s, unsigned nz) {
+  orc_init_operators(s);
+  orc_advance(s, nz);
+}
+
+/* computeLawsonRK() step count (src/MultimodePropagation.cpp:213-216). */
+unsigned orc_default_steps(const orc_solver* s) { return (unsigned)floor(s->z_final / s->delta_z); }
+
+/* getResult (src/MultimodePropagation_CPU.cpp:88-104). Uses the current L (computed by
+ * the last orc_compute_lawson_rk / orc_init_operators). */
+void orc_get_result(orc_solver* s, double* out) {
+  const unsigned M = s->M, nt = s->nt;
+  cx* buf = s->tmp;
+  cx* scr = buf + nt;
+  cx* phi = scr + nt;
+  for (unsigned p = 0; p < M; p++) {
+    fft_forward(&s->plan, s->psi + (size_t)p * nt, buf, s->opt.fft_variant, scr);
+    for (unsigned j = 0; j < nt; j++)
+      buf[j] = cx_mul(buf[j], cx_exp(cx_scale(s->L[(size_t)p * nt + j], s->z)));
+    fft_inverse(&s->plan, buf, phi, s->opt.fft_variant, scr);
+    cx* o = (cx*)out + (size_t)p * nt;
+    for (unsigned j = 0; j < nt; j++) { o[j].re = phi[j].re / s->conv; o[j].im = phi[j].im / s->conv; }
+  }
+}
+
+void orc_get_state(const orc_solver* s, double* psi, double* z) {
+  if (psi) memcpy(psi, s->psi, sizeof(cx) * s->M * s->nt);
+  if (z) *z = s->z;
+}
+
+void orc_set_state(orc_solver* s, const double* psi) {
+  memcpy(s->psi, psi, sizeof(cx) * s->M * s->nt);
+}
+
+void orc_get_setup(const orc_solver* s, orc_setup_info* info) {
+  info->M = s->M; info->nt = s->nt; info->stages = s->s; info->n_pairs = s->P; info->n_beta = s->n_beta;
+  info->t_final = s->t_final; info->delta_t = s->delta_t; info->z_final = s->z_final;
+  info->delta_z = s->delta_z; info->z = s->z; info->conversion_factor = s->conv;
+  info->sign_gamma = s->sign_gamma; info->T0 = s->T0; info->ld = s->LD;
+  info->max_beta2 = s->max_beta2; info->max_q = s->max_Q;
+}
+
+void orc_get_tableau(const orc_solver* s, double* a, double* b, double* c) {
+  for (unsigned i = 0; i < s->s; i++) {
+    for (unsigned j = 0; j < s->s; j++) a[i * s->s + j] = (j < i) ? s->a[i][j] : 0.;
+    b[i] = s->b[i];
+    c[i] = s->c[i];
+  }
+}
+
+void orc_get_beta_tilde(const orc_solver* s, double* out) {
+  memcpy(out, s->beta_tilde, sizeof(double) * s->M * s->n_beta);
+}
+void orc_get_q_tilde(const orc_solver* s, double* out) { memcpy(out, s->q_val, sizeof(double) * s->nnz); }
+void orc_get_unique_pairs(const orc_solver* s, unsigned* codes) { memcpy(codes, s->unique_pairs, sizeof(unsigned) * s->P); }
+void orc_get_pair_table(const orc_solver* s, unsigned* a, unsigned* b) {
+  memcpy(a, s->pair_a, sizeof(unsigned) * s->P);
+  memcpy(b, s->pair_b, sizeof(unsigned) * s->P);
+}
+void orc_get_nnz_slots(const orc_solver* s, unsigned* slots) { memcpy(slots, s->nnz_slot, sizeof(unsigned) * s->nnz); }
+void orc_get_linear_operator(const orc_solver* s, double* L) { memcpy(L, s->L, sizeof(cx) * s->M * s->nt); }
+void orc_get_e1(const orc_solver* s, double* E1) { memcpy(E1, s->E1, sizeof(cx) * s->M * s->nt); }
+void orc_get_e(const orc_solver* s, unsigned stage, double* E) {
+  memcpy(E, s->E + (size_t)stage * s->M * s->nt, sizeof(cx) * s->M * s->nt);
+}
+void orc_get_hr_fft(const orc_solver* s, double* out) { memcpy(out, s->hR_fft, sizeof(cx) * s->nt); }
