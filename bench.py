@@ -1,0 +1,146 @@
+#!/usr/bin/env python3
+"""Benchmark of the Lawson-RK4 propagation step on one MI355X.
+
+Workload (one GPU): the 6-mode GRIN fiber in plain FP64 -- M = 6 modes,
+NT = 16384 samples, 20000 RK4 steps over 1 m, Raman on, fully symmetric GRIN
+coupling tensor (nnz = 648, 36 pair slots), physical pair decode. A "step" is
+one Lawson-RK4 step of the whole M x NT field (4 stages: 4 nonlinearity
+evaluations with their FFTs, Raman convolutions and Kerr sums). The timed
+region starts with the initial field resident in HBM and ends after the last
+step's kernels complete; it covers exactly --steps steps.
+
+Prints one JSON line (metric, value in M*NT*steps/s, roofline of the dominant
+kernel from HIP-event timings, CPU baseline of the FP64 C restatement on a
+bounded sample, rel-L2 vs the FP64 oracle field at z = L when a full solve was
+timed).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "M·NT·n_steps/s at stated rel-L2 vs FP64 oracle; p50/p99 solve latency"
+HBM_PEAK_GBS = 8000.0
+GOLDEN = os.path.join(REPO, "tests", "golden", "grin6_fp64_zL.npz")
+
+
+def workload():
+    from mm_propagation_amd import configs
+    return configs.grin_fiber(M=6, nt=16384, n_steps=20000, symmetric=True)
+
+
+def cpu_baseline(sp, steps: int):
+    """Time the FP64 C restatement of the reference CPU path on a bounded sample."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_binding as ob
+    threads = min(16, os.cpu_count() or 1)
+    o = ob.Oracle(sp, pair_decode=1, n_threads=threads, parallel_pairs=True)
+    o.init_operators()
+    o.advance(2)  # warm caches and thread pool
+    t0 = time.perf_counter()
+    o.advance(steps)
+    dt = time.perf_counter() - t0
+    o.close()
+    return {"value": sp.n_modes * sp.nt * steps / dt, "unit": "M·NT·steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"grin6 M=6 NT=16384: {steps} of 20000 RK4 steps in {dt:.2f} s, OpenMP {threads} threads "
+                      "over modes and Raman pairs, radix-2 FFT"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--profile-steps", type=int, default=400)
+    ap.add_argument("--cpu-steps", type=int, default=200)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch kernels one by one instead of replaying HIP graphs (for kernel tracers)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                          "(independent fibers shard across GPUs with no exchange, see DESIGN.md)"}))
+        return 2
+
+    from mm_propagation_amd import _capi, configs
+    from mm_propagation_amd.solver import Context, MultimodePropagationGPU_RamanON
+
+    sp = workload()
+    M, NT = sp.n_modes, sp.nt
+    ctx = Context(0)
+    flags = _capi.MMP_FLAG_NO_GRAPH if args.no_graph else 0
+    s = MultimodePropagationGPU_RamanON(sp, pair_decode=_capi.MMP_PAIRS_PHYSICAL, flags=flags, ctx=ctx)
+    psi0, _ = s.get_state()
+    if args.warmup:
+        s.advance(args.warmup)
+    ctx.synchronize()
+    s.set_state(psi0)  # back to z = 0 with the initial field
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    s.advance(args.steps)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    value = M * NT * args.steps / elapsed
+    fields = s.getResult()
+    energy = s.energy()
+    e0 = configs.energy_per_mode(sp.initial_fields, sp.time_window)
+    accuracy = {"energy_drift_rel": float(abs(energy.sum() - e0.sum()) / e0.sum())}
+    if args.steps == sp.n_steps and os.path.exists(GOLDEN):
+        ref = np.load(GOLDEN)["fields"]
+        err = configs.rel_l2_per_mode(fields, ref)
+        accuracy["rel_l2_vs_oracle_max"] = float(err.max())
+        accuracy["rel_l2_vs_oracle_per_mode"] = [float(x) for x in err]
+    s.close()
+
+    # per-kernel HIP-event timings on the solver's stream over a profiled run of the same workload
+    ps = MultimodePropagationGPU_RamanON(sp, pair_decode=_capi.MMP_PAIRS_PHYSICAL, flags=_capi.MMP_FLAG_PROFILE,
+                                         ctx=ctx)
+    ps.advance(8)
+    ps.kernel_times()
+    ps.advance(args.profile_steps)
+    kt = ps.kernel_times()
+    ps.close()
+    total_ms = sum(k["total_ms"] for k in kt) or 1.0
+    dom = max(kt, key=lambda k: k["total_ms"])
+    avg_s = dom["total_ms"] / 1e3 / max(dom["launches"], 1)
+    achieved = dom["bytes_per_launch"] / avg_s / 1e9
+    roofline = {"bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "avg_launch_us": round(avg_s * 1e6, 3), "bytes_per_launch": dom["bytes_per_launch"],
+                "share_of_kernel_time": round(dom["total_ms"] / total_ms, 4),
+                "kernels": {k["name"]: {"avg_us": round(k["total_ms"] * 1e3 / max(k["launches"], 1), 3),
+                                        "launches": k["launches"],
+                                        "gbps": round(k["bytes_per_launch"] / max(k["total_ms"] / 1e3 /
+                                                                                  max(k["launches"], 1), 1e-12) / 1e9, 1)}
+                            for k in kt}}
+
+    cpu = None
+    if not args.no_cpu_baseline and args.cpu_steps > 0:
+        cpu = cpu_baseline(sp, args.cpu_steps)
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "M·NT·steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "grin6_fp64: M=6, NT=16384, T_w=20 ps, L=1 m, n_steps=20000, RK4, Raman on, "
+                               "symmetric GRIN coupling nnz=648, physical pair decode",
+                   "global_batch": 1, "seq_len": NT, "parallelism": "single GPU"},
+        "accuracy": accuracy, "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -121,6 +121,19 @@ const char* mmp_last_error(void);
 /* ABI version compiled into the library (MMPROP_ABI_VERSION). */
 int mmp_abi_version(void);
 
+/* Host-only planning (no device needed, no reference counterpart): validates
+ * params and options exactly as mmp_solver_create does and fills the solver
+ * scalars and the device memory a solver would hold (device_bytes). Used for
+ * admission by memory footprint. */
+mmp_status mmp_plan(const mmp_params* params, const mmp_options* options, mmp_info* info_out);
+
+/* Host-only: the prepared arrays of mmp_plan -- the dimensionless initial field
+ * psi0 = conversion_factor * A0 ([M][NT] complex, src/MultimodePropagation.cpp:59-63),
+ * Im L_p(alpha_j) in natural order ([M][NT], src/MultimodePropagation.cpp:165-187)
+ * and the decoded pair (a_j, b_j) of every slot ([n_pairs] each). Any pointer may be NULL. */
+mmp_status mmp_plan_arrays(const mmp_params* params, const mmp_options* options, double* psi0_out,
+                           double* imL_out, uint32_t* pair_a_out, uint32_t* pair_b_out);
+
 /* Device context: one HIP stream on one device. No reference counterpart:
  * the reference hard-codes device 0 and the default stream
  * (src/MultimodePropagation_GPU.cu:17). */

@@ -66,14 +66,18 @@ struct GatherArgs {
   const uint32_t* off;     // [M+1] nonzeros of Q^(p)
   const uint32_t* k;       // [nnz]
   const uint32_t* slot;    // [nnz]
-  const double* c1;        // [nnz] Q (1 - fR)  (or Q with Raman off)
-  const double* c2;        // [nnz] Q fR T0
+  const double* q;         // [nnz] Q / max|Q|
   const uint32_t* slot_a;  // [P]
   const uint32_t* slot_b;  // [P]
   const uint32_t* slot_line;  // [P] Raman conv line
   const uint32_t* slot_kind;  // [P]
+  int n_slots;             // P
   int raman;
+  double kv, kr;           // slot mix: (1 - fR, fR T0) with Raman on, (1, 0) off
 };
+
+// Sample tile of the gather kernel for LDS capacity `cap` complex values (0 if none fits).
+int gather_tile(const Geometry& g, int n_slots, int cap);
 
 struct PairLineArgs {
   const uint32_t* line_a;  // [n_lines]

@@ -335,37 +335,53 @@ __global__ void __launch_bounds__(64) k_row_raman(Geometry g, const cd* __restri
 }
 
 // ---------------------------------------------------------------------------
-// Kerr + Raman gather: sigma_p[t] = sum over the nonzeros of Q^(p) in insertion order
-//   W = V_slot * c1 + VR_slot * c2;  sigma += W * U_k
+// Kerr + Raman gather over a tile of TS consecutive samples:
+//   X_slot = kv * U_a conj(U_b) + kr * VR_slot        (slot mix, into LDS)
+//   sigma_p = sum over the nonzeros (k, slot, q) of Q^(p):  q * X_slot * U_k
+// U and X live in LDS; each (sample, mode) lane accumulates its own sum.
 // ---------------------------------------------------------------------------
+template <int CAP>
 __global__ void __launch_bounds__(256) k_gather(Geometry g, const cd* __restrict__ U, const cd* __restrict__ VR,
-                                                cd* __restrict__ sigma, GatherArgs a) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int p = blockIdx.y;
-  if (t >= g.N) return;
-  cd acc = {0., 0.};
-  const uint32_t e0 = a.off[p], e1 = a.off[p + 1];
-  for (uint32_t e = e0; e < e1; e++) {
-    const uint32_t k = a.k[e], slot = a.slot[e];
-    const cd ua = U[(size_t)a.slot_a[slot] * g.N + t];
-    const cd ub = U[(size_t)a.slot_b[slot] * g.N + t];
-    const cd V = cmulc(ua, ub);
-    cd W = cscale(V, a.c1[e]);
+                                                cd* __restrict__ sigma, GatherArgs a, int TS) {
+  __shared__ cd sh[CAP];
+  const int M = g.M, P = a.n_slots;
+  cd* Ush = sh;            // [M][TS]
+  cd* Xsh = sh + M * TS;   // [P][TS]
+  const int t0 = blockIdx.x * TS;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < M * TS; e += 256) {
+    const int k = e / TS, j = e % TS;
+    Ush[e] = U[(size_t)k * g.N + t0 + j];
+  }
+  __syncthreads();
+  for (int e = tid; e < P * TS; e += 256) {
+    const int s = e / TS, j = e % TS;
+    const cd V = cmulc(Ush[a.slot_a[s] * TS + j], Ush[a.slot_b[s] * TS + j]);
+    cd X = cscale(V, a.kv);
     if (a.raman) {
-      const cd r = VR[(size_t)a.slot_line[slot] * g.N + t];
+      const cd r = VR[(size_t)a.slot_line[s] * g.N + t0 + j];
       cd vr;
-      switch (a.slot_kind[slot]) {
+      switch (a.slot_kind[s]) {
         case kDirect: vr = r; break;
         case kConj: vr = cconj(r); break;
         case kRe: vr = {r.re, 0.}; break;
         default: vr = {r.im, 0.}; break;
       }
-      W = cadd(W, cscale(vr, a.c2[e]));
+      X = cadd(X, cscale(vr, a.kr));
     }
-    W = cmul(W, U[(size_t)k * g.N + t]);
-    acc = cadd(acc, W);
+    Xsh[e] = X;
   }
-  sigma[(size_t)p * g.N + t] = acc;
+  __syncthreads();
+  const int j = tid % TS;
+  for (int p = tid / TS; p < M; p += 256 / TS) {
+    cd acc = {0., 0.};
+    const uint32_t e1 = a.off[p + 1];
+    for (uint32_t e = a.off[p]; e < e1; e++) {
+      const cd W = cmul(Xsh[a.slot[e] * TS + j], Ush[a.k[e] * TS + j]);
+      acc = cadd(acc, cscale(W, a.q[e]));
+    }
+    sigma[(size_t)p * g.N + t0 + j] = acc;
+  }
 }
 
 __global__ void k_advance_z(StepState* st, double h) {
@@ -514,11 +530,29 @@ hipError_t launch_row_raman(const Geometry& g, const Twiddles& tw, cd* data, int
   MMP_DISPATCH_L(g.N2, row_raman_impl, g, tw, data, lines, hspec, hscale, s);
 }
 
+// Largest sample tile (a power of two dividing N, at most 64, keeping N / TS >= 256
+// workgroups when possible) whose U and slot tiles fit the LDS capacity.
+int gather_tile(const Geometry& g, int n_slots, int cap) {
+  int ts = 64;
+  while (ts > 8 && (g.N / ts < 256 || (g.M + n_slots) * ts > cap)) ts >>= 1;
+  return ((g.M + n_slots) * ts <= cap) ? ts : 0;
+}
+
 hipError_t launch_gather(const Geometry& g, const cd* U, const cd* VR, cd* sigma, const GatherArgs& a,
                          hipStream_t s) {
-  dim3 grid((g.N + 255) / 256, g.M);
-  hipLaunchKernelGGL(k_gather, grid, dim3(256), 0, s, g, U, VR, sigma, a);
-  return hipGetLastError();
+  const int caps[3] = {2048, 4096, 8192};
+  for (int ci = 0; ci < 3; ci++) {
+    const int ts = gather_tile(g, a.n_slots, caps[ci]);
+    if (!ts || g.N % ts) continue;
+    dim3 grid(g.N / ts);
+    switch (ci) {
+      case 0: hipLaunchKernelGGL(k_gather<2048>, grid, dim3(256), 0, s, g, U, VR, sigma, a, ts); break;
+      case 1: hipLaunchKernelGGL(k_gather<4096>, grid, dim3(256), 0, s, g, U, VR, sigma, a, ts); break;
+      default: hipLaunchKernelGGL(k_gather<8192>, grid, dim3(256), 0, s, g, U, VR, sigma, a, ts); break;
+    }
+    return hipGetLastError();
+  }
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_advance_z(StepState* st, double h, hipStream_t s) {

@@ -277,23 +277,17 @@ mmp_status prepare_problem(const mmp_params* in, const mmp_options& opt, Problem
       P.slot_kind[j] = (pos % 2 == 0) ? 2u : 3u;  // real / imaginary part
     }
   }
-  // Kerr gather coefficients: W = V*(Q(1-fR)) + VR*(Q fR T0) with Raman on
-  // (src/MultimodePropagation_CPU.cpp:223-225), W = V*Q with Raman off (:152-153).
+  // Kerr gather: W = Q * (kv * V + kr * VR) * U_k with (kv, kr) = (1 - fR, fR T0) when Raman is
+  // on (src/MultimodePropagation_CPU.cpp:223-226) and (1, 0) when off (:152-154).
   P.off.assign(in->coupling_offsets, in->coupling_offsets + M + 1);
   P.k.resize(nnz);
-  P.c1.resize(nnz);
-  P.c2.resize(nnz);
+  P.q.resize(nnz);
   for (uint32_t e = 0; e < nnz; e++) {
     P.k[e] = in->coupling_klm[3 * e];
-    const double Q = in->coupling_values[e] / max_q;
-    if (P.raman) {
-      P.c1[e] = Q * (1 - P.fR);
-      P.c2[e] = Q * P.fR * P.T0;
-    } else {
-      P.c1[e] = Q;
-      P.c2[e] = 0.;
-    }
+    P.q[e] = in->coupling_values[e] / max_q;
   }
+  P.kv = P.raman ? 1 - P.fR : 1.;
+  P.kr = P.raman ? P.fR * P.T0 : 0.;
   // a stage result N_j needs a stash slot when a stage i > j + 1 reads it
   P.stash_needed.assign(P.rk.s, 0);
   for (int i = 0; i < P.rk.s; i++)

@@ -44,7 +44,8 @@ struct Problem {
   std::vector<uint32_t> line_a, line_b, line_self;  // [n_lines]
   // Kerr gather (CSR by p, insertion order)
   std::vector<uint32_t> off, k, slot;  // [M+1], [nnz], [nnz]
-  std::vector<double> c1, c2;          // [nnz]
+  std::vector<double> q;               // [nnz] Q / max|Q|
+  double kv = 1, kr = 0;               // slot mix: X = kv*V + kr*VR
   // which stage results must be kept for later stages
   std::vector<int> stash_needed;       // [s]
 };

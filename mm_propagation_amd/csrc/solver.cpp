@@ -281,6 +281,42 @@ void release(mmp_solver* s) {
   delete s;
 }
 
+// Device bytes create_impl allocates for problem P (kept in step with its dalloc calls).
+size_t planned_device_bytes(const Problem& P) {
+  auto c = [](size_t n) { return n ? n : (size_t)1; };
+  const size_t MN = (size_t)P.M * P.N, L = P.line_a.size();
+  bool any_stash = false;
+  for (int j = 0; j < P.rk.s; j++) any_stash = any_stash || P.stash_needed[j];
+  size_t b = 16 * (4 * MN) + 8 * MN + 16 * (P.N2 + P.N1 + P.N) + sizeof(StepState) + 8 * c(P.M);
+  if (any_stash) b += 16 * (size_t)P.rk.s * MN;
+  if (P.raman) b += 16 * c(L) * P.N + 16 * (size_t)P.N + 8 * (size_t)P.N;
+  b += 4 * (P.M + 1) + 4 * 2 * c(P.nnz) + 8 * c(P.nnz) + 4 * 4 * c(P.P) + 4 * 3 * c(L);
+  return b;
+}
+
+void fill_info(const Problem& P, mmp_info* o) {
+  std::memset(o, 0, sizeof *o);
+  o->n_modes = P.M;
+  o->nt = P.N;
+  o->stages = (uint32_t)P.rk.s;
+  o->n_pairs = P.P;
+  o->nnz = P.nnz;
+  o->n_conv_lines = (uint32_t)P.line_a.size();
+  o->n1 = P.N1;
+  o->n2 = P.N2;
+  o->default_steps = P.default_steps;
+  o->t_final = P.t_final;
+  o->delta_t = P.delta_t;
+  o->z_final = P.z_final;
+  o->delta_z = P.delta_z;
+  o->conversion_factor = P.conversion_factor;
+  o->sign_gamma = P.sign_gamma;
+  o->ld = P.ld;
+  o->max_beta2 = P.max_beta2;
+  o->max_q = P.max_q;
+  o->device_bytes = planned_device_bytes(P);
+}
+
 mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options* options, mmp_solver** out) {
   if (!ctx || !out) return fail(MMP_ERR_INVALID_ARGUMENT, "ctx and out are required");
   *out = nullptr;
@@ -333,13 +369,12 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
     A(dalloc(s, &s->hR, P.N));
   }
   uint32_t *d_off, *d_k, *d_slot, *d_sa, *d_sb, *d_sl, *d_sk, *d_la, *d_lb, *d_ls;
-  double *d_c1, *d_c2;
+  double* d_q;
   const size_t nnz1 = P.nnz ? P.nnz : 1, P1 = P.P ? P.P : 1, L1 = s->n_lines ? s->n_lines : 1;
   A(dalloc(s, &d_off, P.M + 1));
   A(dalloc(s, &d_k, nnz1));
   A(dalloc(s, &d_slot, nnz1));
-  A(dalloc(s, &d_c1, nnz1));
-  A(dalloc(s, &d_c2, nnz1));
+  A(dalloc(s, &d_q, nnz1));
   A(dalloc(s, &d_sa, P1));
   A(dalloc(s, &d_sb, P1));
   A(dalloc(s, &d_sl, P1));
@@ -358,8 +393,7 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
   if (P.nnz) {
     A(upload(s, d_k, P.k.data(), P.nnz * 4));
     A(upload(s, d_slot, P.slot.data(), P.nnz * 4));
-    A(upload(s, d_c1, P.c1.data(), P.nnz * 8));
-    A(upload(s, d_c2, P.c2.data(), P.nnz * 8));
+    A(upload(s, d_q, P.q.data(), P.nnz * 8));
   }
   if (P.P) {
     A(upload(s, d_sa, P.slot_a.data(), P.P * 4));
@@ -374,7 +408,11 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
   }
   if (st != MMP_OK) return cleanup(st);
   s->tw = Twiddles{rowtw, coltw, fourtw};
-  s->gather = GatherArgs{d_off, d_k, d_slot, d_c1, d_c2, d_sa, d_sb, d_sl, d_sk, P.raman};
+  s->gather = GatherArgs{d_off, d_k, d_slot, d_q, d_sa, d_sb, d_sl, d_sk, (int)P.P, P.raman, P.kv, P.kr};
+  if (!gather_tile(s->g, (int)P.P, 8192)) {
+    release(s);
+    return fail(MMP_ERR_UNSUPPORTED, "too many pair slots for the gather tile (M + P must be <= 1024)");
+  }
   s->pairs = PairLineArgs{d_la, d_lb, d_ls};
   // stage heads
   const double h = P.delta_z;
@@ -599,30 +637,51 @@ mmp_status mmp_energy(mmp_solver* s, double* energy_out) {
 
 mmp_status mmp_solver_info(mmp_solver* s, mmp_info* o) {
   if (!s || !o) return fail(MMP_ERR_INVALID_ARGUMENT, "solver and info_out are required");
-  const Problem& P = s->P;
-  std::memset(o, 0, sizeof *o);
-  o->n_modes = P.M;
-  o->nt = P.N;
-  o->stages = (uint32_t)P.rk.s;
-  o->n_pairs = P.P;
-  o->nnz = P.nnz;
-  o->n_conv_lines = (uint32_t)s->n_lines;
-  o->n1 = P.N1;
-  o->n2 = P.N2;
+  fill_info(s->P, o);
   o->steps_taken = s->steps_taken;
-  o->default_steps = P.default_steps;
-  o->t_final = P.t_final;
-  o->delta_t = P.delta_t;
-  o->z_final = P.z_final;
-  o->delta_z = P.delta_z;
   o->z = s->z_host;
-  o->conversion_factor = P.conversion_factor;
-  o->sign_gamma = P.sign_gamma;
-  o->ld = P.ld;
-  o->max_beta2 = P.max_beta2;
-  o->max_q = P.max_q;
   o->device_bytes = s->device_bytes;
   return MMP_OK;
+}
+
+mmp_status mmp_plan(const mmp_params* params, const mmp_options* options, mmp_info* o) {
+  if (!o) return fail(MMP_ERR_INVALID_ARGUMENT, "info_out is NULL");
+  try {
+    mmp_options opt{};
+    if (options) opt = *options;
+    Problem P;
+    std::string err;
+    const mmp_status st = prepare_problem(params, opt, P, err);
+    if (st != MMP_OK) return fail(st, err);
+    fill_info(P, o);
+    return MMP_OK;
+  } catch (const std::bad_alloc&) {
+    return fail(MMP_ERR_OUT_OF_MEMORY, "host allocation failed");
+  }
+}
+
+mmp_status mmp_plan_arrays(const mmp_params* params, const mmp_options* options, double* psi0_out, double* imL_out,
+                           uint32_t* pair_a_out, uint32_t* pair_b_out) {
+  try {
+    mmp_options opt{};
+    if (options) opt = *options;
+    Problem P;
+    std::string err;
+    const mmp_status st = prepare_problem(params, opt, P, err);
+    if (st != MMP_OK) return fail(st, err);
+    if (psi0_out) std::memcpy(psi0_out, P.psi0.data(), P.psi0.size() * sizeof(double));
+    if (imL_out)
+      for (uint32_t p = 0; p < P.M; p++)
+        for (uint32_t j = 0; j < P.N; j++) {
+          const uint32_t k1 = j % P.N1, k2 = j / P.N1;
+          imL_out[(size_t)p * P.N + j] = P.imL_fourstep[(size_t)p * P.N + (size_t)k1 * P.N2 + k2];
+        }
+    if (pair_a_out) std::memcpy(pair_a_out, P.slot_a.data(), P.P * sizeof(uint32_t));
+    if (pair_b_out) std::memcpy(pair_b_out, P.slot_b.data(), P.P * sizeof(uint32_t));
+    return MMP_OK;
+  } catch (const std::bad_alloc&) {
+    return fail(MMP_ERR_OUT_OF_MEMORY, "host allocation failed");
+  }
 }
 
 mmp_status mmp_kernel_times(mmp_solver* s, mmp_kernel_time* out, uint32_t capacity, uint32_t* count_out) {

@@ -106,12 +106,17 @@ class Oracle:
     """The C restatement of the reference CPU solver."""
 
     def __init__(self, sp: SimulationParameters, raman: bool = True, pair_decode: int = 0, fft_variant: int = 0,
-                 parallel_pairs: bool = True, n_threads: int = 0):
+                 parallel_pairs: bool = True, n_threads: int = -1):
         self.sp = sp
         self.cp = CParams(sp, raman)
         self.M = int(sp.n_modes)
         self.nt = int(sp.nt)
         self.nnz = self.cp.nnz
+        if n_threads < 0:
+            # small problems run single-threaded: thousands of tiny OpenMP regions per solve
+            # cost more in barriers than they gain
+            work = int(sp.n_modes) * int(sp.nt)
+            n_threads = 1 if work < 65536 else min(8, os.cpu_count() or 1)
         opt = OrcOptions(pair_decode, fft_variant, 1 if parallel_pairs else 0, n_threads)
         self.h = lib().orc_create(ctypes.byref(self.cp.c), ctypes.byref(opt))
         if not self.h:
