@@ -26,6 +26,13 @@ extern "C" {
 
 #define MMPROP_ABI_VERSION 1
 
+/* Only the entry points below are exported; the library is built with hidden visibility. */
+#if defined(__GNUC__) || defined(__clang__)
+#define MMP_API __attribute__((visibility("default")))
+#else
+#define MMP_API
+#endif
+
 typedef enum mmp_status {
   MMP_OK = 0,
   MMP_ERR_INVALID_ARGUMENT = 1, /* malformed parameters (sizes, indices, NULLs, max|Q| == 0, ...) */
@@ -113,78 +120,81 @@ typedef struct mmp_ctx mmp_ctx;
 typedef struct mmp_solver mmp_solver;
 
 /* Status text for a status code; never NULL. */
-const char* mmp_status_string(mmp_status status);
+MMP_API const char* mmp_status_string(mmp_status status);
 
 /* Detail of the last failed call on this host thread ("" if none). */
-const char* mmp_last_error(void);
+MMP_API const char* mmp_last_error(void);
 
 /* ABI version compiled into the library (MMPROP_ABI_VERSION). */
-int mmp_abi_version(void);
+MMP_API int mmp_abi_version(void);
 
 /* Host-only planning (no device needed, no reference counterpart): validates
  * params and options exactly as mmp_solver_create does and fills the solver
  * scalars and the device memory a solver would hold (device_bytes). Used for
  * admission by memory footprint. */
-mmp_status mmp_plan(const mmp_params* params, const mmp_options* options, mmp_info* info_out);
+MMP_API mmp_status mmp_plan(const mmp_params* params, const mmp_options* options, mmp_info* info_out);
 
 /* Host-only: the prepared arrays of mmp_plan -- the dimensionless initial field
  * psi0 = conversion_factor * A0 ([M][NT] complex, src/MultimodePropagation.cpp:59-63),
- * Im L_p(alpha_j) in natural order ([M][NT], src/MultimodePropagation.cpp:165-187)
- * and the decoded pair (a_j, b_j) of every slot ([n_pairs] each). Any pointer may be NULL. */
-mmp_status mmp_plan_arrays(const mmp_params* params, const mmp_options* options, double* psi0_out,
-                           double* imL_out, uint32_t* pair_a_out, uint32_t* pair_b_out);
+ * Im L_p(alpha_j) in natural order ([M][NT], src/MultimodePropagation.cpp:165-187),
+ * the decoded pair (a_j, b_j) of every slot ([n_pairs] each), and the initial Lawson
+ * tables E1 = exp(h L) ([M][NT] complex) and E_i = exp(c_i h L) ([stages][M][NT] complex)
+ * of src/MultimodePropagation.cpp:190-199, in natural order. Any pointer may be NULL. */
+MMP_API mmp_status mmp_plan_arrays(const mmp_params* params, const mmp_options* options, double* psi0_out,
+                           double* imL_out, uint32_t* pair_a_out, uint32_t* pair_b_out, double* e1_out,
+                           double* e_out);
 
 /* Device context: one HIP stream on one device. No reference counterpart:
  * the reference hard-codes device 0 and the default stream
  * (src/MultimodePropagation_GPU.cu:17). */
-mmp_status mmp_ctx_create(int device, mmp_ctx** out);
-void mmp_ctx_destroy(mmp_ctx* ctx);
+MMP_API mmp_status mmp_ctx_create(int device, mmp_ctx** out);
+MMP_API void mmp_ctx_destroy(mmp_ctx* ctx);
 
 /* Replaces the constructors MultimodePropagationGPU_RamanON/OFF(const
  * SimulationParameters&) (include/MultimodePropagation_GPU.cuh:184,196;
  * src/MultimodePropagation.cpp:4-12): validates and copies the inputs, makes
  * them dimensionless, builds the pair table and uploads everything once.
  * options may be NULL (FP64, REFERENCE_CPU pairs, graphs on). */
-mmp_status mmp_solver_create(mmp_ctx* ctx, const mmp_params* params,
+MMP_API mmp_status mmp_solver_create(mmp_ctx* ctx, const mmp_params* params,
                              const mmp_options* options, mmp_solver** out);
-void mmp_solver_destroy(mmp_solver* solver);
+MMP_API void mmp_solver_destroy(mmp_solver* solver);
 
 /* Replaces computeLawsonRK(nz) (include/MultimodePropagation_GPU.cuh:143;
  * semantics of src/MultimodePropagation_CPU.cpp:24-34): restarts at z = 0
  * with freshly built Lawson operators, keeps the current field, then takes
  * nz steps. */
-mmp_status mmp_compute_lawson_rk(mmp_solver* solver, uint32_t nz);
+MMP_API mmp_status mmp_compute_lawson_rk(mmp_solver* solver, uint32_t nz);
 
 /* Replaces computeLawsonRK() (src/MultimodePropagation.cpp:213-216):
  * mmp_compute_lawson_rk with nz = floor(z_final / delta_z). */
-mmp_status mmp_compute_lawson_rk_default(mmp_solver* solver);
+MMP_API mmp_status mmp_compute_lawson_rk_default(mmp_solver* solver);
 
 /* Resumable stepping (no reference counterpart): continues from the current
  * z without restarting. mmp_compute_lawson_rk(s, a) followed by
  * mmp_advance(s, b) equals mmp_compute_lawson_rk(s, a + b). */
-mmp_status mmp_advance(mmp_solver* solver, uint32_t n_steps);
+MMP_API mmp_status mmp_advance(mmp_solver* solver, uint32_t n_steps);
 
 /* Replaces getResult() (src/MultimodePropagation_CPU.cpp:88-104): the
  * physical field A_p = F^-1[exp(L_p z) F psi_p] / conversion_factor, written
  * to fields_out [M][NT] complex. */
-mmp_status mmp_get_result(mmp_solver* solver, double* fields_out);
+MMP_API mmp_status mmp_get_result(mmp_solver* solver, double* fields_out);
 
 /* Dimensionless interaction-picture field psi [M][NT] complex and the
  * accumulated z (the reference's protected psi and z members). Either
  * pointer may be NULL. */
-mmp_status mmp_get_state(mmp_solver* solver, double* psi_out, double* z_out);
+MMP_API mmp_status mmp_get_state(mmp_solver* solver, double* psi_out, double* z_out);
 
 /* Replaces the dimensionless field and restarts at z = 0 (the reference
  * keeps psi across computeLawsonRK calls; this sets it explicitly).
  * psi is [M][NT] complex, in dimensionless units. */
-mmp_status mmp_set_state(mmp_solver* solver, const double* psi);
+MMP_API mmp_status mmp_set_state(mmp_solver* solver, const double* psi);
 
 /* Per-mode energy of the physical field, E_p = sum_j |A_p,j|^2 * (T_w / NT)
  * in W*ps, computed on the device in FP64. energy_out is [M]. */
-mmp_status mmp_energy(mmp_solver* solver, double* energy_out);
+MMP_API mmp_status mmp_energy(mmp_solver* solver, double* energy_out);
 
 /* Solver scalars and state. */
-mmp_status mmp_solver_info(mmp_solver* solver, mmp_info* info_out);
+MMP_API mmp_status mmp_solver_info(mmp_solver* solver, mmp_info* info_out);
 
 /* With MMP_FLAG_PROFILE: the names and summed HIP-event durations (ms) and
  * launch counts of the kernels issued since the last call (then reset).
@@ -195,11 +205,11 @@ typedef struct mmp_kernel_time {
   uint64_t launches;
   double bytes_per_launch;  /* algorithmic HBM bytes of one launch */
 } mmp_kernel_time;
-mmp_status mmp_kernel_times(mmp_solver* solver, mmp_kernel_time* out,
+MMP_API mmp_status mmp_kernel_times(mmp_solver* solver, mmp_kernel_time* out,
                             uint32_t capacity, uint32_t* count_out);
 
 /* Block until all work queued on the context's stream is complete. */
-mmp_status mmp_ctx_synchronize(mmp_ctx* ctx);
+MMP_API mmp_status mmp_ctx_synchronize(mmp_ctx* ctx);
 
 #ifdef __cplusplus
 }

@@ -106,7 +106,7 @@ EXPORTS = [
     ("mmp_ctx_synchronize", c_int, [c_void_p]),
     ("mmp_plan", c_int, [POINTER(MmpParams), POINTER(MmpOptions), POINTER(MmpInfo)]),
     ("mmp_plan_arrays", c_int, [POINTER(MmpParams), POINTER(MmpOptions), POINTER(c_double), POINTER(c_double),
-                                POINTER(c_uint32), POINTER(c_uint32)]),
+                                POINTER(c_uint32), POINTER(c_uint32), POINTER(c_double), POINTER(c_double)]),
 ]
 
 _LIB = None

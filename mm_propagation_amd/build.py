@@ -27,7 +27,7 @@ DROPIN_SOURCES = ["dropin/ComplexArray.cpp", "dropin/ComplexArraysContainer.cpp"
                   "dropin/RungeKutta.cpp", "dropin/MultimodePropagation.cpp", "dropin/MultimodePropagation_GPU.cpp"]
 
 HIP_FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result",
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result", "-fvisibility=hidden",
     # packed FP32 instructions stay disabled in device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + os.path.join(REPO, "include"),

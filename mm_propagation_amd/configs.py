@@ -15,7 +15,7 @@ from typing import List, Sequence
 
 import numpy as np
 
-from .problem import NonZeroValue, SimulationParameters, Sparse3DArray
+from .problem import NonZeroValue, SimulationParameters, Sparse3DArray  # noqa: F401 (re-exported)
 
 SR = 6.4341e12  # m^-2, coupling scale of the reference examples (simulations/main.cpp:26)
 GAMMA = 1.4e-13  # W^-1 m

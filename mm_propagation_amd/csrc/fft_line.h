@@ -107,64 +107,100 @@ struct LineSlots {
   static constexpr int value = PAD ? L + L / 8 : L;
 };
 
-// One radix-8 Stockham stage with span NS (NS > 1) that reads the line from LDS
-// and writes it back to LDS (not the last stage).
-template <int L, int NS, int SIGN, int STRIDE, int PAD>
-__device__ __forceinline__ void stage8_lds(cd* v, cd* sm, const cd* __restrict__ tw, int t) {
-  constexpr int T = L / 8;
+// Stage twiddles a lane needs for one line transform, loaded before the data
+// arrives so their latency overlaps the data loads. Slots:
+//   [0..6]  radix-8 stage of span 8 (L >= 64):  W_L^((t%8) * m * L/64), m = 1..7
+//   then the trailing stage, butterflies j = t + b*T:
+//     radix-4: W_L^(j*m), m = 1..3 (3 per butterfly, b = 0, 1)
+//     radix-2: W_L^j (b = 0..3)
+template <int L>
+struct LineTw {
+  static constexpr int n8 = LinePlan<L>::n8;
+  static constexpr int tail = LinePlan<L>::tail;
+  static constexpr int base = (n8 == 2) ? 7 : 0;
+  static constexpr int count = (L == 8) ? 1 : base + (tail == 4 ? 6 : (tail == 2 ? 4 : 0));
+  cd w[count];
+};
+
+// Loads the forward-sign twiddles (the inverse transform conjugates them on use).
+template <int L>
+__device__ __forceinline__ void load_line_tw(LineTw<L>& lt, const cd* __restrict__ tw, int t) {
+  using P = LinePlan<L>;
+  constexpr int T = P::T;
+  if constexpr (L == 8) {
+    (void)tw;
+    (void)t;
+    return;
+  } else {
+    if constexpr (P::n8 == 2) {
+      const int k = t % 8;
 #pragma unroll
-  for (int i = 0; i < 8; i++) v[i] = sm[lds_pos<STRIDE, PAD>(t + i * T)];
-  const int k = t % NS;
-  constexpr int step = L / (NS * 8);
+      for (int m = 1; m < 8; m++) lt.w[m - 1] = tw[k * m * (L / 64)];
+    }
+    constexpr int b0 = LineTw<L>::base;
+    if constexpr (P::tail == 4) {
 #pragma unroll
-  for (int m = 1; m < 8; m++) v[m] = cmul(v[m], twiddle<SIGN>(tw, k * m * step));
-  dft8<SIGN>(v);
-  __syncthreads();
-  const int base = (t / NS) * NS * 8 + k;
+      for (int b = 0; b < 2; b++)
 #pragma unroll
-  for (int q = 0; q < 8; q++) sm[lds_pos<STRIDE, PAD>(base + q * NS)] = v[q];
-  __syncthreads();
+        for (int m = 1; m < 4; m++) lt.w[b0 + 3 * b + m - 1] = tw[(t + b * T) * m];
+    } else if constexpr (P::tail == 2) {
+#pragma unroll
+      for (int b = 0; b < 4; b++) lt.w[b0 + b] = tw[t + b * T];
+    }
+  }
 }
 
-// In-place transform of the line held as v[i] = x[t + i*T]; sm is the line's
-// LDS slice (L values). All lanes of the workgroup must call this together.
-// sm points at the line's first slot; element pos is at sm[lds_pos<STRIDE, PAD>(pos)].
+template <int SIGN>
+__device__ __forceinline__ cd tw_dir(cd w) {
+  return SIGN < 0 ? w : cd{w.re, -w.im};
+}
+
+// In-place transform of the line held as v[i] = x[t + i*T]; sm points at the
+// line's first LDS slot and element pos lives at sm[lds_pos<STRIDE, PAD>(pos)].
+// All lanes of the workgroup must call this together.
 template <int L, int SIGN, int STRIDE, int PAD>
-__device__ __forceinline__ void line_fft(cd* v, cd* sm, const cd* __restrict__ tw, int t) {
+__device__ __forceinline__ void line_fft(cd* v, cd* sm, const LineTw<L>& lt, int t) {
   using P = LinePlan<L>;
   constexpr int T = P::T;
   // stage 1: radix-8, span 1, inputs x[t + m*T]
   dft8<SIGN>(v);
-  if constexpr (L == 8) return;
-  __syncthreads();  // the slice may still be read by a previous use
-  // write x'[8t + q]
-#pragma unroll
-  for (int q = 0; q < 8; q++) sm[lds_pos<STRIDE, PAD>(8 * t + q)] = v[q];
-  __syncthreads();
-  if constexpr (P::n8 == 2 && P::tail == 1) {
-    // last stage: radix-8, span 8 = T, outputs stay in registers (positions t + q*T)
-#pragma unroll
-    for (int i = 0; i < 8; i++) v[i] = sm[lds_pos<STRIDE, PAD>(t + i * T)];
-    const int k = t % 8;
-#pragma unroll
-    for (int m = 1; m < 8; m++) v[m] = cmul(v[m], twiddle<SIGN>(tw, k * m * (L / 64)));
-    dft8<SIGN>(v);
-    __syncthreads();
+  if constexpr (L == 8) {
+    (void)sm;
+    (void)lt;
+    (void)t;
     return;
   } else {
-    constexpr int NS = (P::n8 == 2) ? 64 : 8;
-    if constexpr (P::n8 == 2) stage8_lds<L, 8, SIGN, STRIDE, PAD>(v, sm, tw, t);
-    // tail stage with span NS = L / tail: registers slot i = b + tail' * m
+    __syncthreads();  // the slice may still be read by a previous use
+#pragma unroll
+    for (int q = 0; q < 8; q++) sm[lds_pos<STRIDE, PAD>(8 * t + q)] = v[q];
+    __syncthreads();
 #pragma unroll
     for (int i = 0; i < 8; i++) v[i] = sm[lds_pos<STRIDE, PAD>(t + i * T)];
+    if constexpr (P::n8 == 2) {
+      // radix-8 stage of span 8
+#pragma unroll
+      for (int m = 1; m < 8; m++) v[m] = cmul(v[m], tw_dir<SIGN>(lt.w[m - 1]));
+      dft8<SIGN>(v);
+      if constexpr (P::tail == 1) {
+        __syncthreads();
+        return;  // outputs at positions t + q*T
+      }
+      __syncthreads();
+      const int k = t % 8;
+      const int base = (t / 8) * 64 + k;
+#pragma unroll
+      for (int q = 0; q < 8; q++) sm[lds_pos<STRIDE, PAD>(base + q * 8)] = v[q];
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < 8; i++) v[i] = sm[lds_pos<STRIDE, PAD>(t + i * T)];
+    }
     __syncthreads();
+    constexpr int b0 = LineTw<L>::base;
     if constexpr (P::tail == 4) {
-      // two butterflies b = 0, 1: j = t + b*T, inputs slots b + 2m, twiddle W_L^(j*m)
 #pragma unroll
       for (int b = 0; b < 2; b++) {
-        const int j = t + b * T;
-        cd a0 = v[b], a1 = cmul(v[b + 2], twiddle<SIGN>(tw, j)), a2 = cmul(v[b + 4], twiddle<SIGN>(tw, 2 * j)),
-           a3 = cmul(v[b + 6], twiddle<SIGN>(tw, 3 * j));
+        cd a0 = v[b], a1 = cmul(v[b + 2], tw_dir<SIGN>(lt.w[b0 + 3 * b])),
+           a2 = cmul(v[b + 4], tw_dir<SIGN>(lt.w[b0 + 3 * b + 1])), a3 = cmul(v[b + 6], tw_dir<SIGN>(lt.w[b0 + 3 * b + 2]));
         dft4<SIGN>(a0, a1, a2, a3);
         v[b] = a0;
         v[b + 2] = a1;
@@ -174,16 +210,12 @@ __device__ __forceinline__ void line_fft(cd* v, cd* sm, const cd* __restrict__ t
     } else if constexpr (P::tail == 2) {
 #pragma unroll
       for (int b = 0; b < 4; b++) {
-        const int j = t + b * T;
-        cd a0 = v[b], a1 = cmul(v[b + 4], twiddle<SIGN>(tw, j));
+        cd a0 = v[b], a1 = cmul(v[b + 4], tw_dir<SIGN>(lt.w[b0 + b]));
         dft2<SIGN>(a0, a1);
         v[b] = a0;
         v[b + 4] = a1;
       }
-    } else {
-      // tail == 1 with n8 == 1 cannot happen for L > 8
     }
-    (void)NS;
   }
 }
 

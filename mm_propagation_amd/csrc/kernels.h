@@ -40,44 +40,50 @@ struct StepState {
 };
 
 // Arguments of the row "stage head" kernel for stage i of the tableau
-// (i == s finishes the step).
+// (i == s finishes the step). Lawson tables E_u = exp(c_u h L) * E1^n (one per
+// distinct c_u, four-step order) are advanced by the head that last reads them
+// in a step: E_u <- E_u * E1, as src/MultimodePropagation_CPU.cpp:37-44 does.
 struct HeadArgs {
   const cd* ysig;      // column-forwarded sigma of stage i-1 (unused when has_prev == 0)
   cd* psi_hat;         // spectral state
   cd* acc;             // spectral RK accumulator
   cd* stash;           // [s][M][N] spectral N_j kept for later stages
   cd* yout;            // row-inverse output for stage i (unused when final)
-  const double* imL;   // Im L_p(k), four-step order [M][N]
-  const StepState* st;
-  double ch_prev, ch_cur;  // c_{i-1} h, c_i h
-  double bh_prev;          // b_{i-1} h (0 when b_{i-1} == 0)
-  double sgn;              // sign of gamma
-  int has_prev;            // 1 when i > 0
-  int prev_stage;          // i - 1
-  int prev_is_first;       // 1 when i - 1 == 0 (acc starts from psi_hat)
-  int stash_prev;          // 1 when N_{i-1} is read by a later stage
-  int final_stage;         // 1 when i == s: psi_hat = acc, no row-inverse output
-  int n_a;                 // number of nonzero a_ij, j < i, in increasing j
+  cd* e_prev;          // Lawson table of stage i-1 (conjugated for N_{i-1})
+  const cd* e_cur;     // Lawson table of stage i
+  const cd* e1;        // exp(h L)
+  double bh_prev;      // b_{i-1} h (0 when b_{i-1} == 0)
+  double sgn;          // sign of gamma
+  double scale_cur;    // 1/N folded into the stage-i product (exact: N is a power of two)
+  int has_prev;        // 1 when i > 0
+  int prev_stage;      // i - 1
+  int prev_is_first;   // 1 when i - 1 == 0 (acc starts from psi_hat)
+  int stash_prev;      // 1 when N_{i-1} is read by a later stage
+  int final_stage;     // 1 when i == s: psi_hat = acc, no row-inverse output
+  int update_prev;     // 1 when this head is the last reader of e_prev in the step
+  int n_a;             // number of nonzero a_ij, j < i, in increasing j
   int a_idx[kMaxStages];   // j (N_{i-1} comes from registers, earlier N_j from stash)
   double a_h[kMaxStages];  // a_ij h
 };
 
 struct GatherArgs {
-  const uint32_t* off;     // [M+1] nonzeros of Q^(p)
-  const uint32_t* k;       // [nnz]
-  const uint32_t* slot;    // [nnz]
+  const uint32_t* coff;    // [M][n_chunks+1] nonzero ranges of Q^(p) per slot chunk (sorted by slot)
+  const uint32_t* kslot;   // [nnz] k | slot << 8
   const double* q;         // [nnz] Q / max|Q|
   const uint32_t* slot_a;  // [P]
   const uint32_t* slot_b;  // [P]
   const uint32_t* slot_line;  // [P] Raman conv line
   const uint32_t* slot_kind;  // [P]
   int n_slots;             // P
+  int chunk;               // slots per LDS chunk
+  int n_chunks;
+  int cap;                 // LDS capacity (complex values) of the kernel instance
   int raman;
   double kv, kr;           // slot mix: (1 - fR, fR T0) with Raman on, (1, 0) off
 };
 
-// Sample tile of the gather kernel for LDS capacity `cap` complex values (0 if none fits).
-int gather_tile(const Geometry& g, int n_slots, int cap);
+// LDS plan of the gather for M modes and P slots: capacity and slots per chunk (0 if none fits).
+void gather_plan(int M, int P, int& cap, int& chunk);
 
 struct PairLineArgs {
   const uint32_t* line_a;  // [n_lines]

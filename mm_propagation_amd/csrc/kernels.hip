@@ -28,14 +28,24 @@ struct RowCfg {
   static constexpr int SLOTS = LineSlots<L, 1>::value;
 };
 
-constexpr int kCB = 8;  // columns per column-pass workgroup (8 x 16 B = one 128 B line)
-
-template <int L>
+// Column passes: a workgroup transforms CB neighbouring columns of G lines.
+// CB = 8 reads whole 128 B lines; CB = 4 or 2 gives more workgroups for small
+// problems (the LDS layout is then padded to keep lanes on distinct banks).
+template <int L, int CB>
 struct ColCfg {
-  static constexpr int T = L / 8;                                 // lanes per column line
-  static constexpr int G = (kCB * T >= 64) ? 1 : 64 / (kCB * T);  // lines per workgroup
-  static constexpr int THREADS = G * kCB * T;
+  static constexpr int T = L / 8;                               // lanes per column line
+  static constexpr int G = (CB * T >= 64) ? 1 : 64 / (CB * T);  // lines per workgroup
+  static constexpr int THREADS = G * CB * T;
+  static constexpr int PAD = CB < 8 ? 1 : 0;
+  static constexpr int SLOTS = LineSlots<L, PAD>::value;        // per column
 };
+
+// Block ids are dealt round-robin to the 8 XCDs; remap them so that consecutive
+// logical blocks (neighbouring columns) run on one XCD and share its L2.
+__device__ __forceinline__ int xcd_block(int bid, int nb) {
+  if (nb % 8) return bid;
+  return (bid % 8) * (nb / 8) + bid / 8;
+}
 
 __device__ __forceinline__ cd phase(double imL, double tau) {
   double s, c;
@@ -46,6 +56,13 @@ __device__ __forceinline__ cd phase(double imL, double tau) {
 // ---------------------------------------------------------------------------
 // row stage head
 // ---------------------------------------------------------------------------
+// E <- E * E1 with the reference's rounding (no FMA contraction), so stored
+// tables stay bit-identical to src/MultimodePropagation_CPU.cpp:41.
+__device__ __forceinline__ cd cmul_exact(cd a, cd b) {
+  return {__dsub_rn(__dmul_rn(a.re, b.re), __dmul_rn(a.im, b.im)),
+          __dadd_rn(__dmul_rn(a.re, b.im), __dmul_rn(a.im, b.re))};
+}
+
 template <int L>
 __global__ void __launch_bounds__(64) k_row_head(Geometry g, const cd* __restrict__ rowtw, HeadArgs a) {
   using C = RowCfg<L>;
@@ -54,59 +71,67 @@ __global__ void __launch_bounds__(64) k_row_head(Geometry g, const cd* __restric
   const int gl = tid / C::T, t = tid % C::T;
   const int row = blockIdx.x * C::G + gl;
   const bool valid = row < g.M * g.N1;
-  const int p = valid ? row / g.N1 : 0;
-  const int k1 = valid ? row % g.N1 : 0;
-  const size_t base = (size_t)p * g.N + (size_t)k1 * g.N2;
+  const size_t base = valid ? (size_t)row * g.N2 + t : t;
   cd* sm = smem + gl * C::SLOTS;
-  const double z = a.st->z;
   const size_t MN = (size_t)g.M * g.N;
+  LineTw<L> lt;
+  load_line_tw<L>(lt, rowtw, t);
 
-  cd v[8];
+  // issue every load this lane needs before the first barrier
+  cd v[8], psi[8], accin[8], ep[8], ec[8], e1[8];
+  const cd zero = {0., 0.};
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const size_t idx = base + i * C::T;
+    v[i] = (valid && a.has_prev) ? a.ysig[idx] : zero;
+    ep[i] = (valid && a.has_prev) ? a.e_prev[idx] : zero;
+    e1[i] = (valid && a.update_prev) ? a.e1[idx] : zero;
+    psi[i] = (valid && (a.prev_is_first || !a.final_stage)) ? a.psi_hat[idx] : zero;
+    accin[i] = (valid && a.has_prev && !a.prev_is_first) ? a.acc[idx] : zero;
+    ec[i] = (valid && !a.final_stage) ? a.e_cur[idx] : zero;
+  }
   cd nh[8];
-#pragma unroll
-  for (int i = 0; i < 8; i++) nh[i] = {0., 0.};
   if (a.has_prev) {
+    line_fft<L, -1, 1, 1>(v, sm, lt, t);
 #pragma unroll
-    for (int i = 0; i < 8; i++) v[i] = valid ? a.ysig[base + t + i * C::T] : cd{0., 0.};
-    line_fft<L, -1, 1, 1>(v, sm, rowtw, t);
+    for (int i = 0; i < 8; i++) {
+      const cd x = cmulc(v[i], ep[i]);             // sigma_hat * conj(E_{i-1})
+      nh[i] = {-x.im * a.sgn, x.re * a.sgn};       // * (i sgn)
+      cd acc = a.prev_is_first ? psi[i] : accin[i];
+      if (a.bh_prev != 0.) acc = cadd(acc, cscale(nh[i], a.bh_prev));
+      accin[i] = acc;
+    }
     if (valid) {
-      const double tau = z + a.ch_prev;
 #pragma unroll
       for (int i = 0; i < 8; i++) {
-        const size_t idx = base + t + i * C::T;
-        const cd E = phase(a.imL[idx], tau);
-        const cd x = cmulc(v[i], E);                 // sigma_hat * conj(E)
-        const cd n = {-x.im * a.sgn, x.re * a.sgn};  // * (i sgn)
-        nh[i] = n;
-        cd acc = a.prev_is_first ? a.psi_hat[idx] : a.acc[idx];
-        if (a.bh_prev != 0.) acc = cadd(acc, cscale(n, a.bh_prev));
-        if (a.final_stage) a.psi_hat[idx] = acc;
-        else a.acc[idx] = acc;
-        if (a.stash_prev) a.stash[(size_t)a.prev_stage * MN + idx] = n;
+        const size_t idx = base + i * C::T;
+        if (a.final_stage) a.psi_hat[idx] = accin[i];
+        else a.acc[idx] = accin[i];
+        if (a.stash_prev) a.stash[(size_t)a.prev_stage * MN + idx] = nh[i];
+        if (a.update_prev) a.e_prev[idx] = cmul_exact(ep[i], e1[i]);
       }
     }
   }
   if (a.final_stage) return;
-  const double tau = z + a.ch_cur;
 #pragma unroll
   for (int i = 0; i < 8; i++) {
-    const size_t idx = base + t + i * C::T;
-    if (!valid) {
-      v[i] = {0., 0.};
-      continue;
-    }
-    cd psi = a.psi_hat[idx];
-    for (int e = 0; e < a.n_a; e++) {
+    const size_t idx = base + i * C::T;
+    cd x = psi[i];
+#pragma unroll
+    for (int e = 0; e < kMaxStages; e++) {
+      if (e >= a.n_a) break;
       const int j = a.a_idx[e];
-      const cd nj = (j == a.prev_stage && a.has_prev) ? nh[i] : a.stash[(size_t)j * MN + idx];
-      psi = cadd(psi, cscale(nj, a.a_h[e]));
+      cd nj;
+      if (j == a.prev_stage) nj = nh[i];
+      else nj = valid ? a.stash[(size_t)j * MN + idx] : zero;
+      x = cadd(x, cscale(nj, a.a_h[e]));
     }
-    v[i] = cmul(phase(a.imL[idx], tau), psi);
+    v[i] = cscale(cmul(ec[i], x), a.scale_cur);
   }
-  line_fft<L, 1, 1, 1>(v, sm, rowtw, t);
+  line_fft<L, 1, 1, 1>(v, sm, lt, t);
   if (valid) {
 #pragma unroll
-    for (int i = 0; i < 8; i++) a.yout[base + t + i * C::T] = v[i];
+    for (int i = 0; i < 8; i++) a.yout[base + i * C::T] = v[i];
   }
 }
 
@@ -114,36 +139,40 @@ __global__ void __launch_bounds__(64) k_row_head(Geometry g, const cd* __restric
 // column passes
 // ---------------------------------------------------------------------------
 struct ColIdx {
-  int c, t, gl;
+  int c, t, gl, cb, lg;
 };
 
-template <int L>
-__device__ __forceinline__ ColIdx col_index() {
-  using C = ColCfg<L>;
+// lane -> (column c, line lane t, line gl) and block -> (column block cb, line group lg)
+template <int L, int CB>
+__device__ __forceinline__ ColIdx col_index(int ncb) {
+  using C = ColCfg<L, CB>;
   const int tid = threadIdx.x;
-  return {tid % kCB, (tid / kCB) % C::T, tid / (kCB * C::T)};
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  return {tid % CB, (tid / CB) % C::T, tid / (CB * C::T), lb % ncb, lb / ncb};
 }
 
 // inverse column DFT: in = spectral-side rows (k1), out = time domain, times scale, divided by divisor
-template <int L>
+template <int L, int CB>
 __global__ void __launch_bounds__(256) k_col_inverse(Geometry g, const cd* __restrict__ coltw,
                                                      const cd* __restrict__ fourtw, const cd* in, cd* out, int lines,
                                                      double scale, double divisor) {
-  using C = ColCfg<L>;
-  __shared__ cd smem[C::G * L * kCB];
-  const ColIdx ix = col_index<L>();
-  const int q = blockIdx.y * C::G + ix.gl;
-  const int n2 = blockIdx.x * kCB + ix.c;
+  using C = ColCfg<L, CB>;
+  __shared__ cd smem[C::G * C::SLOTS * CB];
+  const ColIdx ix = col_index<L, CB>(g.N2 / CB);
+  const int q = ix.lg * C::G + ix.gl;
+  const int n2 = ix.cb * CB + ix.c;
   const bool valid = q < lines;
   const size_t base = (size_t)(valid ? q : 0) * g.N + n2;
-  cd* sm = smem + ix.gl * L * kCB + ix.c;
+  cd* sm = smem + ix.gl * C::SLOTS * CB + ix.c;
+  LineTw<L> lt;
+  load_line_tw<L>(lt, coltw, ix.t);
   cd v[8];
 #pragma unroll
   for (int i = 0; i < 8; i++) {
     const int k1 = ix.t + i * C::T;
     v[i] = valid ? cmulc(in[base + (size_t)k1 * g.N2], fourtw[n2 * k1]) : cd{0., 0.};
   }
-  line_fft<L, 1, kCB, 0>(v, sm, coltw, ix.t);
+  line_fft<L, 1, CB, C::PAD>(v, sm, lt, ix.t);
   if (valid) {
 #pragma unroll
     for (int i = 0; i < 8; i++) {
@@ -156,51 +185,56 @@ __global__ void __launch_bounds__(256) k_col_inverse(Geometry g, const cd* __res
 }
 
 // forward column DFT + four-step twiddle: in = time domain, out = spectral-side rows
-template <int L>
+template <int L, int CB>
 __global__ void __launch_bounds__(256) k_col_forward(Geometry g, const cd* __restrict__ coltw,
                                                      const cd* __restrict__ fourtw, const cd* in, cd* out, int lines) {
-  using C = ColCfg<L>;
-  __shared__ cd smem[C::G * L * kCB];
-  const ColIdx ix = col_index<L>();
-  const int q = blockIdx.y * C::G + ix.gl;
-  const int n2 = blockIdx.x * kCB + ix.c;
+  using C = ColCfg<L, CB>;
+  __shared__ cd smem[C::G * C::SLOTS * CB];
+  const ColIdx ix = col_index<L, CB>(g.N2 / CB);
+  const int q = ix.lg * C::G + ix.gl;
+  const int n2 = ix.cb * CB + ix.c;
   const bool valid = q < lines;
   const size_t base = (size_t)(valid ? q : 0) * g.N + n2;
-  cd* sm = smem + ix.gl * L * kCB + ix.c;
-  cd v[8];
+  cd* sm = smem + ix.gl * C::SLOTS * CB + ix.c;
+  LineTw<L> lt;
+  load_line_tw<L>(lt, coltw, ix.t);
+  cd v[8], w4[8];
 #pragma unroll
   for (int i = 0; i < 8; i++) {
     const int n1 = ix.t + i * C::T;
     v[i] = valid ? in[base + (size_t)n1 * g.N2] : cd{0., 0.};
+    w4[i] = fourtw[n2 * n1];  // index k1 = n1 after the transform
   }
-  line_fft<L, -1, kCB, 0>(v, sm, coltw, ix.t);
+  line_fft<L, -1, CB, C::PAD>(v, sm, lt, ix.t);
   if (valid) {
 #pragma unroll
     for (int i = 0; i < 8; i++) {
       const int k1 = ix.t + i * C::T;
-      out[base + (size_t)k1 * g.N2] = cmul(v[i], fourtw[n2 * k1]);
+      out[base + (size_t)k1 * g.N2] = cmul(v[i], w4[i]);
     }
   }
 }
 
 // forward column DFT of one real line (the Raman response)
-template <int L>
+template <int L, int CB>
 __global__ void __launch_bounds__(256) k_col_forward_real(Geometry g, const cd* __restrict__ coltw,
                                                           const cd* __restrict__ fourtw, const double* __restrict__ in,
                                                           cd* __restrict__ out) {
-  using C = ColCfg<L>;
-  __shared__ cd smem[C::G * L * kCB];
-  const ColIdx ix = col_index<L>();
-  const bool valid = blockIdx.y * C::G + ix.gl < 1;
-  const int n2 = blockIdx.x * kCB + ix.c;
-  cd* sm = smem + ix.gl * L * kCB + ix.c;
+  using C = ColCfg<L, CB>;
+  __shared__ cd smem[C::G * C::SLOTS * CB];
+  const ColIdx ix = col_index<L, CB>(g.N2 / CB);
+  const bool valid = ix.lg * C::G + ix.gl < 1;
+  const int n2 = ix.cb * CB + ix.c;
+  cd* sm = smem + ix.gl * C::SLOTS * CB + ix.c;
+  LineTw<L> lt;
+  load_line_tw<L>(lt, coltw, ix.t);
   cd v[8];
 #pragma unroll
   for (int i = 0; i < 8; i++) {
     const int n1 = ix.t + i * C::T;
     v[i] = valid ? cd{in[(size_t)n1 * g.N2 + n2], 0.} : cd{0., 0.};
   }
-  line_fft<L, -1, kCB, 0>(v, sm, coltw, ix.t);
+  line_fft<L, -1, CB, C::PAD>(v, sm, lt, ix.t);
   if (valid) {
 #pragma unroll
     for (int i = 0; i < 8; i++) {
@@ -211,20 +245,22 @@ __global__ void __launch_bounds__(256) k_col_forward_real(Geometry g, const cd* 
 }
 
 // pair products of the time-domain fields U, then forward column DFT
-template <int L>
+template <int L, int CB>
 __global__ void __launch_bounds__(256) k_col_pairs_forward(Geometry g, const cd* __restrict__ coltw,
                                                            const cd* __restrict__ fourtw, const cd* __restrict__ U,
                                                            cd* __restrict__ out, int lines, PairLineArgs pl) {
-  using C = ColCfg<L>;
-  __shared__ cd smem[C::G * L * kCB];
-  const ColIdx ix = col_index<L>();
-  const int q = blockIdx.y * C::G + ix.gl;
-  const int n2 = blockIdx.x * kCB + ix.c;
+  using C = ColCfg<L, CB>;
+  __shared__ cd smem[C::G * C::SLOTS * CB];
+  const ColIdx ix = col_index<L, CB>(g.N2 / CB);
+  const int q = ix.lg * C::G + ix.gl;
+  const int n2 = ix.cb * CB + ix.c;
   const bool valid = q < lines;
   const int qq = valid ? q : 0;
   const uint32_t la = pl.line_a[qq], lb = pl.line_b[qq], self = pl.line_self[qq];
-  cd* sm = smem + ix.gl * L * kCB + ix.c;
-  cd v[8];
+  cd* sm = smem + ix.gl * C::SLOTS * CB + ix.c;
+  LineTw<L> lt;
+  load_line_tw<L>(lt, coltw, ix.t);
+  cd v[8], w4[8];
 #pragma unroll
   for (int i = 0; i < 8; i++) {
     const int n1 = ix.t + i * C::T;
@@ -243,13 +279,14 @@ __global__ void __launch_bounds__(256) k_col_pairs_forward(Geometry g, const cd*
       }
     }
     v[i] = x;
+    w4[i] = fourtw[n2 * n1];
   }
-  line_fft<L, -1, kCB, 0>(v, sm, coltw, ix.t);
+  line_fft<L, -1, CB, C::PAD>(v, sm, lt, ix.t);
   if (valid) {
 #pragma unroll
     for (int i = 0; i < 8; i++) {
       const int k1 = ix.t + i * C::T;
-      out[(size_t)q * g.N + (size_t)k1 * g.N2 + n2] = cmul(v[i], fourtw[n2 * k1]);
+      out[(size_t)q * g.N + (size_t)k1 * g.N2 + n2] = cmul(v[i], w4[i]);
     }
   }
 }
@@ -268,10 +305,12 @@ __global__ void __launch_bounds__(64) k_row_forward(Geometry g, const cd* __rest
   const bool valid = row < lines * g.N1;
   const size_t base = valid ? (size_t)row * g.N2 : 0;  // rows of consecutive lines are contiguous
   cd* sm = smem + gl * C::SLOTS;
+  LineTw<L> lt;
+  load_line_tw<L>(lt, rowtw, t);
   cd v[8];
 #pragma unroll
   for (int i = 0; i < 8; i++) v[i] = valid ? data[base + t + i * C::T] : cd{0., 0.};
-  line_fft<L, -1, 1, 1>(v, sm, rowtw, t);
+  line_fft<L, -1, 1, 1>(v, sm, lt, t);
   if (valid) {
 #pragma unroll
     for (int i = 0; i < 8; i++) data[base + t + i * C::T] = v[i];
@@ -293,6 +332,8 @@ __global__ void __launch_bounds__(64) k_row_inverse_phase(Geometry g, const cd* 
   const size_t base = valid ? (size_t)row * g.N2 : 0;
   cd* sm = smem + gl * C::SLOTS;
   const double z = st->z;
+  LineTw<L> lt;
+  load_line_tw<L>(lt, rowtw, t);
   cd v[8];
 #pragma unroll
   for (int i = 0; i < 8; i++) {
@@ -301,7 +342,7 @@ __global__ void __launch_bounds__(64) k_row_inverse_phase(Geometry g, const cd* 
     if (use_phase && valid) x = cmul(x, phase(imL[idx], z));
     v[i] = x;
   }
-  line_fft<L, 1, 1, 1>(v, sm, rowtw, t);
+  line_fft<L, 1, 1, 1>(v, sm, lt, t);
   if (valid) {
 #pragma unroll
     for (int i = 0; i < 8; i++) out[base + t + i * C::T] = v[i];
@@ -321,13 +362,18 @@ __global__ void __launch_bounds__(64) k_row_raman(Geometry g, const cd* __restri
   const size_t base = valid ? (size_t)row * g.N2 : 0;
   const size_t hbase = valid ? (size_t)(row % g.N1) * g.N2 : 0;
   cd* sm = smem + gl * C::SLOTS;
-  cd v[8];
+  LineTw<L> lt;
+  load_line_tw<L>(lt, rowtw, t);
+  cd v[8], hs[8];
 #pragma unroll
-  for (int i = 0; i < 8; i++) v[i] = valid ? data[base + t + i * C::T] : cd{0., 0.};
-  line_fft<L, -1, 1, 1>(v, sm, rowtw, t);
+  for (int i = 0; i < 8; i++) {
+    v[i] = valid ? data[base + t + i * C::T] : cd{0., 0.};
+    hs[i] = hspec[hbase + t + i * C::T];
+  }
+  line_fft<L, -1, 1, 1>(v, sm, lt, t);
 #pragma unroll
-  for (int i = 0; i < 8; i++) v[i] = cscale(cmul(v[i], hspec[hbase + t + i * C::T]), hscale);
-  line_fft<L, 1, 1, 1>(v, sm, rowtw, t);
+  for (int i = 0; i < 8; i++) v[i] = cscale(cmul(v[i], hs[i]), hscale);
+  line_fft<L, 1, 1, 1>(v, sm, lt, t);
   if (valid) {
 #pragma unroll
     for (int i = 0; i < 8; i++) data[base + t + i * C::T] = v[i];
@@ -335,52 +381,104 @@ __global__ void __launch_bounds__(64) k_row_raman(Geometry g, const cd* __restri
 }
 
 // ---------------------------------------------------------------------------
-// Kerr + Raman gather over a tile of TS consecutive samples:
+// Kerr + Raman gather over a tile of 64 consecutive samples:
 //   X_slot = kv * U_a conj(U_b) + kr * VR_slot        (slot mix, into LDS)
 //   sigma_p = sum over the nonzeros (k, slot, q) of Q^(p):  q * X_slot * U_k
-// U and X live in LDS; each (sample, mode) lane accumulates its own sum.
+// One wavefront per mode p (lanes = samples), so the nonzero loop and its
+// metadata loads are uniform; nonzeros are sorted by slot so consecutive ones
+// reuse X. Slots are staged through LDS in chunks when M + P exceeds the tile.
 // ---------------------------------------------------------------------------
+constexpr int kGatherTS = 64;
+constexpr int kGatherMaxPW = 8;  // modes per wavefront
+
+void gather_plan(int M, int P, int& cap, int& chunk) {
+  cap = 0;
+  chunk = 0;
+  const int caps[2] = {4096, 8192};
+  for (int c : caps) {
+    const int room = c / kGatherTS - M;
+    if (room <= 0) continue;
+    if (room >= P || c == 8192) {
+      cap = c;
+      chunk = room < P ? room : P;
+      return;
+    }
+  }
+}
+
 template <int CAP>
-__global__ void __launch_bounds__(256) k_gather(Geometry g, const cd* __restrict__ U, const cd* __restrict__ VR,
-                                                cd* __restrict__ sigma, GatherArgs a, int TS) {
+__global__ void __launch_bounds__(512) k_gather(Geometry g, const cd* __restrict__ U, const cd* __restrict__ VR,
+                                                cd* __restrict__ sigma, GatherArgs a) {
   __shared__ cd sh[CAP];
-  const int M = g.M, P = a.n_slots;
+  constexpr int TS = kGatherTS;
+  const int M = g.M;
   cd* Ush = sh;            // [M][TS]
-  cd* Xsh = sh + M * TS;   // [P][TS]
+  cd* Xsh = sh + M * TS;   // [chunk][TS]
   const int t0 = blockIdx.x * TS;
-  const int tid = threadIdx.x;
-  for (int e = tid; e < M * TS; e += 256) {
-    const int k = e / TS, j = e % TS;
-    Ush[e] = U[(size_t)k * g.N + t0 + j];
-  }
-  __syncthreads();
-  for (int e = tid; e < P * TS; e += 256) {
-    const int s = e / TS, j = e % TS;
-    const cd V = cmulc(Ush[a.slot_a[s] * TS + j], Ush[a.slot_b[s] * TS + j]);
-    cd X = cscale(V, a.kv);
-    if (a.raman) {
-      const cd r = VR[(size_t)a.slot_line[s] * g.N + t0 + j];
-      cd vr;
-      switch (a.slot_kind[s]) {
-        case kDirect: vr = r; break;
-        case kConj: vr = cconj(r); break;
-        case kRe: vr = {r.re, 0.}; break;
-        default: vr = {r.im, 0.}; break;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int lane = tid % TS, wave = tid / TS, W = nthr / TS;
+  for (int e = tid; e < M * TS; e += nthr) Ush[e] = U[(size_t)(e / TS) * g.N + t0 + e % TS];
+  cd acc[kGatherMaxPW];
+#pragma unroll
+  for (int r = 0; r < kGatherMaxPW; r++) acc[r] = {0., 0.};
+  for (int c = 0; c < a.n_chunks; c++) {
+    const int s0 = c * a.chunk;
+    const int ns = min(a.chunk, a.n_slots - s0);
+    __syncthreads();  // Ush loaded / previous chunk consumed
+    for (int e = tid; e < ns * TS; e += nthr) {
+      const int s = s0 + e / TS, j = e % TS;
+      const cd V = cmulc(Ush[a.slot_a[s] * TS + j], Ush[a.slot_b[s] * TS + j]);
+      cd X = cscale(V, a.kv);
+      if (a.raman) {
+        const cd r = VR[(size_t)a.slot_line[s] * g.N + t0 + j];
+        cd vr;
+        switch (a.slot_kind[s]) {
+          case kDirect: vr = r; break;
+          case kConj: vr = cconj(r); break;
+          case kRe: vr = {r.re, 0.}; break;
+          default: vr = {r.im, 0.}; break;
+        }
+        X = cadd(X, cscale(vr, a.kr));
       }
-      X = cadd(X, cscale(vr, a.kr));
+      Xsh[e] = X;
     }
-    Xsh[e] = X;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kGatherMaxPW; r++) {
+      const int p = wave + r * W;
+      if (p >= M) break;
+      const uint32_t* co = a.coff + (size_t)p * (a.n_chunks + 1) + c;
+      const uint32_t e0 = co[0], e1 = co[1];
+      cd sum = acc[r];
+      // 64 nonzeros at a time: lane l holds nonzero eb + l; the loop broadcasts them with
+      // v_readlane so no memory access sits on the loop's dependency chain
+      for (uint32_t eb = e0; eb < e1; eb += TS) {
+        const uint32_t mine = eb + lane;
+        const uint32_t ks = mine < e1 ? a.kslot[mine] : 0u;
+        const double qv = mine < e1 ? a.q[mine] : 0.;
+        const int qlo = __double2loint(qv), qhi = __double2hiint(qv);
+        const int cnt = (int)min((uint32_t)TS, e1 - eb);
+        int cur = -1;
+        cd X = {0., 0.};
+        for (int j = 0; j < cnt; j++) {
+          const uint32_t ksj = (uint32_t)__builtin_amdgcn_readlane((int)ks, j);
+          const double qj = __hiloint2double(__builtin_amdgcn_readlane(qhi, j), __builtin_amdgcn_readlane(qlo, j));
+          const int sl = (int)(ksj >> 8) - s0;
+          if (sl != cur) {
+            X = Xsh[sl * TS + lane];
+            cur = sl;
+          }
+          sum = cadd(sum, cscale(cmul(X, Ush[(ksj & 255u) * TS + lane]), qj));
+        }
+      }
+      acc[r] = sum;
+    }
   }
-  __syncthreads();
-  const int j = tid % TS;
-  for (int p = tid / TS; p < M; p += 256 / TS) {
-    cd acc = {0., 0.};
-    const uint32_t e1 = a.off[p + 1];
-    for (uint32_t e = a.off[p]; e < e1; e++) {
-      const cd W = cmul(Xsh[a.slot[e] * TS + j], Ush[a.k[e] * TS + j]);
-      acc = cadd(acc, cscale(W, a.q[e]));
-    }
-    sigma[(size_t)p * g.N + t0 + j] = acc;
+#pragma unroll
+  for (int r = 0; r < kGatherMaxPW; r++) {
+    const int p = wave + r * W;
+    if (p >= M) break;
+    sigma[(size_t)p * g.N + t0 + lane] = acc[r];
   }
 }
 
@@ -436,14 +534,40 @@ hipError_t launch_row_head(const Geometry& g, const Twiddles& tw, const HeadArgs
   MMP_DISPATCH_L(g.N2, row_head_impl, g, tw, a, s);
 }
 
+// Column-block width: 8 when that already gives >= 256 workgroups, else 4, else 2.
+template <int L>
+static int pick_cb(const Geometry& g, int lines) {
+  const int cbs[3] = {8, 4, 2};
+  for (int cb : cbs) {
+    const int T = L / 8;
+    const int G = (cb * T >= 64) ? 1 : 64 / (cb * T);
+    const int blocks = (g.N2 / cb) * ((lines + G - 1) / G);
+    if (blocks >= 256 || cb == 2) return cb;
+  }
+  return 2;
+}
+
+template <int L, int CB>
+static dim3 col_grid(const Geometry& g, int lines) {
+  using C = ColCfg<L, CB>;
+  return dim3((g.N2 / CB) * ((lines + C::G - 1) / C::G));
+}
+
+#define MMP_COL_LAUNCH(KERNEL, LINES, ...)                                                                      \
+  switch (pick_cb<L>(g, LINES)) {                                                                               \
+    case 8: hipLaunchKernelGGL((KERNEL<L, 8>), (col_grid<L, 8>(g, LINES)), (dim3(ColCfg<L, 8>::THREADS)), 0, s,    \
+                               __VA_ARGS__); break;                                                            \
+    case 4: hipLaunchKernelGGL((KERNEL<L, 4>), (col_grid<L, 4>(g, LINES)), (dim3(ColCfg<L, 4>::THREADS)), 0, s,    \
+                               __VA_ARGS__); break;                                                            \
+    default: hipLaunchKernelGGL((KERNEL<L, 2>), (col_grid<L, 2>(g, LINES)), (dim3(ColCfg<L, 2>::THREADS)), 0, s,   \
+                                __VA_ARGS__); break;                                                           \
+  }                                                                                                             \
+  return hipGetLastError();
+
 template <int L>
 static hipError_t col_inverse_impl(const Geometry& g, const Twiddles& tw, const cd* in, cd* out, int lines,
                                    double scale, double divisor, hipStream_t s) {
-  using C = ColCfg<L>;
-  dim3 grid(g.N2 / kCB, (lines + C::G - 1) / C::G);
-  hipLaunchKernelGGL(k_col_inverse<L>, grid, dim3(C::THREADS), 0, s, g, tw.coltw, tw.fourtw, in, out, lines, scale,
-                     divisor);
-  return hipGetLastError();
+  MMP_COL_LAUNCH(k_col_inverse, lines, g, tw.coltw, tw.fourtw, in, out, lines, scale, divisor)
 }
 hipError_t launch_col_inverse(const Geometry& g, const Twiddles& tw, const cd* in, cd* out, int lines, double scale,
                               double divisor, hipStream_t s) {
@@ -453,10 +577,7 @@ hipError_t launch_col_inverse(const Geometry& g, const Twiddles& tw, const cd* i
 template <int L>
 static hipError_t col_forward_impl(const Geometry& g, const Twiddles& tw, const cd* in, cd* out, int lines,
                                    hipStream_t s) {
-  using C = ColCfg<L>;
-  dim3 grid(g.N2 / kCB, (lines + C::G - 1) / C::G);
-  hipLaunchKernelGGL(k_col_forward<L>, grid, dim3(C::THREADS), 0, s, g, tw.coltw, tw.fourtw, in, out, lines);
-  return hipGetLastError();
+  MMP_COL_LAUNCH(k_col_forward, lines, g, tw.coltw, tw.fourtw, in, out, lines)
 }
 hipError_t launch_col_forward(const Geometry& g, const Twiddles& tw, const cd* in, cd* out, int lines,
                               hipStream_t s) {
@@ -466,10 +587,7 @@ hipError_t launch_col_forward(const Geometry& g, const Twiddles& tw, const cd* i
 template <int L>
 static hipError_t col_forward_real_impl(const Geometry& g, const Twiddles& tw, const double* in, cd* out,
                                         hipStream_t s) {
-  using C = ColCfg<L>;
-  dim3 grid(g.N2 / kCB, 1);
-  hipLaunchKernelGGL(k_col_forward_real<L>, grid, dim3(C::THREADS), 0, s, g, tw.coltw, tw.fourtw, in, out);
-  return hipGetLastError();
+  MMP_COL_LAUNCH(k_col_forward_real, 1, g, tw.coltw, tw.fourtw, in, out)
 }
 hipError_t launch_col_forward_real(const Geometry& g, const Twiddles& tw, const double* in, cd* out,
                                    hipStream_t s) {
@@ -479,11 +597,7 @@ hipError_t launch_col_forward_real(const Geometry& g, const Twiddles& tw, const 
 template <int L>
 static hipError_t col_pairs_forward_impl(const Geometry& g, const Twiddles& tw, const cd* U, cd* out, int lines,
                                          const PairLineArgs& pl, hipStream_t s) {
-  using C = ColCfg<L>;
-  dim3 grid(g.N2 / kCB, (lines + C::G - 1) / C::G);
-  hipLaunchKernelGGL(k_col_pairs_forward<L>, grid, dim3(C::THREADS), 0, s, g, tw.coltw, tw.fourtw, U, out, lines,
-                     pl);
-  return hipGetLastError();
+  MMP_COL_LAUNCH(k_col_pairs_forward, lines, g, tw.coltw, tw.fourtw, U, out, lines, pl)
 }
 hipError_t launch_col_pairs_forward(const Geometry& g, const Twiddles& tw, const cd* U, cd* out, int lines,
                                     const PairLineArgs& pl, hipStream_t s) {
@@ -530,29 +644,16 @@ hipError_t launch_row_raman(const Geometry& g, const Twiddles& tw, cd* data, int
   MMP_DISPATCH_L(g.N2, row_raman_impl, g, tw, data, lines, hspec, hscale, s);
 }
 
-// Largest sample tile (a power of two dividing N, at most 64, keeping N / TS >= 256
-// workgroups when possible) whose U and slot tiles fit the LDS capacity.
-int gather_tile(const Geometry& g, int n_slots, int cap) {
-  int ts = 64;
-  while (ts > 8 && (g.N / ts < 256 || (g.M + n_slots) * ts > cap)) ts >>= 1;
-  return ((g.M + n_slots) * ts <= cap) ? ts : 0;
-}
-
 hipError_t launch_gather(const Geometry& g, const cd* U, const cd* VR, cd* sigma, const GatherArgs& a,
                          hipStream_t s) {
-  const int caps[3] = {2048, 4096, 8192};
-  for (int ci = 0; ci < 3; ci++) {
-    const int ts = gather_tile(g, a.n_slots, caps[ci]);
-    if (!ts || g.N % ts) continue;
-    dim3 grid(g.N / ts);
-    switch (ci) {
-      case 0: hipLaunchKernelGGL(k_gather<2048>, grid, dim3(256), 0, s, g, U, VR, sigma, a, ts); break;
-      case 1: hipLaunchKernelGGL(k_gather<4096>, grid, dim3(256), 0, s, g, U, VR, sigma, a, ts); break;
-      default: hipLaunchKernelGGL(k_gather<8192>, grid, dim3(256), 0, s, g, U, VR, sigma, a, ts); break;
-    }
-    return hipGetLastError();
-  }
-  return hipErrorInvalidValue;
+  if (g.N % kGatherTS) return hipErrorInvalidValue;
+  int waves = g.M < 8 ? g.M : 8;
+  if ((g.M + waves - 1) / waves > kGatherMaxPW) return hipErrorInvalidValue;
+  dim3 grid(g.N / kGatherTS), block(kGatherTS * waves);
+  if (a.cap == 4096) hipLaunchKernelGGL(k_gather<4096>, grid, block, 0, s, g, U, VR, sigma, a);
+  else if (a.cap == 8192) hipLaunchKernelGGL(k_gather<8192>, grid, block, 0, s, g, U, VR, sigma, a);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
 }
 
 hipError_t launch_advance_z(StepState* st, double h, hipStream_t s) {

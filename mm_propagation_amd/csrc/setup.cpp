@@ -10,10 +10,25 @@
 
 // Keep a*b+c as two roundings, as the reference's x86-64 build does.
 #pragma clang fp contract(off)
+#include <algorithm>
 #include <complex>
 #include <cstring>
 
+#include "kernels.h"
+
+// glibc's cexp: the routine std::exp(std::complex<double>) resolves to in the reference's
+// g++ build (ComplexArray.cpp exp()), called directly so the Lawson tables match it bit for bit.
+extern "C" double _Complex cexp(double _Complex z);
+
 namespace mmp {
+
+static std::complex<double> exp_c99(std::complex<double> z) {
+  double _Complex c;
+  __real__ c = z.real();
+  __imag__ c = z.imag();
+  const double _Complex r = cexp(c);
+  return {__real__ r, __imag__ r};
+}
 
 Tableau make_tableau(uint32_t method_order) {
   Tableau t;
@@ -199,8 +214,22 @@ mmp_status prepare_problem(const mmp_params* in, const mmp_options& opt, Problem
   P.psi0.resize(2 * (size_t)M * nt);
   for (size_t j = 0; j < 2 * (size_t)M * nt; j++) P.psi0[j] = in->initial_fields[j] * P.conversion_factor;
 
-  // Im L_p(alpha_j) (src/MultimodePropagation.cpp:165-187), stored in four-step order.
+  // Distinct stage nodes c_i, one Lawson table each.
+  for (int i = 0; i < P.rk.s; i++) {
+    int u = 0;
+    while (u < (int)P.c_unique.size() && P.c_unique[u] != P.rk.c[i]) u++;
+    if (u == (int)P.c_unique.size()) P.c_unique.push_back(P.rk.c[i]);
+    P.stage_table.push_back(u);
+  }
+  P.table_last_head.assign(P.c_unique.size(), 0);
+  for (int i = 0; i < P.rk.s; i++) P.table_last_head[P.stage_table[i]] = i + 1;
+  const size_t n_tab = P.c_unique.size();
+
+  // L_p(alpha_j) (src/MultimodePropagation.cpp:165-187) and the Lawson tables
+  // E1 = exp(L h), E_u = exp(L (c_u h)) (:190-199), stored in four-step order.
   P.imL_fourstep.resize((size_t)M * nt);
+  P.etab0.resize(2 * n_tab * (size_t)M * nt);
+  P.e1.resize(2 * (size_t)M * nt);
   const std::complex<double> iu(0., 1.);
   for (uint32_t p = 0; p < M; p++) {
     const double* bt = &P.beta_tilde[(size_t)p * nb];
@@ -211,7 +240,16 @@ mmp_status prepare_problem(const mmp_params* in, const mmp_options& opt, Problem
       for (uint32_t o = 3; o < nb; o++) L += iu * (std::pow(-1, o) * bt[o] / factorial_int((int)o) * std::pow(alpha, o));
       // natural index j = k1 + N1*k2  ->  four-step slot k1*N2 + k2
       const uint32_t k1 = j % P.N1, k2 = j / P.N1;
-      P.imL_fourstep[(size_t)p * nt + (size_t)k1 * P.N2 + k2] = L.imag();
+      const size_t slot4 = (size_t)p * nt + (size_t)k1 * P.N2 + k2;
+      P.imL_fourstep[slot4] = L.imag();
+      const std::complex<double> e1 = exp_c99(L * P.delta_z);
+      P.e1[2 * slot4] = e1.real();
+      P.e1[2 * slot4 + 1] = e1.imag();
+      for (size_t u = 0; u < n_tab; u++) {
+        const std::complex<double> eu = exp_c99(L * (P.c_unique[u] * P.delta_z));
+        P.etab0[2 * (u * M * nt + slot4)] = eu.real();
+        P.etab0[2 * (u * M * nt + slot4) + 1] = eu.imag();
+      }
     }
   }
 
@@ -279,12 +317,40 @@ mmp_status prepare_problem(const mmp_params* in, const mmp_options& opt, Problem
   }
   // Kerr gather: W = Q * (kv * V + kr * VR) * U_k with (kv, kr) = (1 - fR, fR T0) when Raman is
   // on (src/MultimodePropagation_CPU.cpp:223-226) and (1, 0) when off (:152-154).
+  // Within each Q^(p) the nonzeros are sorted by (slot, k) so the device reuses a slot's mix
+  // across its k's; this reorders the sum over nonzeros (the reference sums in insertion order).
   P.off.assign(in->coupling_offsets, in->coupling_offsets + M + 1);
   P.k.resize(nnz);
   P.q.resize(nnz);
-  for (uint32_t e = 0; e < nnz; e++) {
-    P.k[e] = in->coupling_klm[3 * e];
-    P.q[e] = in->coupling_values[e] / max_q;
+  std::vector<uint32_t> slot_in = P.slot;
+  for (uint32_t p = 0; p < M; p++) {
+    std::vector<uint32_t> idx;
+    for (uint32_t e = P.off[p]; e < P.off[p + 1]; e++) idx.push_back(e);
+    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) {
+      if (slot_in[x] != slot_in[y]) return slot_in[x] < slot_in[y];
+      return in->coupling_klm[3 * x] < in->coupling_klm[3 * y];
+    });
+    for (size_t r = 0; r < idx.size(); r++) {
+      const uint32_t e = P.off[p] + (uint32_t)r, src = idx[r];
+      P.k[e] = in->coupling_klm[3 * src];
+      P.slot[e] = slot_in[src];
+      P.q[e] = in->coupling_values[src] / max_q;
+    }
+  }
+  gather_plan((int)M, (int)P.P, P.gather_cap, P.gather_chunk);
+  if (!P.gather_cap || P.gather_chunk <= 0) {
+    err = "the Kerr gather tile does not fit (n_modes too large)";
+    return MMP_ERR_UNSUPPORTED;
+  }
+  P.gather_chunks = ((int)P.P + P.gather_chunk - 1) / P.gather_chunk;
+  P.coff.assign((size_t)M * (P.gather_chunks + 1), 0);
+  for (uint32_t p = 0; p < M; p++) {
+    uint32_t e = P.off[p];
+    for (int c = 0; c <= P.gather_chunks; c++) {
+      const uint32_t lim = (uint32_t)std::min<int>((int)P.P, c * P.gather_chunk);
+      while (e < P.off[p + 1] && P.slot[e] < lim) e++;
+      P.coff[(size_t)p * (P.gather_chunks + 1) + c] = (c == P.gather_chunks) ? P.off[p + 1] : e;
+    }
   }
   P.kv = P.raman ? 1 - P.fR : 1.;
   P.kr = P.raman ? P.fR * P.T0 : 0.;

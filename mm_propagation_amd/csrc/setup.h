@@ -36,14 +36,23 @@ struct Problem {
   std::vector<double> beta_tilde;   // [M][n_beta]
   std::vector<double> psi0;         // [M][N] complex, conv * A0 (time domain)
   std::vector<double> imL_fourstep; // [M][N] Im L_p(k), at S[k1*N2+k2] <-> k = k1 + N1*k2
+  // Lawson tables (src/MultimodePropagation.cpp:190-199), four-step order, complex:
+  // one table exp(c_u h L) per distinct c value, and E1 = exp(h L).
+  std::vector<double> c_unique;       // distinct c_i in first-occurrence order
+  std::vector<int> stage_table;       // [s] index into c_unique
+  std::vector<int> table_last_head;   // [n_tables] head (stage i + 1) that last reads the table in a step
+  std::vector<double> etab0;          // [n_tables][M][N] complex
+  std::vector<double> e1;             // [M][N] complex
   std::vector<double> raman_response;  // [N]
   // pair slots (src/MultimodePropagation.cpp:139-155)
   std::vector<uint32_t> pair_codes;  // [P] first-occurrence order of l + m*M
   std::vector<uint32_t> slot_a, slot_b, slot_line, slot_kind;  // [P]
   // Raman convolution lines
   std::vector<uint32_t> line_a, line_b, line_self;  // [n_lines]
-  // Kerr gather (CSR by p, insertion order)
+  // Kerr gather: nonzeros of each Q^(p) sorted by (slot, k), with per-chunk ranges
   std::vector<uint32_t> off, k, slot;  // [M+1], [nnz], [nnz]
+  int gather_cap = 0, gather_chunk = 0, gather_chunks = 0;
+  std::vector<uint32_t> coff;          // [M][gather_chunks + 1]
   std::vector<double> q;               // [nnz] Q / max|Q|
   double kv = 1, kr = 0;               // slot mix: X = kv*V + kr*VR
   // which stage results must be kept for later stages

@@ -75,6 +75,8 @@ struct mmp_solver {
   cd* conv = nullptr;
   cd* stash = nullptr;
   cd* hspec = nullptr;
+  cd* etab = nullptr;   // Lawson tables, one per distinct c_i
+  cd* e1 = nullptr;     // exp(h L)
   double* imL = nullptr;
   double* hR = nullptr;
   double* energy = nullptr;
@@ -239,9 +241,11 @@ mmp_status set_device(mmp_solver* s) {
   return MMP_OK;
 }
 
+// initLawsonRK (src/MultimodePropagation.cpp:158-163): z = 0 and fresh Lawson tables.
 mmp_status reset_z(mmp_solver* s) {
   static const StepState zero{0., 0u, 0u};
   MMP_TRY(upload(s, s->st, &zero, sizeof zero));
+  MMP_TRY(upload(s, s->etab, s->P.etab0.data(), s->P.etab0.size() * sizeof(double)));
   s->z_host = 0.;
   s->steps_taken = 0;
   return MMP_OK;
@@ -288,9 +292,10 @@ size_t planned_device_bytes(const Problem& P) {
   bool any_stash = false;
   for (int j = 0; j < P.rk.s; j++) any_stash = any_stash || P.stash_needed[j];
   size_t b = 16 * (4 * MN) + 8 * MN + 16 * (P.N2 + P.N1 + P.N) + sizeof(StepState) + 8 * c(P.M);
+  b += 16 * MN * (P.c_unique.size() + 1);
   if (any_stash) b += 16 * (size_t)P.rk.s * MN;
   if (P.raman) b += 16 * c(L) * P.N + 16 * (size_t)P.N + 8 * (size_t)P.N;
-  b += 4 * (P.M + 1) + 4 * 2 * c(P.nnz) + 8 * c(P.nnz) + 4 * 4 * c(P.P) + 4 * 3 * c(L);
+  b += 4 * c(P.coff.size()) + 4 * c(P.nnz) + 8 * c(P.nnz) + 4 * 4 * c(P.P) + 4 * 3 * c(L);
   return b;
 }
 
@@ -355,6 +360,8 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
   A(dalloc(s, &s->ybuf, MN));
   A(dalloc(s, &s->U, MN));
   A(dalloc(s, &s->imL, MN));
+  A(dalloc(s, &s->etab, MN * P.c_unique.size()));
+  A(dalloc(s, &s->e1, MN));
   A(dalloc(s, &rowtw, P.N2));
   A(dalloc(s, &coltw, P.N1));
   A(dalloc(s, &fourtw, P.N));
@@ -368,12 +375,11 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
     A(dalloc(s, &s->hspec, P.N));
     A(dalloc(s, &s->hR, P.N));
   }
-  uint32_t *d_off, *d_k, *d_slot, *d_sa, *d_sb, *d_sl, *d_sk, *d_la, *d_lb, *d_ls;
+  uint32_t *d_coff, *d_k, *d_sa, *d_sb, *d_sl, *d_sk, *d_la, *d_lb, *d_ls;
   double* d_q;
   const size_t nnz1 = P.nnz ? P.nnz : 1, P1 = P.P ? P.P : 1, L1 = s->n_lines ? s->n_lines : 1;
-  A(dalloc(s, &d_off, P.M + 1));
+  A(dalloc(s, &d_coff, P.coff.size()));
   A(dalloc(s, &d_k, nnz1));
-  A(dalloc(s, &d_slot, nnz1));
   A(dalloc(s, &d_q, nnz1));
   A(dalloc(s, &d_sa, P1));
   A(dalloc(s, &d_sb, P1));
@@ -383,16 +389,18 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
   A(dalloc(s, &d_lb, L1));
   A(dalloc(s, &d_ls, L1));
   if (st != MMP_OK) return cleanup(st);
-  // tables
+  // tables (host copies stay alive until the stream synchronisation below)
+  std::vector<uint32_t> kslot(P.nnz);
   std::vector<double> tr = twiddle_table(P.N2), tc = twiddle_table(P.N1), t4 = twiddle_table(P.N);
   A(upload(s, rowtw, tr.data(), tr.size() * 8));
   A(upload(s, coltw, tc.data(), tc.size() * 8));
   A(upload(s, fourtw, t4.data(), t4.size() * 8));
   A(upload(s, s->imL, P.imL_fourstep.data(), MN * 8));
-  A(upload(s, d_off, P.off.data(), (P.M + 1) * 4));
+  A(upload(s, d_coff, P.coff.data(), P.coff.size() * 4));
+  A(upload(s, s->e1, P.e1.data(), P.e1.size() * sizeof(double)));
   if (P.nnz) {
-    A(upload(s, d_k, P.k.data(), P.nnz * 4));
-    A(upload(s, d_slot, P.slot.data(), P.nnz * 4));
+    for (uint32_t e = 0; e < P.nnz; e++) kslot[e] = P.k[e] | (P.slot[e] << 8);
+    A(upload(s, d_k, kslot.data(), P.nnz * 4));
     A(upload(s, d_q, P.q.data(), P.nnz * 8));
   }
   if (P.P) {
@@ -408,11 +416,8 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
   }
   if (st != MMP_OK) return cleanup(st);
   s->tw = Twiddles{rowtw, coltw, fourtw};
-  s->gather = GatherArgs{d_off, d_k, d_slot, d_q, d_sa, d_sb, d_sl, d_sk, (int)P.P, P.raman, P.kv, P.kr};
-  if (!gather_tile(s->g, (int)P.P, 8192)) {
-    release(s);
-    return fail(MMP_ERR_UNSUPPORTED, "too many pair slots for the gather tile (M + P must be <= 1024)");
-  }
+  s->gather = GatherArgs{d_coff, d_k, d_q, d_sa, d_sb, d_sl, d_sk, (int)P.P, P.gather_chunk,
+                         P.gather_chunks, P.gather_cap, P.raman, P.kv, P.kr};
   s->pairs = PairLineArgs{d_la, d_lb, d_ls};
   // stage heads
   const double h = P.delta_z;
@@ -423,20 +428,22 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
     a.acc = s->acc;
     a.stash = s->stash;
     a.yout = s->ybuf;
-    a.imL = s->imL;
-    a.st = s->st;
+    a.e1 = s->e1;
     a.sgn = P.sign_gamma;
+    a.scale_cur = 1.0;
     a.has_prev = i > 0;
     a.prev_stage = i - 1;
     a.prev_is_first = (i == 1);
     a.final_stage = (i == P.rk.s);
     if (i > 0) {
-      a.ch_prev = P.rk.c[i - 1] * h;
+      const int u = P.stage_table[i - 1];
+      a.e_prev = s->etab + (size_t)u * MN;
+      a.update_prev = (P.table_last_head[u] == i);
       a.bh_prev = (P.rk.b[i - 1] != 0.) ? P.rk.b[i - 1] * h : 0.;
       a.stash_prev = P.stash_needed[i - 1];
     }
     if (i < P.rk.s) {
-      a.ch_cur = P.rk.c[i] * h;
+      a.e_cur = s->etab + (size_t)P.stage_table[i] * MN;
       for (int j = 0; j < i; j++)
         if (P.rk.a[i][j] != 0.) {
           a.a_idx[a.n_a] = j;
@@ -661,7 +668,7 @@ mmp_status mmp_plan(const mmp_params* params, const mmp_options* options, mmp_in
 }
 
 mmp_status mmp_plan_arrays(const mmp_params* params, const mmp_options* options, double* psi0_out, double* imL_out,
-                           uint32_t* pair_a_out, uint32_t* pair_b_out) {
+                           uint32_t* pair_a_out, uint32_t* pair_b_out, double* e1_out, double* e_out) {
   try {
     mmp_options opt{};
     if (options) opt = *options;
@@ -678,6 +685,22 @@ mmp_status mmp_plan_arrays(const mmp_params* params, const mmp_options* options,
         }
     if (pair_a_out) std::memcpy(pair_a_out, P.slot_a.data(), P.P * sizeof(uint32_t));
     if (pair_b_out) std::memcpy(pair_b_out, P.slot_b.data(), P.P * sizeof(uint32_t));
+    const size_t MN = (size_t)P.M * P.N;
+    for (uint32_t p = 0; p < P.M; p++)
+      for (uint32_t j = 0; j < P.N; j++) {
+        const uint32_t k1 = j % P.N1, k2 = j / P.N1;
+        const size_t nat = (size_t)p * P.N + j, four = (size_t)p * P.N + (size_t)k1 * P.N2 + k2;
+        if (e1_out) {
+          e1_out[2 * nat] = P.e1[2 * four];
+          e1_out[2 * nat + 1] = P.e1[2 * four + 1];
+        }
+        if (e_out)
+          for (int i = 0; i < P.rk.s; i++) {
+            const size_t src = (size_t)P.stage_table[i] * MN + four, dst = (size_t)i * MN + nat;
+            e_out[2 * dst] = P.etab0[2 * src];
+            e_out[2 * dst + 1] = P.etab0[2 * src + 1];
+          }
+      }
     return MMP_OK;
   } catch (const std::bad_alloc&) {
     return fail(MMP_ERR_OUT_OF_MEMORY, "host allocation failed");

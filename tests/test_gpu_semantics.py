@@ -1,0 +1,114 @@
+"""Reference semantics of the device solver: restart on every computeLawsonRK(nz) (F5),
+z by repeated addition (F3), the default step count (F6), resumable stepping, state
+round trips, energy diagnostics and error statuses."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import oracle_binding as ob
+from gpu_util import note, require_device
+from mm_propagation_amd import _capi, configs
+from mm_propagation_amd.solver import MultimodePropagationGPU_RamanON, make_solver
+
+pytestmark = pytest.mark.gpu
+
+
+def test_restart_semantics_match_oracle():
+    require_device()
+    sp = configs.two_mode_gaussian(nt=1024)
+    note("case restart semantics M=2 NT=1024")
+    g = MultimodePropagationGPU_RamanON(sp)
+    o = ob.Oracle(sp)
+    for nz in (7, 5, 11):
+        g.computeLawsonRK(nz)
+        o.compute_lawson_rk(nz)
+    err = configs.rel_l2_per_mode(g.getResult(), o.get_result())
+    assert np.all(err < 1e-12), err
+    psi_g, z_g = g.get_state()
+    psi_o, z_o = o.get_state()
+    assert z_g == z_o  # bitwise: both sum h eleven times
+    assert np.all(configs.rel_l2_per_mode(psi_g, psi_o) < 1e-12)
+    g.close()
+
+
+def test_resumable_advance_equals_single_run():
+    require_device()
+    sp = configs.grin_fiber(M=3, nt=1024, symmetric=True)
+    note("case advance split M=3 NT=1024")
+    a = MultimodePropagationGPU_RamanON(sp, pair_decode=1)
+    a.computeLawsonRK(10)
+    a.advance(23)
+    b = MultimodePropagationGPU_RamanON(sp, pair_decode=1)
+    b.computeLawsonRK(33)
+    assert np.array_equal(a.getResult(), b.getResult())
+    assert a.info()["z"] == b.info()["z"]
+    a.close()
+    b.close()
+
+
+def test_default_step_count_and_info():
+    require_device()
+    sp = configs.two_mode_gaussian(nt=256, n_steps=40)
+    note("case default steps")
+    g = MultimodePropagationGPU_RamanON(sp)
+    i = g.info()
+    assert i["default_steps"] == math.floor(i["z_final"] / i["delta_z"])
+    g.computeLawsonRK()
+    assert g.info()["steps_taken"] == i["default_steps"]
+    o = ob.Oracle(sp)
+    o.compute_lawson_rk(o.default_steps())
+    assert np.all(configs.rel_l2_per_mode(g.getResult(), o.get_result()) < 1e-12)
+    g.close()
+
+
+def test_state_round_trip_and_energy():
+    require_device()
+    sp = configs.grin_fiber(M=4, nt=2048, symmetric=True)
+    note("case state round trip and energy")
+    g = MultimodePropagationGPU_RamanON(sp, pair_decode=1)
+    psi0, z0 = g.get_state()
+    assert z0 == 0.0
+    conv = g.info()["conversion_factor"]
+    assert np.allclose(psi0, sp.initial_fields * conv, rtol=1e-14, atol=1e-14 * np.abs(psi0).max())
+    g.computeLawsonRK(12)
+    res = g.getResult()
+    e = g.energy()
+    assert np.allclose(e, configs.energy_per_mode(res, sp.time_window), rtol=1e-13)
+    g.set_state(psi0)
+    assert g.info()["z"] == 0.0
+    back, _ = g.get_state()
+    assert np.allclose(back, psi0, rtol=0, atol=1e-14 * np.abs(psi0).max())
+    g.close()
+
+
+def test_nonfinite_field_is_reported():
+    """A step of h = z_final / 3 drives the two-mode workload to overflow: getResult must
+    report MMP_ERR_NONFINITE, as the oracle's field is non-finite too."""
+    require_device()
+    sp = configs.two_mode_gaussian(nt=256, n_steps=3)
+    note("case non-finite detection")
+    g = MultimodePropagationGPU_RamanON(sp)
+    g.computeLawsonRK(3)
+    with pytest.raises(_capi.MmpError) as ei:
+        g.getResult()
+    assert ei.value.status == 6
+    g.close()
+
+
+@pytest.mark.parametrize("mutate,status", [
+    (lambda sp: setattr(sp, "nt", 100) or setattr(sp, "initial_fields", np.zeros((2, 100), complex)) or
+     setattr(sp, "raman_response", np.zeros(100)), 2),
+    (lambda sp: setattr(sp, "dispersion_coefficients", [[0, 0], [0, 0]]), 1),
+    (lambda sp: sp.coupling_coefficients[0].resetValue(0, 0.0) or sp.coupling_coefficients[0].resetValue(1, 0.0) or
+     sp.coupling_coefficients[0].resetValue(2, 0.0) or [sp.coupling_coefficients[1].resetValue(i, 0.0) for i in range(3)],
+     1),
+])
+def test_invalid_inputs_rejected_on_device_path(mutate, status):
+    require_device()
+    sp = configs.two_mode_gaussian(nt=256)
+    mutate(sp)
+    with pytest.raises(_capi.MmpError) as ei:
+        MultimodePropagationGPU_RamanON(sp)
+    assert ei.value.status == status

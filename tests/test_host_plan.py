@@ -1,0 +1,97 @@
+"""The library's host-side preparation (C ABI mmp_plan / mmp_plan_arrays, no device):
+validation statuses, scalars, and the prepared psi0, L and Lawson tables, bit for bit
+against the reference's own setup code (oracle/_ref) and the oracle."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_binding as ob
+from mm_propagation_amd import _capi, configs, plan
+from test_oracle_setup import CASES
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_prepared_arrays_match_oracle_bitwise(name):
+    sp = CASES[name]()
+    for decode in (0, 1):
+        got = plan.plan_arrays(sp, pair_decode=decode)
+        o = ob.Oracle(sp, pair_decode=decode)
+        o.init_operators()
+        assert np.array_equal(got["psi0"], o.get_state()[0])
+        L = o.linear_operator()
+        assert np.array_equal(got["imL"], L.imag)
+        assert np.array_equal(got["e1"], o.e1())
+        for i in range(o.info.stages):
+            assert np.array_equal(got["e"][i], o.e(i))
+        a, b = o.pair_table()
+        assert np.array_equal(got["pair_a"], a) and np.array_equal(got["pair_b"], b)
+        info = got["info"]
+        for k in ("t_final", "delta_t", "z_final", "delta_z", "conversion_factor", "sign_gamma"):
+            assert info[k] == getattr(o.info, k), k
+        assert info["default_steps"] == o.default_steps()
+
+
+@pytest.mark.skipif(not ob.ref_available(), reason="reference setup probe not built (make -C oracle ref)")
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_prepared_arrays_match_reference_setup_bitwise(name):
+    sp = CASES[name]()
+    got = plan.plan_arrays(sp)
+    r = ob.ref_setup(sp)
+    assert np.array_equal(got["psi0"], r["psi"])
+    assert np.array_equal(got["imL"], r["L"].imag)
+    assert np.array_equal(got["e1"], r["E1"])
+    assert np.array_equal(got["e"], r["E"])
+
+
+def test_plan_sizes_and_memory():
+    sp = configs.grin_fiber(M=6, nt=16384, symmetric=True)
+    i = plan.plan(sp, pair_decode=1)
+    assert (i["n_modes"], i["nt"], i["stages"], i["n_pairs"], i["nnz"]) == (6, 16384, 4, 36, 648)
+    assert i["n1"] * i["n2"] == 16384 and i["n1"] == 128
+    # 21 Raman lines: 15 cross pairs + 3 lines of packed self pairs
+    assert i["n_conv_lines"] == 18
+    assert 10e6 < i["device_bytes"] < 40e6
+
+
+def expect_status(sp, status, raman=True):
+    with pytest.raises(_capi.MmpError) as ei:
+        plan.plan(sp, raman=raman)
+    assert ei.value.status == status, str(ei.value)
+
+
+def test_validation_statuses():
+    base = configs.two_mode_gaussian(nt=256)
+    sp = configs.two_mode_gaussian(nt=256)
+    sp.nt = 100
+    sp.initial_fields = np.zeros((2, 100), complex)
+    sp.raman_response = np.zeros(100)
+    expect_status(sp, 2)  # NT not a power of two: unsupported
+    sp = configs.two_mode_gaussian(nt=32)
+    expect_status(sp, 2)  # below the smallest supported length
+    sp = configs.two_mode_gaussian(nt=256)
+    sp.dispersion_coefficients = [[0, 0], [0, 0]]
+    expect_status(sp, 1)  # fewer than 3 orders
+    sp = configs.two_mode_gaussian(nt=256)
+    sp.dispersion_coefficients = [[0, 0, 0], [0, 0, 0]]
+    expect_status(sp, 1)  # max |beta2| = 0
+    sp = configs.two_mode_gaussian(nt=256)
+    for q in sp.coupling_coefficients:
+        for i in range(len(q)):
+            q.resetValue(i, 0.0)
+    expect_status(sp, 1)  # max |Q| = 0
+    sp = configs.two_mode_gaussian(nt=256)
+    sp.coupling_coefficients[0].addNonZeroValue(configs.NonZeroValue((0, 2, 0), 1.0))
+    expect_status(sp, 1)  # index out of range
+    sp = configs.two_mode_gaussian(nt=256)
+    sp.initial_fields = sp.initial_fields.copy()
+    sp.initial_fields[1, 3] = np.nan
+    expect_status(sp, 6)  # non-finite input
+    sp = configs.two_mode_gaussian(nt=256)
+    sp.n_steps = 0
+    expect_status(sp, 1)
+    # Raman off needs no response
+    sp = configs.two_mode_gaussian(nt=256)
+    sp.raman_response = None
+    assert plan.plan(sp, raman=False)["n_conv_lines"] == 0
+    assert plan.plan(base)["n_conv_lines"] == 2
