@@ -65,6 +65,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels one by one instead of replaying HIP graphs (for kernel tracers)")
+    ap.add_argument("--unfused", action="store_true", help="separate column passes (7 per stage instead of 4)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -77,7 +78,7 @@ def main() -> int:
     sp = workload()
     M, NT = sp.n_modes, sp.nt
     ctx = Context(0)
-    flags = _capi.MMP_FLAG_NO_GRAPH if args.no_graph else 0
+    flags = (_capi.MMP_FLAG_NO_GRAPH if args.no_graph else 0) | (_capi.MMP_FLAG_UNFUSED if args.unfused else 0)
     s = MultimodePropagationGPU_RamanON(sp, pair_decode=_capi.MMP_PAIRS_PHYSICAL, flags=flags, ctx=ctx)
     psi0, _ = s.get_state()
     if args.warmup:
@@ -103,8 +104,8 @@ def main() -> int:
     s.close()
 
     # per-kernel HIP-event timings on the solver's stream over a profiled run of the same workload
-    ps = MultimodePropagationGPU_RamanON(sp, pair_decode=_capi.MMP_PAIRS_PHYSICAL, flags=_capi.MMP_FLAG_PROFILE,
-                                         ctx=ctx)
+    ps = MultimodePropagationGPU_RamanON(sp, pair_decode=_capi.MMP_PAIRS_PHYSICAL,
+                                         flags=_capi.MMP_FLAG_PROFILE | (flags & _capi.MMP_FLAG_UNFUSED), ctx=ctx)
     ps.advance(8)
     ps.kernel_times()
     ps.advance(args.profile_steps)

@@ -91,6 +91,20 @@ struct PairLineArgs {
   const uint32_t* line_self;  // [n_lines] 1 = packed self pairs, 0 = cross pair a, b
 };
 
+// Fused column passes (one workgroup per column, Raman on, N1 >= 64).
+struct FusedArgs {
+  const uint32_t* line_slot0;  // [n_lines + 1] first device slot of each Raman line
+  const uint32_t* chunk_slot;  // [n_chunks + 1] first slot of each chunk of Raman lines
+  const uint32_t* coff;        // [M][n_chunks + 1] nonzero ranges per chunk
+  int n_lines, chunk_lines, n_chunks;
+};
+
+// LDS budget of the fused column kernels (complex values), the part the line transforms
+// use, and the largest mode count the gather lanes can hold for column length L.
+int fused_lds_capacity();
+int fused_fft_slots();
+int fused_max_modes(int L);
+
 // Launchers (return hipSuccess or the launch error).
 hipError_t launch_row_head(const Geometry& g, const Twiddles& tw, const HeadArgs& a, hipStream_t s);
 hipError_t launch_col_inverse(const Geometry& g, const Twiddles& tw, const cd* in, cd* out, int lines, double scale,
@@ -108,6 +122,15 @@ hipError_t launch_row_raman(const Geometry& g, const Twiddles& tw, cd* data, int
                             double hscale, hipStream_t s);
 hipError_t launch_gather(const Geometry& g, const cd* U, const cd* VR, cd* sigma, const GatherArgs& a,
                          hipStream_t s);
+hipError_t launch_col_fields_pairs(const Geometry& g, const Twiddles& tw, const cd* yin, cd* U, cd* conv,
+                                   int n_lines, const PairLineArgs& pl, double scale, hipStream_t s);
+hipError_t launch_col_raman_gather(const Geometry& g, const Twiddles& tw, const cd* U, const cd* conv, cd* yout,
+                                   const GatherArgs& ga, const FusedArgs& fa, double scale, hipStream_t s);
+// Dense-block gather (M <= 16): Qd is [P][M][M] with Qd[s][p][k] = sum of Q/max|Q| over the
+// nonzeros (k, slot s) of Q^(p). dense_gather_chunk(M) is 0 when M is out of range.
+int dense_gather_chunk(int M);
+hipError_t launch_gather_dense(const Geometry& g, const cd* U, const cd* VR, cd* sigma, const GatherArgs& a,
+                               const double* Qd, hipStream_t s);
 hipError_t launch_advance_z(StepState* st, double h, hipStream_t s);
 hipError_t launch_energy(const Geometry& g, const cd* fields, double* out, double dt, hipStream_t s);
 

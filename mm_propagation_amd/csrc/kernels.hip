@@ -388,6 +388,83 @@ __global__ void __launch_bounds__(64) k_row_raman(Geometry g, const cd* __restri
 // metadata loads are uniform; nonzeros are sorted by slot so consecutive ones
 // reuse X. Slots are staged through LDS in chunks when M + P exceeds the tile.
 // ---------------------------------------------------------------------------
+// sum += q_e * X[slot_e - s0] * U[k_e] over the nonzeros [e0, e1) for this lane's sample.
+// e0, e1 and s0 are wave-uniform. Xs and Us point at the lane's sample in LDS rows of
+// `stride` values. Lane l loads nonzero eb + l; v_readlane broadcasts it, eight at a
+// time, so the sixteen LDS reads of a batch are in flight together. The products are
+// added in nonzero order.
+__device__ __forceinline__ cd gather_sum(cd sum, const uint32_t* __restrict__ kslot, const double* __restrict__ q,
+                                         uint32_t e0, uint32_t e1, int s0, const cd* Xs, const cd* Us, int stride,
+                                         int lane) {
+  constexpr int B = 8;
+  for (uint32_t eb = e0; eb < e1; eb += 64) {
+    const uint32_t mine = eb + lane;
+    const uint32_t ks = mine < e1 ? kslot[mine] : 0u;
+    const double qv = mine < e1 ? q[mine] : 0.;
+    const int qlo = __double2loint(qv), qhi = __double2hiint(qv);
+    const int cnt = (int)min(64u, e1 - eb);
+    int j = 0;
+    for (; j + B <= cnt; j += B) {
+      cd xs[B], us[B];
+      double qs[B];
+#pragma unroll
+      for (int u = 0; u < B; u++) {
+        const uint32_t ksj = (uint32_t)__builtin_amdgcn_readlane((int)ks, j + u);
+        qs[u] = __hiloint2double(__builtin_amdgcn_readlane(qhi, j + u), __builtin_amdgcn_readlane(qlo, j + u));
+        xs[u] = Xs[((int)(ksj >> 8) - s0) * stride];
+        us[u] = Us[(int)(ksj & 255u) * stride];
+      }
+#pragma unroll
+      for (int u = 0; u < B; u++) sum = cadd(sum, cscale(cmul(xs[u], us[u]), qs[u]));
+    }
+    for (; j < cnt; j++) {
+      const uint32_t ksj = (uint32_t)__builtin_amdgcn_readlane((int)ks, j);
+      const double qj = __hiloint2double(__builtin_amdgcn_readlane(qhi, j), __builtin_amdgcn_readlane(qlo, j));
+      sum = cadd(sum, cscale(cmul(Xs[((int)(ksj >> 8) - s0) * stride], Us[(int)(ksj & 255u) * stride]), qj));
+    }
+  }
+  return sum;
+}
+
+// Slot mixes of a 64-sample tile into LDS: X[s - s0][j] = kv U_a conj(U_b) + kr VR_s[t0 + j]
+// for slots [s0, s0 + ns). Every Raman value a lane needs is loaded before any is used, so
+// the loads overlap instead of each waiting for the previous one.
+__device__ __forceinline__ void slot_mix_tile(const GatherArgs& a, const Geometry& g, const cd* __restrict__ VR,
+                                              const cd* Ush, cd* Xsh, int s0, int ns, int t0, int tid, int NTHR) {
+  constexpr int TS = 64, R = 16;
+  const int total = ns * TS;
+  for (int base = 0; base < total; base += R * NTHR) {
+    cd r[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+      const int e = base + tid + i * NTHR;
+      r[i] = {0., 0.};
+      // e / 64 is the same for the whole wavefront (e = base + 64 * wave + lane + i * NTHR)
+      const int sl = __builtin_amdgcn_readfirstlane(s0 + e / TS);
+      if (a.raman && e < total) r[i] = VR[(size_t)a.slot_line[sl] * g.N + t0 + e % TS];
+    }
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+      const int e = base + tid + i * NTHR;
+      if (e >= total) break;
+      const int sl = __builtin_amdgcn_readfirstlane(s0 + e / TS), j = e % TS;
+      const cd V = cmulc(Ush[a.slot_a[sl] * TS + j], Ush[a.slot_b[sl] * TS + j]);
+      cd X = cscale(V, a.kv);
+      if (a.raman) {
+        cd vr;
+        switch (a.slot_kind[sl]) {
+          case kDirect: vr = r[i]; break;
+          case kConj: vr = cconj(r[i]); break;
+          case kRe: vr = {r[i].re, 0.}; break;
+          default: vr = {r[i].im, 0.}; break;
+        }
+        X = cadd(X, cscale(vr, a.kr));
+      }
+      Xsh[e] = X;
+    }
+  }
+}
+
 constexpr int kGatherTS = 64;
 constexpr int kGatherMaxPW = 8;  // modes per wavefront
 
@@ -416,7 +493,9 @@ __global__ void __launch_bounds__(512) k_gather(Geometry g, const cd* __restrict
   cd* Xsh = sh + M * TS;   // [chunk][TS]
   const int t0 = blockIdx.x * TS;
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int lane = tid % TS, wave = tid / TS, W = nthr / TS;
+  const int lane = tid % TS, W = nthr / TS;
+  const int wave = __builtin_amdgcn_readfirstlane(tid / TS);  // wave-uniform for the compiler
+#pragma unroll 4
   for (int e = tid; e < M * TS; e += nthr) Ush[e] = U[(size_t)(e / TS) * g.N + t0 + e % TS];
   cd acc[kGatherMaxPW];
 #pragma unroll
@@ -425,53 +504,14 @@ __global__ void __launch_bounds__(512) k_gather(Geometry g, const cd* __restrict
     const int s0 = c * a.chunk;
     const int ns = min(a.chunk, a.n_slots - s0);
     __syncthreads();  // Ush loaded / previous chunk consumed
-    for (int e = tid; e < ns * TS; e += nthr) {
-      const int s = s0 + e / TS, j = e % TS;
-      const cd V = cmulc(Ush[a.slot_a[s] * TS + j], Ush[a.slot_b[s] * TS + j]);
-      cd X = cscale(V, a.kv);
-      if (a.raman) {
-        const cd r = VR[(size_t)a.slot_line[s] * g.N + t0 + j];
-        cd vr;
-        switch (a.slot_kind[s]) {
-          case kDirect: vr = r; break;
-          case kConj: vr = cconj(r); break;
-          case kRe: vr = {r.re, 0.}; break;
-          default: vr = {r.im, 0.}; break;
-        }
-        X = cadd(X, cscale(vr, a.kr));
-      }
-      Xsh[e] = X;
-    }
+    slot_mix_tile(a, g, VR, Ush, Xsh, s0, ns, t0, tid, nthr);
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < kGatherMaxPW; r++) {
       const int p = wave + r * W;
       if (p >= M) break;
       const uint32_t* co = a.coff + (size_t)p * (a.n_chunks + 1) + c;
-      const uint32_t e0 = co[0], e1 = co[1];
-      cd sum = acc[r];
-      // 64 nonzeros at a time: lane l holds nonzero eb + l; the loop broadcasts them with
-      // v_readlane so no memory access sits on the loop's dependency chain
-      for (uint32_t eb = e0; eb < e1; eb += TS) {
-        const uint32_t mine = eb + lane;
-        const uint32_t ks = mine < e1 ? a.kslot[mine] : 0u;
-        const double qv = mine < e1 ? a.q[mine] : 0.;
-        const int qlo = __double2loint(qv), qhi = __double2hiint(qv);
-        const int cnt = (int)min((uint32_t)TS, e1 - eb);
-        int cur = -1;
-        cd X = {0., 0.};
-        for (int j = 0; j < cnt; j++) {
-          const uint32_t ksj = (uint32_t)__builtin_amdgcn_readlane((int)ks, j);
-          const double qj = __hiloint2double(__builtin_amdgcn_readlane(qhi, j), __builtin_amdgcn_readlane(qlo, j));
-          const int sl = (int)(ksj >> 8) - s0;
-          if (sl != cur) {
-            X = Xsh[sl * TS + lane];
-            cur = sl;
-          }
-          sum = cadd(sum, cscale(cmul(X, Ush[(ksj & 255u) * TS + lane]), qj));
-        }
-      }
-      acc[r] = sum;
+      acc[r] = gather_sum(acc[r], a.kslot, a.q, co[0], co[1], s0, Xsh + lane, Ush + lane, TS, lane);
     }
   }
 #pragma unroll
@@ -479,6 +519,264 @@ __global__ void __launch_bounds__(512) k_gather(Geometry g, const cd* __restrict
     const int p = wave + r * W;
     if (p >= M) break;
     sigma[(size_t)p * g.N + t0 + lane] = acc[r];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused column passes. One workgroup of 256 lanes owns one column n2; its
+// lanes form 2048/L line slots of L/8 lanes that transform one line each per
+// round. The column of every mode field stays in LDS between the transforms.
+// ---------------------------------------------------------------------------
+constexpr int kFusedCap = 10240;  // complex LDS values (160 KB)
+constexpr int kFusedThreads = 256;
+constexpr int kFusedMaxPW = 16;   // modes per wavefront in the gather
+
+template <int L>
+struct FusedCfg {
+  static constexpr int T = L / 8;
+  static constexpr int NL = kFusedThreads / T;  // line slots per round
+  static constexpr int SLOTS = LineSlots<L, 1>::value;
+  static constexpr int FFT = NL * SLOTS;        // 2304 for every L
+  static constexpr int SG = L >= 64 ? L / 64 : 1;  // sample groups of 64 in a column
+  static constexpr int PG = 4 / SG;             // mode groups (4 wavefronts)
+  static_assert(L >= 64 && L <= 256, "fused column passes need 64 <= N1 <= 256");
+};
+
+int fused_lds_capacity() { return kFusedCap; }
+int fused_fft_slots() { return FusedCfg<64>::FFT; }
+int fused_max_modes(int L) { return kFusedMaxPW * (L >= 256 ? 1 : (L >= 128 ? 2 : 4)); }
+
+// C1: inverse column DFT of the M mode lines -> U (global and LDS), then the Raman
+// pair products and their forward column DFT -> conv.
+template <int L>
+__global__ void __launch_bounds__(256) k_col_fields_pairs(Geometry g, const cd* __restrict__ coltw,
+                                                          const cd* __restrict__ fourtw, const cd* __restrict__ yin,
+                                                          cd* __restrict__ U, cd* __restrict__ conv, int n_lines,
+                                                          PairLineArgs pl, double scale) {
+  using F = FusedCfg<L>;
+  __shared__ cd sh[kFusedCap];
+  cd* Ush = sh + F::FFT;  // [M][L]
+  const int tid = threadIdx.x;
+  const int job = tid / F::T, t = tid % F::T;
+  const int n2 = xcd_block(blockIdx.x, gridDim.x);
+  cd* fsm = sh + job * F::SLOTS;
+  LineTw<L> lt;
+  load_line_tw<L>(lt, coltw, t);
+  cd w4[8], v[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) w4[i] = fourtw[n2 * (t + i * F::T)];
+  for (int r0 = 0; r0 < g.M; r0 += F::NL) {
+    const int q = r0 + job;
+    const bool valid = q < g.M;
+    const size_t base = (size_t)(valid ? q : 0) * g.N + n2;
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = valid ? cmulc(yin[base + (size_t)(t + i * F::T) * g.N2], w4[i]) : cd{0., 0.};
+    line_fft<L, 1, 1, 1>(v, fsm, lt, t);
+    if (valid) {
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const int n1 = t + i * F::T;
+        const cd x = cscale(v[i], scale);
+        U[base + (size_t)n1 * g.N2] = x;
+        Ush[q * L + n1] = x;
+      }
+    }
+  }
+  __syncthreads();
+  for (int r0 = 0; r0 < n_lines; r0 += F::NL) {
+    const int q = r0 + job;
+    const bool valid = q < n_lines;
+    const int qq = valid ? q : 0;
+    const uint32_t la = pl.line_a[qq], lb = pl.line_b[qq], self = pl.line_self[qq];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int n1 = t + i * F::T;
+      cd x = {0., 0.};
+      if (valid) {
+        const cd ua = Ush[la * L + n1];
+        if (self) {
+          x.re = ua.re * ua.re + ua.im * ua.im;
+          if (lb != 0xffffffffu) {
+            const cd ub = Ush[lb * L + n1];
+            x.im = ub.re * ub.re + ub.im * ub.im;
+          }
+        } else {
+          x = cmulc(ua, Ush[lb * L + n1]);
+        }
+      }
+      v[i] = x;
+    }
+    line_fft<L, -1, 1, 1>(v, fsm, lt, t);
+    if (valid) {
+      const size_t base = (size_t)q * g.N + n2;
+#pragma unroll
+      for (int i = 0; i < 8; i++) conv[base + (size_t)(t + i * F::T) * g.N2] = cmul(v[i], w4[i]);
+    }
+  }
+}
+
+// C2: inverse column DFT of the Raman lines (chunk by chunk) -> slot mixes X in LDS ->
+// Kerr + Raman gather for the column's samples -> sigma in LDS -> forward column DFT.
+template <int L>
+__global__ void __launch_bounds__(256) k_col_raman_gather(Geometry g, const cd* __restrict__ coltw,
+                                                          const cd* __restrict__ fourtw, const cd* __restrict__ U,
+                                                          const cd* __restrict__ conv, cd* __restrict__ yout,
+                                                          GatherArgs a, FusedArgs fa, double scale) {
+  using F = FusedCfg<L>;
+  __shared__ cd sh[kFusedCap];
+  const int M = g.M;
+  cd* Ush = sh + F::FFT;     // [M][L]; holds sigma after the gather
+  cd* Xsh = Ush + M * L;     // [slots of a chunk][L]
+  const int tid = threadIdx.x;
+  const int job = tid / F::T, t = tid % F::T;
+  const int n2 = xcd_block(blockIdx.x, gridDim.x);
+  cd* fsm = sh + job * F::SLOTS;
+  LineTw<L> lt;
+  load_line_tw<L>(lt, coltw, t);
+  cd w4[8], v[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) w4[i] = fourtw[n2 * (t + i * F::T)];
+#pragma unroll 4
+  for (int e = tid; e < M * L; e += kFusedThreads) Ush[e] = U[(size_t)(e / L) * g.N + (size_t)(e % L) * g.N2 + n2];
+  // gather lanes: wavefront w covers samples sg*64 + lane of modes pg, pg + PG, ...
+  const int lane = tid % 64;
+  const int wave = __builtin_amdgcn_readfirstlane(tid / 64);  // wave-uniform for the compiler
+  const int sg = wave % F::SG, pg = wave / F::SG;
+  const int n1g = sg * 64 + lane;
+  cd acc[kFusedMaxPW];
+#pragma unroll
+  for (int r = 0; r < kFusedMaxPW; r++) acc[r] = {0., 0.};
+  for (int c = 0; c < fa.n_chunks; c++) {
+    const int l0 = c * fa.chunk_lines;
+    const int l1 = min(fa.n_lines, l0 + fa.chunk_lines);
+    const int s0 = (int)fa.chunk_slot[c];
+    __syncthreads();  // Ush loaded / Xsh of the previous chunk consumed
+    for (int r0 = l0; r0 < l1; r0 += F::NL) {
+      const int q = r0 + job;
+      const bool valid = q < l1;
+      const size_t base = (size_t)(valid ? q : 0) * g.N + n2;
+#pragma unroll
+      for (int i = 0; i < 8; i++)
+        v[i] = valid ? cmulc(conv[base + (size_t)(t + i * F::T) * g.N2], w4[i]) : cd{0., 0.};
+      line_fft<L, 1, 1, 1>(v, fsm, lt, t);
+      if (valid) {
+        const int sa = (int)fa.line_slot0[q], sb = (int)fa.line_slot0[q + 1];
+        for (int sl = sa; sl < sb; sl++) {
+          const uint32_t pa = a.slot_a[sl], pb = a.slot_b[sl], kind = a.slot_kind[sl];
+#pragma unroll
+          for (int i = 0; i < 8; i++) {
+            const int n1 = t + i * F::T;
+            const cd r = cscale(v[i], scale);
+            cd vr;
+            switch (kind) {
+              case kDirect: vr = r; break;
+              case kConj: vr = cconj(r); break;
+              case kRe: vr = {r.re, 0.}; break;
+              default: vr = {r.im, 0.}; break;
+            }
+            const cd V = cmulc(Ush[pa * L + n1], Ush[pb * L + n1]);
+            Xsh[(sl - s0) * L + n1] = cadd(cscale(V, a.kv), cscale(vr, a.kr));
+          }
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kFusedMaxPW; r++) {
+      const int p = pg + r * F::PG;
+      if (p >= M) break;
+      const uint32_t* co = fa.coff + (size_t)p * (fa.n_chunks + 1) + c;
+      acc[r] = gather_sum(acc[r], a.kslot, a.q, co[0], co[1], s0, Xsh + n1g, Ush + n1g, L, lane);
+    }
+  }
+  __syncthreads();  // every lane done reading Ush and Xsh
+#pragma unroll
+  for (int r = 0; r < kFusedMaxPW; r++) {
+    const int p = pg + r * F::PG;
+    if (p >= M) break;
+    Ush[p * L + n1g] = acc[r];
+  }
+  __syncthreads();
+  for (int r0 = 0; r0 < M; r0 += F::NL) {
+    const int q = r0 + job;
+    const bool valid = q < M;
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = valid ? Ush[q * L + t + i * F::T] : cd{0., 0.};
+    line_fft<L, -1, 1, 1>(v, fsm, lt, t);
+    if (valid) {
+      const size_t base = (size_t)q * g.N + n2;
+#pragma unroll
+      for (int i = 0; i < 8; i++) yout[base + (size_t)(t + i * F::T) * g.N2] = cmul(v[i], w4[i]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Dense-block Kerr + Raman gather for M <= 16 modes, 64 samples per workgroup:
+//   sigma_p = sum over slots s of  sum_k Qd[s][p][k] * X_s * U_k
+// Each lane keeps its sample's U_k (all k) and sigma_p (all p) in registers; X_s
+// comes from LDS (one read per slot) and the M x M block Qd[s] (zeros included)
+// from scalar loads. The four wavefronts take every fourth slot and their
+// partial sums are added in wavefront order at the end.
+// ---------------------------------------------------------------------------
+constexpr int kDenseCap = 4096;  // complex LDS values (64 KB)
+
+int dense_gather_chunk(int M) {
+  if (M < 1 || M > 16) return 0;
+  const int room = kDenseCap / 64 - M;
+  return room >= 4 * M ? room : 0;
+}
+
+template <int M>
+__global__ void __launch_bounds__(256) k_gather_dense(Geometry g, const cd* __restrict__ U, const cd* __restrict__ VR,
+                                                      cd* __restrict__ sigma, GatherArgs a, const double* __restrict__ Qd,
+                                                      int chunk) {
+  __shared__ cd sh[kDenseCap];
+  constexpr int TS = 64;
+  cd* Ush = sh;           // [M][TS]
+  cd* Xsh = sh + M * TS;  // [chunk][TS]; reused for the partial sums
+  const int t0 = blockIdx.x * TS;
+  const int tid = threadIdx.x, lane = tid % TS;
+  const int wave = __builtin_amdgcn_readfirstlane(tid / TS);
+#pragma unroll 4
+  for (int e = tid; e < M * TS; e += 256) Ush[e] = U[(size_t)(e / TS) * g.N + t0 + e % TS];
+  __syncthreads();
+  cd u[M], acc[M];
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    u[k] = Ush[k * TS + lane];
+    acc[k] = {0., 0.};
+  }
+  for (int s0 = 0; s0 < a.n_slots; s0 += chunk) {
+    const int ns = min(chunk, a.n_slots - s0);
+    __syncthreads();  // previous chunk consumed
+    slot_mix_tile(a, g, VR, Ush, Xsh, s0, ns, t0, tid, 256);
+    __syncthreads();
+    for (int sl = s0 + wave; sl < s0 + ns; sl += 4) {
+      const cd X = Xsh[(sl - s0) * TS + lane];
+      const double* qb = Qd + (size_t)sl * M * M;
+      cd y[M];
+#pragma unroll
+      for (int k = 0; k < M; k++) y[k] = cmul(X, u[k]);
+#pragma unroll
+      for (int p = 0; p < M; p++) {
+        cd t = acc[p];
+#pragma unroll
+        for (int k = 0; k < M; k++) t = cadd(t, cscale(y[k], qb[p * M + k]));
+        acc[p] = t;
+      }
+    }
+  }
+  __syncthreads();  // X consumed: the region takes the partial sums
+#pragma unroll
+  for (int p = 0; p < M; p++) Xsh[(wave * M + p) * TS + lane] = acc[p];
+  __syncthreads();
+  for (int e = tid; e < M * TS; e += 256) {
+    const int p = e / TS, j = e % TS;
+    cd t = Xsh[p * TS + j];
+#pragma unroll
+    for (int w = 1; w < 4; w++) t = cadd(t, Xsh[(w * M + p) * TS + j]);
+    sigma[(size_t)p * g.N + t0 + j] = t;
   }
 }
 
@@ -653,6 +951,63 @@ hipError_t launch_gather(const Geometry& g, const cd* U, const cd* VR, cd* sigma
   if (a.cap == 4096) hipLaunchKernelGGL(k_gather<4096>, grid, block, 0, s, g, U, VR, sigma, a);
   else if (a.cap == 8192) hipLaunchKernelGGL(k_gather<8192>, grid, block, 0, s, g, U, VR, sigma, a);
   else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+template <int L>
+static hipError_t fields_pairs_impl(const Geometry& g, const Twiddles& tw, const cd* yin, cd* U, cd* conv,
+                                    int n_lines, const PairLineArgs& pl, double scale, hipStream_t s) {
+  if (FusedCfg<L>::FFT + g.M * L > kFusedCap) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_col_fields_pairs<L>, dim3(g.N2), dim3(kFusedThreads), 0, s, g, tw.coltw, tw.fourtw, yin, U,
+                     conv, n_lines, pl, scale);
+  return hipGetLastError();
+}
+hipError_t launch_col_fields_pairs(const Geometry& g, const Twiddles& tw, const cd* yin, cd* U, cd* conv,
+                                   int n_lines, const PairLineArgs& pl, double scale, hipStream_t s) {
+  switch (g.N1) {
+    case 64: return fields_pairs_impl<64>(g, tw, yin, U, conv, n_lines, pl, scale, s);
+    case 128: return fields_pairs_impl<128>(g, tw, yin, U, conv, n_lines, pl, scale, s);
+    case 256: return fields_pairs_impl<256>(g, tw, yin, U, conv, n_lines, pl, scale, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int L>
+static hipError_t raman_gather_impl(const Geometry& g, const Twiddles& tw, const cd* U, const cd* conv, cd* yout,
+                                    const GatherArgs& ga, const FusedArgs& fa, double scale, hipStream_t s) {
+  using F = FusedCfg<L>;
+  // the largest chunk holds at most two slots per line
+  if (F::FFT + g.M * L + 2 * fa.chunk_lines * L > kFusedCap) return hipErrorInvalidValue;
+  if ((g.M + F::PG - 1) / F::PG > kFusedMaxPW) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_col_raman_gather<L>, dim3(g.N2), dim3(kFusedThreads), 0, s, g, tw.coltw, tw.fourtw, U, conv,
+                     yout, ga, fa, scale);
+  return hipGetLastError();
+}
+hipError_t launch_col_raman_gather(const Geometry& g, const Twiddles& tw, const cd* U, const cd* conv, cd* yout,
+                                   const GatherArgs& ga, const FusedArgs& fa, double scale, hipStream_t s) {
+  switch (g.N1) {
+    case 64: return raman_gather_impl<64>(g, tw, U, conv, yout, ga, fa, scale, s);
+    case 128: return raman_gather_impl<128>(g, tw, U, conv, yout, ga, fa, scale, s);
+    case 256: return raman_gather_impl<256>(g, tw, U, conv, yout, ga, fa, scale, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+#define MMP_DENSE_CASE(MV)                                                                                  \
+  case MV:                                                                                                   \
+    hipLaunchKernelGGL(k_gather_dense<MV>, dim3(g.N / 64), dim3(256), 0, s, g, U, VR, sigma, a, Qd, chunk); \
+    break;
+
+hipError_t launch_gather_dense(const Geometry& g, const cd* U, const cd* VR, cd* sigma, const GatherArgs& a,
+                               const double* Qd, hipStream_t s) {
+  const int chunk = dense_gather_chunk(g.M);
+  if (!chunk || g.N % 64) return hipErrorInvalidValue;
+  switch (g.M) {
+    MMP_DENSE_CASE(1) MMP_DENSE_CASE(2) MMP_DENSE_CASE(3) MMP_DENSE_CASE(4) MMP_DENSE_CASE(5) MMP_DENSE_CASE(6)
+    MMP_DENSE_CASE(7) MMP_DENSE_CASE(8) MMP_DENSE_CASE(9) MMP_DENSE_CASE(10) MMP_DENSE_CASE(11) MMP_DENSE_CASE(12)
+    MMP_DENSE_CASE(13) MMP_DENSE_CASE(14) MMP_DENSE_CASE(15) MMP_DENSE_CASE(16)
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -315,6 +315,34 @@ mmp_status prepare_problem(const mmp_params* in, const mmp_options& opt, Problem
       P.slot_kind[j] = (pos % 2 == 0) ? 2u : 3u;  // real / imaginary part
     }
   }
+  P.pair_a_ref = P.slot_a;
+  P.pair_b_ref = P.slot_b;
+  // Internal slot order: slots grouped by Raman line (a line's one or two slots are
+  // consecutive), so a chunk of lines owns a contiguous slot range on the device.
+  {
+    std::vector<uint32_t> order(P.P);
+    for (uint32_t j = 0; j < P.P; j++) order[j] = j;
+    if (P.raman)
+      std::stable_sort(order.begin(), order.end(),
+                       [&](uint32_t x, uint32_t y) { return P.slot_line[x] < P.slot_line[y]; });
+    std::vector<uint32_t> newid(P.P);
+    for (uint32_t j = 0; j < P.P; j++) newid[order[j]] = j;
+    auto permute = [&](std::vector<uint32_t>& v) {
+      std::vector<uint32_t> w(v.size());
+      for (uint32_t j = 0; j < P.P; j++) w[j] = v[order[j]];
+      v.swap(w);
+    };
+    permute(P.slot_a);
+    permute(P.slot_b);
+    permute(P.slot_line);
+    permute(P.slot_kind);
+    for (uint32_t e = 0; e < nnz; e++) P.slot[e] = newid[P.slot[e]];
+    const size_t nl = P.line_a.size();
+    P.line_slot0.assign(nl + 1, P.P);
+    for (uint32_t j = P.P; j-- > 0;)
+      if (P.raman) P.line_slot0[P.slot_line[j]] = j;
+    if (nl) P.line_slot0[nl] = P.P;
+  }
   // Kerr gather: W = Q * (kv * V + kr * VR) * U_k with (kv, kr) = (1 - fR, fR T0) when Raman is
   // on (src/MultimodePropagation_CPU.cpp:223-226) and (1, 0) when off (:152-154).
   // Within each Q^(p) the nonzeros are sorted by (slot, k) so the device reuses a slot's mix
@@ -354,6 +382,29 @@ mmp_status prepare_problem(const mmp_params* in, const mmp_options& opt, Problem
   }
   P.kv = P.raman ? 1 - P.fR : 1.;
   P.kr = P.raman ? P.fR * P.T0 : 0.;
+  // Fused column passes (Raman on, column length N1 >= 64): Raman lines are processed in
+  // chunks of fused_lines; chunk c covers slots [fused_slot[c], fused_slot[c+1]).
+  P.fused_lines = 0;
+  P.fused_chunks = 0;
+  if (P.raman && P.N1 >= 64) {
+    const int nl = (int)P.line_a.size();
+    const int room = fused_lds_capacity() - fused_fft_slots() - (int)(M * P.N1);
+    const int cl = room > 0 ? room / (2 * (int)P.N1) : 0;
+    if (cl >= 1 && (int)M <= fused_max_modes((int)P.N1) && nl > 0) {
+      P.fused_lines = cl < nl ? cl : nl;
+      P.fused_chunks = (nl + P.fused_lines - 1) / P.fused_lines;
+      P.fused_slot.assign(P.fused_chunks + 1, P.P);
+      for (int c = 0; c < P.fused_chunks; c++) P.fused_slot[c] = P.line_slot0[c * P.fused_lines];
+      P.fused_coff.assign((size_t)M * (P.fused_chunks + 1), 0);
+      for (uint32_t p = 0; p < M; p++) {
+        uint32_t e = P.off[p];
+        for (int c = 0; c <= P.fused_chunks; c++) {
+          while (e < P.off[p + 1] && P.slot[e] < P.fused_slot[c]) e++;
+          P.fused_coff[(size_t)p * (P.fused_chunks + 1) + c] = (c == P.fused_chunks) ? P.off[p + 1] : e;
+        }
+      }
+    }
+  }
   // a stage result N_j needs a stash slot when a stage i > j + 1 reads it
   P.stash_needed.assign(P.rk.s, 0);
   for (int i = 0; i < P.rk.s; i++)

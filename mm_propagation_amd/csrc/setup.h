@@ -47,8 +47,14 @@ struct Problem {
   // pair slots (src/MultimodePropagation.cpp:139-155)
   std::vector<uint32_t> pair_codes;  // [P] first-occurrence order of l + m*M
   std::vector<uint32_t> slot_a, slot_b, slot_line, slot_kind;  // [P]
+  std::vector<uint32_t> pair_a_ref, pair_b_ref;  // [P] decoded pairs in the reference's slot order
   // Raman convolution lines
   std::vector<uint32_t> line_a, line_b, line_self;  // [n_lines]
+  std::vector<uint32_t> line_slot0;                 // [n_lines + 1] first device slot of each line
+  // fused column passes: Raman lines per LDS chunk, chunk slot bounds, per-chunk nonzero ranges
+  int fused_lines = 0, fused_chunks = 0;
+  std::vector<uint32_t> fused_slot;                 // [fused_chunks + 1]
+  std::vector<uint32_t> fused_coff;                 // [M][fused_chunks + 1]
   // Kerr gather: nonzeros of each Q^(p) sorted by (slot, k), with per-chunk ranges
   std::vector<uint32_t> off, k, slot;  // [M+1], [nnz], [nnz]
   int gather_cap = 0, gather_chunk = 0, gather_chunks = 0;

@@ -85,6 +85,9 @@ struct mmp_solver {
   std::vector<HeadArgs> heads;  // s + 1 entries
   GatherArgs gather{};
   PairLineArgs pairs{};
+  FusedArgs fused{};
+  bool use_fused = false;
+  double* qdense = nullptr;  // [P][M][M] dense coupling blocks for the register gather (M <= 12)
   double z_host = 0;
   uint32_t steps_taken = 0;
   bool graphs = true;
@@ -192,6 +195,17 @@ mmp_status enqueue_step(mmp_solver* s) {
     MMP_TRY(run_kernel(s, i == P.rk.s ? "row_close_step" : "row_stage_head", hb,
                        [&] { return launch_row_head(g, s->tw, h, st); }));
     if (i == P.rk.s) break;
+    if (s->use_fused) {
+      MMP_TRY(run_kernel(s, "col_fields_pairs", cb * (3 * MN + LN), [&] {
+        return launch_col_fields_pairs(g, s->tw, s->ybuf, s->U, s->conv, s->n_lines, s->pairs, invN, st);
+      }));
+      MMP_TRY(run_kernel(s, "row_raman", cb * (2 * LN + P.N),
+                         [&] { return launch_row_raman(g, s->tw, s->conv, s->n_lines, s->hspec, P.delta_t, st); }));
+      MMP_TRY(run_kernel(s, "col_raman_gather", cb * (2 * MN + LN), [&] {
+        return launch_col_raman_gather(g, s->tw, s->U, s->conv, s->ybuf, s->gather, s->fused, invN, st);
+      }));
+      continue;
+    }
     MMP_TRY(run_kernel(s, "col_inverse_fields", 2 * cb * MN,
                        [&] { return launch_col_inverse(g, s->tw, s->ybuf, s->U, P.M, invN, 1.0, st); }));
     if (P.raman) {
@@ -204,8 +218,10 @@ mmp_status enqueue_step(mmp_solver* s) {
         return launch_col_inverse(g, s->tw, s->conv, s->conv, s->n_lines, invN, 1.0, st);
       }));
     }
-    MMP_TRY(run_kernel(s, "gather_kerr_raman", cb * (2 * MN + LN),
-                       [&] { return launch_gather(g, s->U, s->conv, s->ybuf, s->gather, st); }));
+    MMP_TRY(run_kernel(s, "gather_kerr_raman", cb * (2 * MN + LN), [&] {
+      return s->qdense ? launch_gather_dense(g, s->U, s->conv, s->ybuf, s->gather, s->qdense, st)
+                       : launch_gather(g, s->U, s->conv, s->ybuf, s->gather, st);
+    }));
     MMP_TRY(run_kernel(s, "col_forward_sigma", 2 * cb * MN,
                        [&] { return launch_col_forward(g, s->tw, s->ybuf, s->ybuf, P.M, st); }));
   }
@@ -296,6 +312,8 @@ size_t planned_device_bytes(const Problem& P) {
   if (any_stash) b += 16 * (size_t)P.rk.s * MN;
   if (P.raman) b += 16 * c(L) * P.N + 16 * (size_t)P.N + 8 * (size_t)P.N;
   b += 4 * c(P.coff.size()) + 4 * c(P.nnz) + 8 * c(P.nnz) + 4 * 4 * c(P.P) + 4 * 3 * c(L);
+  if (P.fused_chunks > 0) b += 4 * (P.line_slot0.size() + P.fused_slot.size() + P.fused_coff.size());
+  if (dense_gather_chunk((int)P.M) > 0) b += 8 * (size_t)P.P * P.M * P.M;
   return b;
 }
 
@@ -419,6 +437,27 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
   s->gather = GatherArgs{d_coff, d_k, d_q, d_sa, d_sb, d_sl, d_sk, (int)P.P, P.gather_chunk,
                          P.gather_chunks, P.gather_cap, P.raman, P.kv, P.kr};
   s->pairs = PairLineArgs{d_la, d_lb, d_ls};
+  std::vector<double> qd;
+  if (dense_gather_chunk((int)P.M) > 0) {
+    qd.assign((size_t)P.P * P.M * P.M, 0.);
+    for (uint32_t p = 0; p < P.M; p++)
+      for (uint32_t e = P.off[p]; e < P.off[p + 1]; e++) qd[((size_t)P.slot[e] * P.M + p) * P.M + P.k[e]] += P.q[e];
+    A(dalloc(s, &s->qdense, qd.size()));
+    if (st != MMP_OK) return cleanup(st);
+    A(upload(s, s->qdense, qd.data(), qd.size() * sizeof(double)));
+  }
+  if (P.fused_chunks > 0 && !(opt.flags & MMP_FLAG_UNFUSED)) {
+    uint32_t *d_ls0, *d_cs, *d_fc;
+    A(dalloc(s, &d_ls0, P.line_slot0.size()));
+    A(dalloc(s, &d_cs, P.fused_slot.size()));
+    A(dalloc(s, &d_fc, P.fused_coff.size()));
+    if (st != MMP_OK) return cleanup(st);
+    A(upload(s, d_ls0, P.line_slot0.data(), P.line_slot0.size() * 4));
+    A(upload(s, d_cs, P.fused_slot.data(), P.fused_slot.size() * 4));
+    A(upload(s, d_fc, P.fused_coff.data(), P.fused_coff.size() * 4));
+    s->fused = FusedArgs{d_ls0, d_cs, d_fc, s->n_lines, P.fused_lines, P.fused_chunks};
+    s->use_fused = true;
+  }
   // stage heads
   const double h = P.delta_z;
   for (int i = 0; i <= P.rk.s; i++) {
@@ -683,8 +722,8 @@ mmp_status mmp_plan_arrays(const mmp_params* params, const mmp_options* options,
           const uint32_t k1 = j % P.N1, k2 = j / P.N1;
           imL_out[(size_t)p * P.N + j] = P.imL_fourstep[(size_t)p * P.N + (size_t)k1 * P.N2 + k2];
         }
-    if (pair_a_out) std::memcpy(pair_a_out, P.slot_a.data(), P.P * sizeof(uint32_t));
-    if (pair_b_out) std::memcpy(pair_b_out, P.slot_b.data(), P.P * sizeof(uint32_t));
+    if (pair_a_out) std::memcpy(pair_a_out, P.pair_a_ref.data(), P.P * sizeof(uint32_t));
+    if (pair_b_out) std::memcpy(pair_b_out, P.pair_b_ref.data(), P.P * sizeof(uint32_t));
     const size_t MN = (size_t)P.M * P.N;
     for (uint32_t p = 0; p < P.M; p++)
       for (uint32_t j = 0; j < P.N; j++) {

@@ -85,3 +85,24 @@ def test_two_mode_gaussian_full_length():
     sp = configs.two_mode_gaussian()
     err, got, ref = run_pair(sp, sp.n_steps, n_threads=8)
     assert np.all(err <= TOL_FULL), err
+
+
+@pytest.mark.parametrize("M,nt", [(2, 4096), (6, 16384), (10, 65536), (15, 8192)])
+def test_fused_column_passes_match_separate_passes(M, nt):
+    """The fused column passes (4 per stage) against the separate ones (7 per stage), and
+    both against the oracle; 10 and 15 modes run the Raman lines in several LDS chunks."""
+    require_device()
+    sp = configs.two_mode_gaussian(nt=nt) if M == 2 else configs.grin_fiber(M=M, nt=nt, symmetric=True)
+    dec = 0 if M == 2 else 1
+    note(f"case fused vs separate M={M} NT={nt}")
+    a = MultimodePropagationGPU_RamanON(sp, pair_decode=dec)
+    b = MultimodePropagationGPU_RamanON(sp, pair_decode=dec, flags=_capi.MMP_FLAG_UNFUSED)
+    a.computeLawsonRK(5)
+    b.computeLawsonRK(5)
+    ra, rb = a.getResult(), b.getResult()
+    a.close()
+    b.close()
+    assert np.all(configs.rel_l2_per_mode(ra, rb) <= 1e-14)
+    o = ob.Oracle(sp, pair_decode=dec)
+    o.compute_lawson_rk(5)
+    assert np.all(configs.rel_l2_per_mode(ra, o.get_result()) <= TOL_SHORT)

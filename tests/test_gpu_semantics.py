@@ -50,7 +50,7 @@ def test_resumable_advance_equals_single_run():
 
 def test_default_step_count_and_info():
     require_device()
-    sp = configs.two_mode_gaussian(nt=256, n_steps=40)
+    sp = configs.two_mode_gaussian(nt=256, n_steps=40, fiber_length=0.004)  # same h as the 0.5 m run
     note("case default steps")
     g = MultimodePropagationGPU_RamanON(sp)
     i = g.info()
