@@ -145,6 +145,13 @@ MMP_API mmp_status mmp_plan_arrays(const mmp_params* params, const mmp_options* 
                            double* imL_out, uint32_t* pair_a_out, uint32_t* pair_b_out, double* e1_out,
                            double* e_out);
 
+/* Host-only: the Butcher tableau the solver uses for method_order
+ * (src/MultimodePropagation.cpp:15-27 with src/RungeKutta.cpp:3-72,125-175):
+ * *stages_out = s (4, 7 or 11); a_out is [11][11] row-major (a_ij for j < i, zero
+ * elsewhere), b_out and c_out are [11]. */
+MMP_API mmp_status mmp_tableau(uint32_t method_order, uint32_t* stages_out, double* a_out, double* b_out,
+                               double* c_out);
+
 /* Device context: one HIP stream on one device. No reference counterpart:
  * the reference hard-codes device 0 and the default stream
  * (src/MultimodePropagation_GPU.cu:17). */

@@ -1,0 +1,72 @@
+// MultimodePropagation_GPU.hpp -- the drop-in solver classes for gfx950
+// (names, constructors and methods of include/MultimodePropagation_GPU.cuh:10-85 of
+// MM_Propagation), implemented over the C ABI of include/mmprop.h.
+//
+// Errors are reported by throwing MmpropError (the reference has no error reporting);
+// destruction releases the device memory (the destructor is virtual through the base).
+#ifndef MMPROP_MULTIMODE_PROPAGATION_GPU_HPP
+#define MMPROP_MULTIMODE_PROPAGATION_GPU_HPP
+
+#include <stdexcept>
+#include <string>
+
+#include "MultimodePropagation.hpp"
+
+struct mmp_ctx;
+struct mmp_solver;
+
+class MmpropError : public std::runtime_error {
+ public:
+  MmpropError(int status, const std::string& what) : std::runtime_error(what), status_(status) {}
+  int status() const { return status_; }
+
+ private:
+  int status_;
+};
+
+// Choices with no counterpart in the reference (defaults reproduce its CPU behaviour).
+struct MultimodePropagationGPUOptions {
+  unsigned int pair_decode = 0;  // MMP_PAIRS_REFERENCE_CPU (0) or MMP_PAIRS_PHYSICAL (1)
+  unsigned int flags = 0;        // MMP_FLAG_* of include/mmprop.h
+  int device = 0;
+};
+
+class MultimodePropagationGPU : public MultimodePropagation {
+ public:
+  ~MultimodePropagationGPU() override;
+  MultimodePropagationGPU(const MultimodePropagationGPU&) = delete;
+  MultimodePropagationGPU& operator=(const MultimodePropagationGPU&) = delete;
+
+  void computeLawsonRK(const unsigned int nz) override;
+  void computeLawsonRK() override;
+  ComplexArraysContainerCPU getResult() override;
+
+  // Extensions: resumable stepping, per-mode energy (W ps), current z.
+  void advance(const unsigned int n);
+  std::vector<double> getEnergy();
+  double getZ();
+
+ protected:
+  MultimodePropagationGPU(const SimulationParameters& in, bool raman, const MultimodePropagationGPUOptions& opt);
+
+ private:
+  mmp_ctx* ctx_;
+  mmp_solver* solver_;
+  unsigned int M_, nt_;
+};
+
+class MultimodePropagationGPU_RamanOFF : public MultimodePropagationGPU {
+ public:
+  explicit MultimodePropagationGPU_RamanOFF(const SimulationParameters& in,
+                                            const MultimodePropagationGPUOptions& opt = MultimodePropagationGPUOptions())
+      : MultimodePropagationGPU(in, false, opt) {}
+};
+
+class MultimodePropagationGPU_RamanON : public MultimodePropagationGPU {
+ public:
+  explicit MultimodePropagationGPU_RamanON(const SimulationParameters& in,
+                                           const MultimodePropagationGPUOptions& opt = MultimodePropagationGPUOptions())
+      : MultimodePropagationGPU(in, true, opt) {}
+};
+
+#endif

@@ -106,6 +106,7 @@ EXPORTS = [
     ("mmp_kernel_times", c_int, [c_void_p, POINTER(MmpKernelTime), c_uint32, POINTER(c_uint32)]),
     ("mmp_ctx_synchronize", c_int, [c_void_p]),
     ("mmp_plan", c_int, [POINTER(MmpParams), POINTER(MmpOptions), POINTER(MmpInfo)]),
+    ("mmp_tableau", c_int, [c_uint32, POINTER(c_uint32), POINTER(c_double), POINTER(c_double), POINTER(c_double)]),
     ("mmp_plan_arrays", c_int, [POINTER(MmpParams), POINTER(MmpOptions), POINTER(c_double), POINTER(c_double),
                                 POINTER(c_uint32), POINTER(c_uint32), POINTER(c_double), POINTER(c_double)]),
 ]

@@ -75,6 +75,25 @@ def build_dropin_library() -> str:
     return out
 
 
+CPP_TESTS = ["dropin_propagate", "dropin_selftest"]
+
+
+def build_cpp_tests() -> list:
+    """The C++ programs under tests/cpp that exercise the drop-in API (tests/cpp/_build/)."""
+    src_dir = os.path.join(REPO, "tests", "cpp")
+    out_dir = os.path.join(src_dir, "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    cxx = shutil.which("g++") or "g++"
+    outs = []
+    for name in CPP_TESTS:
+        out = os.path.join(out_dir, name)
+        _run([cxx, "-std=c++11", "-O2", "-Wall", "-I" + os.path.join(REPO, "include", "mmprop"),
+              os.path.join(src_dir, name + ".cpp"), "-o", out, "-L" + LIBDIR, "-lmmprop", "-lmmprop_hip",
+              "-Wl,-rpath,$ORIGIN/../../../mm_propagation_amd/lib"])
+        outs.append(out)
+    return outs
+
+
 def build_all(verbose: bool = True) -> None:
     lib = build_hip_library()
     if verbose:
@@ -82,6 +101,9 @@ def build_all(verbose: bool = True) -> None:
     d = build_dropin_library()
     if verbose and d:
         print("built", d)
+    for t in build_cpp_tests():
+        if verbose:
+            print("built", t)
 
 
 if __name__ == "__main__":

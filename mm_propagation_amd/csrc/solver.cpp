@@ -690,6 +690,18 @@ mmp_status mmp_solver_info(mmp_solver* s, mmp_info* o) {
   return MMP_OK;
 }
 
+mmp_status mmp_tableau(uint32_t method_order, uint32_t* stages_out, double* a_out, double* b_out, double* c_out) {
+  if (!stages_out || !a_out || !b_out || !c_out) return fail(MMP_ERR_INVALID_ARGUMENT, "output pointers are required");
+  const Tableau t = make_tableau(method_order);
+  *stages_out = (uint32_t)t.s;
+  for (int i = 0; i < 11; i++) {
+    for (int j = 0; j < 11; j++) a_out[i * 11 + j] = t.a[i][j];
+    b_out[i] = t.b[i];
+    c_out[i] = t.c[i];
+  }
+  return MMP_OK;
+}
+
 mmp_status mmp_plan(const mmp_params* params, const mmp_options* options, mmp_info* o) {
   if (!o) return fail(MMP_ERR_INVALID_ARGUMENT, "info_out is NULL");
   try {
