@@ -63,7 +63,8 @@ enum {
   MMP_FLAG_NONE = 0,
   MMP_FLAG_PROFILE = 1u << 0,     /* record per-kernel HIP event times (see mmp_kernel_times) */
   MMP_FLAG_NO_GRAPH = 1u << 1,    /* launch step kernels one by one instead of replaying a HIP graph */
-  MMP_FLAG_UNFUSED = 1u << 2      /* keep the column passes separate (7 passes per stage instead of 4) */
+  MMP_FLAG_FUSED_COLUMNS = 1u << 2 /* fuse the column passes around the Raman row pass (4 passes per stage
+                                      instead of 7; Raman on, N1 >= 64); separate passes are the default */
 };
 
 /*

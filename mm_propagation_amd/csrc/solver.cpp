@@ -446,7 +446,7 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
     if (st != MMP_OK) return cleanup(st);
     A(upload(s, s->qdense, qd.data(), qd.size() * sizeof(double)));
   }
-  if (P.fused_chunks > 0 && !(opt.flags & MMP_FLAG_UNFUSED)) {
+  if (P.fused_chunks > 0 && (opt.flags & MMP_FLAG_FUSED_COLUMNS)) {
     uint32_t *d_ls0, *d_cs, *d_fc;
     A(dalloc(s, &d_ls0, P.line_slot0.size()));
     A(dalloc(s, &d_cs, P.fused_slot.size()));

@@ -95,8 +95,8 @@ def test_fused_column_passes_match_separate_passes(M, nt):
     sp = configs.two_mode_gaussian(nt=nt) if M == 2 else configs.grin_fiber(M=M, nt=nt, symmetric=True)
     dec = 0 if M == 2 else 1
     note(f"case fused vs separate M={M} NT={nt}")
-    a = MultimodePropagationGPU_RamanON(sp, pair_decode=dec)
-    b = MultimodePropagationGPU_RamanON(sp, pair_decode=dec, flags=_capi.MMP_FLAG_UNFUSED)
+    a = MultimodePropagationGPU_RamanON(sp, pair_decode=dec, flags=_capi.MMP_FLAG_FUSED_COLUMNS)
+    b = MultimodePropagationGPU_RamanON(sp, pair_decode=dec)
     a.computeLawsonRK(5)
     b.computeLawsonRK(5)
     ra, rb = a.getResult(), b.getResult()
