@@ -46,7 +46,11 @@ typedef enum mmp_status {
 
 /* Precision of the propagation work (FFTs, Kerr/Raman sums). */
 enum {
-  MMP_PRECISION_FP64 = 0 /* plain FP64 solver: the drop-in for the reference */
+  MMP_PRECISION_FP64 = 0, /* plain FP64 solver: the drop-in for the reference */
+  MMP_PRECISION_MIXED = 1 /* FFTs, pair products, Raman convolutions and the Kerr sum in FP32 (twiddles
+                             computed in FP64 and rounded); the spectral state, the Lawson phases
+                             exp(c_i h L) (the large accumulated beta0 z + beta1 w z phase), the RK stage
+                             combination and accumulation, getResult and the energy stay FP64 */
 };
 
 /* Which (a, b) pair a Raman/Kerr slot j holds (V_j = U_a * conj(U_b)).

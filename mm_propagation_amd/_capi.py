@@ -15,6 +15,9 @@ LIB_NAME = "libmmprop_hip.so"
 LIB_PATH = os.path.join(PKG_DIR, "lib", LIB_NAME)
 
 MMP_OK = 0
+MMP_ERR_INVALID_ARGUMENT = 1
+MMP_ERR_UNSUPPORTED = 2
+MMP_ERR_NONFINITE = 6
 STATUS_NAMES = {
     0: "MMP_OK",
     1: "MMP_ERR_INVALID_ARGUMENT",
@@ -27,6 +30,7 @@ STATUS_NAMES = {
 }
 
 MMP_PRECISION_FP64 = 0
+MMP_PRECISION_MIXED = 1
 MMP_PAIRS_REFERENCE_CPU = 0
 MMP_PAIRS_PHYSICAL = 1
 MMP_FLAG_PROFILE = 1 << 0

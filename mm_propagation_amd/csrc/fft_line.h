@@ -16,56 +16,87 @@
 
 namespace mmp {
 
-struct cd {
-  double re, im;
+// Complex value of the work precision T (double, or float for the mixed-precision path).
+template <class T>
+struct cpx {
+  T re, im;
 };
+using cd = cpx<double>;
+using cf = cpx<float>;
 
-__device__ __forceinline__ cd cmul(cd a, cd b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
-__device__ __forceinline__ cd cmulc(cd a, cd b) {  // a * conj(b)
+template <class T>
+__device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <class T>
+__device__ __forceinline__ cpx<T> cmulc(cpx<T> a, cpx<T> b) {  // a * conj(b)
   return {a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im};
 }
-__device__ __forceinline__ cd cadd(cd a, cd b) { return {a.re + b.re, a.im + b.im}; }
-__device__ __forceinline__ cd csub(cd a, cd b) { return {a.re - b.re, a.im - b.im}; }
-__device__ __forceinline__ cd cscale(cd a, double s) { return {a.re * s, a.im * s}; }
-__device__ __forceinline__ cd cconj(cd a) { return {a.re, -a.im}; }
+template <class T>
+__device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) {
+  return {a.re + b.re, a.im + b.im};
+}
+template <class T>
+__device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) {
+  return {a.re - b.re, a.im - b.im};
+}
+template <class T, class S>
+__device__ __forceinline__ cpx<T> cscale(cpx<T> a, S s) {
+  const T f = (T)s;
+  return {a.re * f, a.im * f};
+}
+template <class T>
+__device__ __forceinline__ cpx<T> cconj(cpx<T> a) {
+  return {a.re, -a.im};
+}
+// precision conversions (float -> double is exact; double -> float rounds to nearest)
+__device__ __forceinline__ cd widen(cf a) { return {(double)a.re, (double)a.im}; }
+__device__ __forceinline__ cd widen(cd a) { return a; }
+template <class T>
+__device__ __forceinline__ cpx<T> narrow(cd a) {
+  return {(T)a.re, (T)a.im};
+}
 
 // multiply by (SIGN * i)
-template <int SIGN>
-__device__ __forceinline__ cd mul_si(cd a) {
-  return SIGN < 0 ? cd{a.im, -a.re} : cd{-a.im, a.re};
+template <int SIGN, class T>
+__device__ __forceinline__ cpx<T> mul_si(cpx<T> a) {
+  return SIGN < 0 ? cpx<T>{a.im, -a.re} : cpx<T>{-a.im, a.re};
 }
 
 // DFT of length 2, 4, 8 with kernel exp(SIGN * 2*pi*i*m*q/R), in place, natural order.
-template <int SIGN>
-__device__ __forceinline__ void dft2(cd& a0, cd& a1) {
-  cd t = a0;
+template <int SIGN, class T>
+__device__ __forceinline__ void dft2(cpx<T>& a0, cpx<T>& a1) {
+  cpx<T> t = a0;
   a0 = cadd(t, a1);
   a1 = csub(t, a1);
 }
 
-template <int SIGN>
-__device__ __forceinline__ void dft4(cd& a0, cd& a1, cd& a2, cd& a3) {
-  cd c0 = cadd(a0, a2), c1 = csub(a0, a2);
-  cd d0 = cadd(a1, a3), d1 = mul_si<SIGN>(csub(a1, a3));
+template <int SIGN, class T>
+__device__ __forceinline__ void dft4(cpx<T>& a0, cpx<T>& a1, cpx<T>& a2, cpx<T>& a3) {
+  cpx<T> c0 = cadd(a0, a2), c1 = csub(a0, a2);
+  cpx<T> d0 = cadd(a1, a3), d1 = mul_si<SIGN>(csub(a1, a3));
   a0 = cadd(c0, d0);
   a2 = csub(c0, d0);
   a1 = cadd(c1, d1);
   a3 = csub(c1, d1);
 }
 
-template <int SIGN>
-__device__ __forceinline__ void dft8(cd* v) {
-  constexpr double r = 0.70710678118654752440;  // cos(pi/4)
-  cd a[4], b[4];
+template <int SIGN, class T>
+__device__ __forceinline__ void dft8(cpx<T>* v) {
+  constexpr T r = (T)0.70710678118654752440L;  // cos(pi/4), rounded once to T
+  cpx<T> a[4], b[4];
 #pragma unroll
   for (int m = 0; m < 4; m++) {
     a[m] = cadd(v[m], v[m + 4]);
     b[m] = csub(v[m], v[m + 4]);
   }
   // b[m] *= w8^m, w8 = exp(SIGN*i*pi/4)
-  b[1] = cscale(SIGN < 0 ? cd{b[1].re + b[1].im, b[1].im - b[1].re} : cd{b[1].re - b[1].im, b[1].im + b[1].re}, r);
+  b[1] = cscale(SIGN < 0 ? cpx<T>{b[1].re + b[1].im, b[1].im - b[1].re} : cpx<T>{b[1].re - b[1].im, b[1].im + b[1].re},
+                r);
   b[2] = mul_si<SIGN>(b[2]);
-  b[3] = cscale(SIGN < 0 ? cd{-b[3].re + b[3].im, -b[3].im - b[3].re} : cd{-b[3].re - b[3].im, -b[3].im + b[3].re}, r);
+  b[3] = cscale(SIGN < 0 ? cpx<T>{-b[3].re + b[3].im, -b[3].im - b[3].re}
+                         : cpx<T>{-b[3].re - b[3].im, -b[3].im + b[3].re},
+                r);
   dft4<SIGN>(a[0], a[1], a[2], a[3]);
   dft4<SIGN>(b[0], b[1], b[2], b[3]);
 #pragma unroll
@@ -75,9 +106,9 @@ __device__ __forceinline__ void dft8(cd* v) {
   }
 }
 
-template <int SIGN>
-__device__ __forceinline__ cd twiddle(const cd* __restrict__ tw, int j) {
-  cd w = tw[j];
+template <int SIGN, class T>
+__device__ __forceinline__ cpx<T> twiddle(const cpx<T>* __restrict__ tw, int j) {
+  cpx<T> w = tw[j];
   if (SIGN > 0) w.im = -w.im;
   return w;
 }
@@ -113,20 +144,20 @@ struct LineSlots {
 //   then the trailing stage, butterflies j = t + b*T:
 //     radix-4: W_L^(j*m), m = 1..3 (3 per butterfly, b = 0, 1)
 //     radix-2: W_L^j (b = 0..3)
-template <int L>
+template <int L, class T = double>
 struct LineTw {
   static constexpr int n8 = LinePlan<L>::n8;
   static constexpr int tail = LinePlan<L>::tail;
   static constexpr int base = (n8 == 2) ? 7 : 0;
   static constexpr int count = (L == 8) ? 1 : base + (tail == 4 ? 6 : (tail == 2 ? 4 : 0));
-  cd w[count];
+  cpx<T> w[count];
 };
 
 // Loads the forward-sign twiddles (the inverse transform conjugates them on use).
-template <int L>
-__device__ __forceinline__ void load_line_tw(LineTw<L>& lt, const cd* __restrict__ tw, int t) {
+template <int L, class T>
+__device__ __forceinline__ void load_line_tw(LineTw<L, T>& lt, const cpx<T>* __restrict__ tw, int t) {
   using P = LinePlan<L>;
-  constexpr int T = P::T;
+  constexpr int LT = P::T;
   if constexpr (L == 8) {
     (void)tw;
     (void)t;
@@ -137,31 +168,31 @@ __device__ __forceinline__ void load_line_tw(LineTw<L>& lt, const cd* __restrict
 #pragma unroll
       for (int m = 1; m < 8; m++) lt.w[m - 1] = tw[k * m * (L / 64)];
     }
-    constexpr int b0 = LineTw<L>::base;
+    constexpr int b0 = LineTw<L, T>::base;
     if constexpr (P::tail == 4) {
 #pragma unroll
       for (int b = 0; b < 2; b++)
 #pragma unroll
-        for (int m = 1; m < 4; m++) lt.w[b0 + 3 * b + m - 1] = tw[(t + b * T) * m];
+        for (int m = 1; m < 4; m++) lt.w[b0 + 3 * b + m - 1] = tw[(t + b * LT) * m];
     } else if constexpr (P::tail == 2) {
 #pragma unroll
-      for (int b = 0; b < 4; b++) lt.w[b0 + b] = tw[t + b * T];
+      for (int b = 0; b < 4; b++) lt.w[b0 + b] = tw[t + b * LT];
     }
   }
 }
 
-template <int SIGN>
-__device__ __forceinline__ cd tw_dir(cd w) {
-  return SIGN < 0 ? w : cd{w.re, -w.im};
+template <int SIGN, class T>
+__device__ __forceinline__ cpx<T> tw_dir(cpx<T> w) {
+  return SIGN < 0 ? w : cpx<T>{w.re, -w.im};
 }
 
 // In-place transform of the line held as v[i] = x[t + i*T]; sm points at the
 // line's first LDS slot and element pos lives at sm[lds_pos<STRIDE, PAD>(pos)].
 // All lanes of the workgroup must call this together.
-template <int L, int SIGN, int STRIDE, int PAD>
-__device__ __forceinline__ void line_fft(cd* v, cd* sm, const LineTw<L>& lt, int t) {
+template <int L, int SIGN, int STRIDE, int PAD, class T>
+__device__ __forceinline__ void line_fft(cpx<T>* v, cpx<T>* sm, const LineTw<L, T>& lt, int t) {
   using P = LinePlan<L>;
-  constexpr int T = P::T;
+  constexpr int LT = P::T;
   // stage 1: radix-8, span 1, inputs x[t + m*T]
   dft8<SIGN>(v);
   if constexpr (L == 8) {
@@ -175,7 +206,7 @@ __device__ __forceinline__ void line_fft(cd* v, cd* sm, const LineTw<L>& lt, int
     for (int q = 0; q < 8; q++) sm[lds_pos<STRIDE, PAD>(8 * t + q)] = v[q];
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < 8; i++) v[i] = sm[lds_pos<STRIDE, PAD>(t + i * T)];
+    for (int i = 0; i < 8; i++) v[i] = sm[lds_pos<STRIDE, PAD>(t + i * LT)];
     if constexpr (P::n8 == 2) {
       // radix-8 stage of span 8
 #pragma unroll
@@ -183,7 +214,7 @@ __device__ __forceinline__ void line_fft(cd* v, cd* sm, const LineTw<L>& lt, int
       dft8<SIGN>(v);
       if constexpr (P::tail == 1) {
         __syncthreads();
-        return;  // outputs at positions t + q*T
+        return;  // outputs at positions t + q*LT
       }
       __syncthreads();
       const int k = t % 8;
@@ -192,14 +223,14 @@ __device__ __forceinline__ void line_fft(cd* v, cd* sm, const LineTw<L>& lt, int
       for (int q = 0; q < 8; q++) sm[lds_pos<STRIDE, PAD>(base + q * 8)] = v[q];
       __syncthreads();
 #pragma unroll
-      for (int i = 0; i < 8; i++) v[i] = sm[lds_pos<STRIDE, PAD>(t + i * T)];
+      for (int i = 0; i < 8; i++) v[i] = sm[lds_pos<STRIDE, PAD>(t + i * LT)];
     }
     __syncthreads();
-    constexpr int b0 = LineTw<L>::base;
+    constexpr int b0 = LineTw<L, T>::base;
     if constexpr (P::tail == 4) {
 #pragma unroll
       for (int b = 0; b < 2; b++) {
-        cd a0 = v[b], a1 = cmul(v[b + 2], tw_dir<SIGN>(lt.w[b0 + 3 * b])),
+        cpx<T> a0 = v[b], a1 = cmul(v[b + 2], tw_dir<SIGN>(lt.w[b0 + 3 * b])),
            a2 = cmul(v[b + 4], tw_dir<SIGN>(lt.w[b0 + 3 * b + 1])), a3 = cmul(v[b + 6], tw_dir<SIGN>(lt.w[b0 + 3 * b + 2]));
         dft4<SIGN>(a0, a1, a2, a3);
         v[b] = a0;
@@ -210,7 +241,7 @@ __device__ __forceinline__ void line_fft(cd* v, cd* sm, const LineTw<L>& lt, int
     } else if constexpr (P::tail == 2) {
 #pragma unroll
       for (int b = 0; b < 4; b++) {
-        cd a0 = v[b], a1 = cmul(v[b + 4], tw_dir<SIGN>(lt.w[b0 + b]));
+        cpx<T> a0 = v[b], a1 = cmul(v[b + 4], tw_dir<SIGN>(lt.w[b0 + b]));
         dft2<SIGN>(a0, a1);
         v[b] = a0;
         v[b + 4] = a1;

@@ -25,11 +25,29 @@ struct Geometry {
   int N, N1, N2, M;
 };
 
-// Twiddle tables on the device.
+// Twiddle tables on the device, FP64 and (mixed precision only) the same values rounded to FP32.
 struct Twiddles {
   const cd* rowtw;   // W_N2^j, j < N2
   const cd* coltw;   // W_N1^j, j < N1
   const cd* fourtw;  // W_N^e, e < N
+  const cf* rowtw32;
+  const cf* coltw32;
+  const cf* fourtw32;
+};
+
+template <class W>
+struct TwOf;
+template <>
+struct TwOf<double> {
+  static const cd* row(const Twiddles& t) { return t.rowtw; }
+  static const cd* col(const Twiddles& t) { return t.coltw; }
+  static const cd* four(const Twiddles& t) { return t.fourtw; }
+};
+template <>
+struct TwOf<float> {
+  static const cf* row(const Twiddles& t) { return t.rowtw32; }
+  static const cf* col(const Twiddles& t) { return t.coltw32; }
+  static const cf* four(const Twiddles& t) { return t.fourtw32; }
 };
 
 // Per-step device state updated on the stream (z by repeated FP64 addition).
@@ -43,12 +61,14 @@ struct StepState {
 // (i == s finishes the step). Lawson tables E_u = exp(c_u h L) * E1^n (one per
 // distinct c_u, four-step order) are advanced by the head that last reads them
 // in a step: E_u <- E_u * E1, as src/MultimodePropagation_CPU.cpp:37-44 does.
+// The stage head works in FP64 on the spectral state; its row DFTs run in the work
+// precision W of the step (ysig and yout are cpx<W> arrays).
 struct HeadArgs {
-  const cd* ysig;      // column-forwarded sigma of stage i-1 (unused when has_prev == 0)
+  const void* ysig;    // column-forwarded sigma of stage i-1 (unused when has_prev == 0)
   cd* psi_hat;         // spectral state
   cd* acc;             // spectral RK accumulator
   cd* stash;           // [s][M][N] spectral N_j kept for later stages
-  cd* yout;            // row-inverse output for stage i (unused when final)
+  void* yout;          // row-inverse output for stage i (unused when final)
   cd* e_prev;          // Lawson table of stage i-1 (conjugated for N_{i-1})
   const cd* e_cur;     // Lawson table of stage i
   const cd* e1;        // exp(h L)
@@ -69,7 +89,7 @@ struct HeadArgs {
 struct GatherArgs {
   const uint32_t* coff;    // [M][n_chunks+1] nonzero ranges of Q^(p) per slot chunk (sorted by slot)
   const uint32_t* kslot;   // [nnz] k | slot << 8
-  const double* q;         // [nnz] Q / max|Q|
+  const void* q;           // [nnz] Q / max|Q| in the work precision
   const uint32_t* slot_a;  // [P]
   const uint32_t* slot_b;  // [P]
   const uint32_t* slot_line;  // [P] Raman conv line
@@ -105,22 +125,29 @@ int fused_lds_capacity();
 int fused_fft_slots();
 int fused_max_modes(int L);
 
-// Launchers (return hipSuccess or the launch error).
+// Launchers (return hipSuccess or the launch error). Those templated on W run in the
+// work precision W (double, or float for the mixed-precision step); the others are FP64.
+template <class W>
 hipError_t launch_row_head(const Geometry& g, const Twiddles& tw, const HeadArgs& a, hipStream_t s);
-hipError_t launch_col_inverse(const Geometry& g, const Twiddles& tw, const cd* in, cd* out, int lines, double scale,
-                              double divisor, hipStream_t s);
-hipError_t launch_col_forward(const Geometry& g, const Twiddles& tw, const cd* in, cd* out, int lines,
+template <class W>
+hipError_t launch_col_inverse(const Geometry& g, const Twiddles& tw, const cpx<W>* in, cpx<W>* out, int lines,
+                              double scale, double divisor, hipStream_t s);
+template <class W>
+hipError_t launch_col_forward(const Geometry& g, const Twiddles& tw, const cpx<W>* in, cpx<W>* out, int lines,
                               hipStream_t s);
 hipError_t launch_col_forward_real(const Geometry& g, const Twiddles& tw, const double* in, cd* out,
                                    hipStream_t s);
-hipError_t launch_col_pairs_forward(const Geometry& g, const Twiddles& tw, const cd* U, cd* out, int lines,
+template <class W>
+hipError_t launch_col_pairs_forward(const Geometry& g, const Twiddles& tw, const cpx<W>* U, cpx<W>* out, int lines,
                                     const PairLineArgs& pl, hipStream_t s);
 hipError_t launch_row_forward(const Geometry& g, const Twiddles& tw, cd* data, int lines, hipStream_t s);
 hipError_t launch_row_inverse_phase(const Geometry& g, const Twiddles& tw, const cd* in, cd* out,
                                     const double* imL, const StepState* st, int use_phase, hipStream_t s);
-hipError_t launch_row_raman(const Geometry& g, const Twiddles& tw, cd* data, int lines, const cd* hspec,
+template <class W>
+hipError_t launch_row_raman(const Geometry& g, const Twiddles& tw, cpx<W>* data, int lines, const cpx<W>* hspec,
                             double hscale, hipStream_t s);
-hipError_t launch_gather(const Geometry& g, const cd* U, const cd* VR, cd* sigma, const GatherArgs& a,
+template <class W>
+hipError_t launch_gather(const Geometry& g, const cpx<W>* U, const cpx<W>* VR, cpx<W>* sigma, const GatherArgs& a,
                          hipStream_t s);
 hipError_t launch_col_fields_pairs(const Geometry& g, const Twiddles& tw, const cd* yin, cd* U, cd* conv,
                                    int n_lines, const PairLineArgs& pl, double scale, hipStream_t s);
@@ -129,8 +156,11 @@ hipError_t launch_col_raman_gather(const Geometry& g, const Twiddles& tw, const 
 // Dense-block gather (M <= 16): Qd is [P][M][M] with Qd[s][p][k] = sum of Q/max|Q| over the
 // nonzeros (k, slot s) of Q^(p). dense_gather_chunk(M) is 0 when M is out of range.
 int dense_gather_chunk(int M);
-hipError_t launch_gather_dense(const Geometry& g, const cd* U, const cd* VR, cd* sigma, const GatherArgs& a,
-                               const double* Qd, hipStream_t s);
+template <class W>
+hipError_t launch_gather_dense(const Geometry& g, const cpx<W>* U, const cpx<W>* VR, cpx<W>* sigma,
+                               const GatherArgs& a, const W* Qd, hipStream_t s);
+// out[j] = in[j] rounded to FP32, j < n
+hipError_t launch_narrow(const cd* in, cf* out, size_t n, hipStream_t s);
 hipError_t launch_advance_z(StepState* st, double h, hipStream_t s);
 hipError_t launch_energy(const Geometry& g, const cd* fields, double* out, double dt, hipStream_t s);
 

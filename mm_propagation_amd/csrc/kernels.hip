@@ -63,27 +63,31 @@ __device__ __forceinline__ cd cmul_exact(cd a, cd b) {
           __dadd_rn(__dmul_rn(a.re, b.im), __dmul_rn(a.im, b.re))};
 }
 
-template <int L>
-__global__ void __launch_bounds__(64) k_row_head(Geometry g, const cd* __restrict__ rowtw, HeadArgs a) {
+template <int L, class W>
+__global__ void __launch_bounds__(64) k_row_head(Geometry g, const cpx<W>* __restrict__ rowtw, HeadArgs a) {
   using C = RowCfg<L>;
-  __shared__ cd smem[C::G * C::SLOTS];
+  using cw = cpx<W>;
+  __shared__ cw smem[C::G * C::SLOTS];
   const int tid = threadIdx.x;
   const int gl = tid / C::T, t = tid % C::T;
   const int row = blockIdx.x * C::G + gl;
   const bool valid = row < g.M * g.N1;
   const size_t base = valid ? (size_t)row * g.N2 + t : t;
-  cd* sm = smem + gl * C::SLOTS;
+  cw* sm = smem + gl * C::SLOTS;
   const size_t MN = (size_t)g.M * g.N;
-  LineTw<L> lt;
+  const cw* __restrict__ ysig = static_cast<const cw*>(a.ysig);
+  cw* __restrict__ yout = static_cast<cw*>(a.yout);
+  LineTw<L, W> lt;
   load_line_tw<L>(lt, rowtw, t);
 
   // issue every load this lane needs before the first barrier
-  cd v[8], psi[8], accin[8], ep[8], ec[8], e1[8];
+  cw v[8];
+  cd psi[8], accin[8], ep[8], ec[8], e1[8];
   const cd zero = {0., 0.};
 #pragma unroll
   for (int i = 0; i < 8; i++) {
     const size_t idx = base + i * C::T;
-    v[i] = (valid && a.has_prev) ? a.ysig[idx] : zero;
+    v[i] = (valid && a.has_prev) ? ysig[idx] : cw{0, 0};
     ep[i] = (valid && a.has_prev) ? a.e_prev[idx] : zero;
     e1[i] = (valid && a.update_prev) ? a.e1[idx] : zero;
     psi[i] = (valid && (a.prev_is_first || !a.final_stage)) ? a.psi_hat[idx] : zero;
@@ -95,7 +99,7 @@ __global__ void __launch_bounds__(64) k_row_head(Geometry g, const cd* __restric
     line_fft<L, -1, 1, 1>(v, sm, lt, t);
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-      const cd x = cmulc(v[i], ep[i]);             // sigma_hat * conj(E_{i-1})
+      const cd x = cmulc(widen(v[i]), ep[i]);      // sigma_hat * conj(E_{i-1})
       nh[i] = {-x.im * a.sgn, x.re * a.sgn};       // * (i sgn)
       cd acc = a.prev_is_first ? psi[i] : accin[i];
       if (a.bh_prev != 0.) acc = cadd(acc, cscale(nh[i], a.bh_prev));
@@ -126,12 +130,12 @@ __global__ void __launch_bounds__(64) k_row_head(Geometry g, const cd* __restric
       else nj = valid ? a.stash[(size_t)j * MN + idx] : zero;
       x = cadd(x, cscale(nj, a.a_h[e]));
     }
-    v[i] = cscale(cmul(ec[i], x), a.scale_cur);
+    v[i] = narrow<W>(cscale(cmul(ec[i], x), a.scale_cur));
   }
   line_fft<L, 1, 1, 1>(v, sm, lt, t);
   if (valid) {
 #pragma unroll
-    for (int i = 0; i < 8; i++) a.yout[base + i * C::T] = v[i];
+    for (int i = 0; i < 8; i++) yout[base + i * C::T] = v[i];
   }
 }
 
@@ -152,57 +156,61 @@ __device__ __forceinline__ ColIdx col_index(int ncb) {
 }
 
 // inverse column DFT: in = spectral-side rows (k1), out = time domain, times scale, divided by divisor
-template <int L, int CB>
-__global__ void __launch_bounds__(256) k_col_inverse(Geometry g, const cd* __restrict__ coltw,
-                                                     const cd* __restrict__ fourtw, const cd* in, cd* out, int lines,
-                                                     double scale, double divisor) {
+template <int L, int CB, class W>
+__global__ void __launch_bounds__(256) k_col_inverse(Geometry g, const cpx<W>* __restrict__ coltw,
+                                                     const cpx<W>* __restrict__ fourtw, const cpx<W>* in,
+                                                     cpx<W>* out, int lines, double scale, double divisor) {
   using C = ColCfg<L, CB>;
-  __shared__ cd smem[C::G * C::SLOTS * CB];
+  using cw = cpx<W>;
+  __shared__ cw smem[C::G * C::SLOTS * CB];
   const ColIdx ix = col_index<L, CB>(g.N2 / CB);
   const int q = ix.lg * C::G + ix.gl;
   const int n2 = ix.cb * CB + ix.c;
   const bool valid = q < lines;
   const size_t base = (size_t)(valid ? q : 0) * g.N + n2;
-  cd* sm = smem + ix.gl * C::SLOTS * CB + ix.c;
-  LineTw<L> lt;
+  cw* sm = smem + ix.gl * C::SLOTS * CB + ix.c;
+  LineTw<L, W> lt;
   load_line_tw<L>(lt, coltw, ix.t);
-  cd v[8];
+  cw v[8];
 #pragma unroll
   for (int i = 0; i < 8; i++) {
     const int k1 = ix.t + i * C::T;
-    v[i] = valid ? cmulc(in[base + (size_t)k1 * g.N2], fourtw[n2 * k1]) : cd{0., 0.};
+    v[i] = valid ? cmulc(in[base + (size_t)k1 * g.N2], fourtw[n2 * k1]) : cw{0, 0};
   }
   line_fft<L, 1, CB, C::PAD>(v, sm, lt, ix.t);
   if (valid) {
+    const W sc = (W)scale, dv = (W)divisor;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
       const int n1 = ix.t + i * C::T;
-      cd x = cscale(v[i], scale);
-      if (divisor != 1.0) x = {x.re / divisor, x.im / divisor};
+      cw x = cscale(v[i], sc);
+      if (divisor != 1.0) x = {x.re / dv, x.im / dv};
       out[base + (size_t)n1 * g.N2] = x;
     }
   }
 }
 
 // forward column DFT + four-step twiddle: in = time domain, out = spectral-side rows
-template <int L, int CB>
-__global__ void __launch_bounds__(256) k_col_forward(Geometry g, const cd* __restrict__ coltw,
-                                                     const cd* __restrict__ fourtw, const cd* in, cd* out, int lines) {
+template <int L, int CB, class W>
+__global__ void __launch_bounds__(256) k_col_forward(Geometry g, const cpx<W>* __restrict__ coltw,
+                                                     const cpx<W>* __restrict__ fourtw, const cpx<W>* in,
+                                                     cpx<W>* out, int lines) {
   using C = ColCfg<L, CB>;
-  __shared__ cd smem[C::G * C::SLOTS * CB];
+  using cw = cpx<W>;
+  __shared__ cw smem[C::G * C::SLOTS * CB];
   const ColIdx ix = col_index<L, CB>(g.N2 / CB);
   const int q = ix.lg * C::G + ix.gl;
   const int n2 = ix.cb * CB + ix.c;
   const bool valid = q < lines;
   const size_t base = (size_t)(valid ? q : 0) * g.N + n2;
-  cd* sm = smem + ix.gl * C::SLOTS * CB + ix.c;
-  LineTw<L> lt;
+  cw* sm = smem + ix.gl * C::SLOTS * CB + ix.c;
+  LineTw<L, W> lt;
   load_line_tw<L>(lt, coltw, ix.t);
-  cd v[8], w4[8];
+  cw v[8], w4[8];
 #pragma unroll
   for (int i = 0; i < 8; i++) {
     const int n1 = ix.t + i * C::T;
-    v[i] = valid ? in[base + (size_t)n1 * g.N2] : cd{0., 0.};
+    v[i] = valid ? in[base + (size_t)n1 * g.N2] : cw{0, 0};
     w4[i] = fourtw[n2 * n1];  // index k1 = n1 after the transform
   }
   line_fft<L, -1, CB, C::PAD>(v, sm, lt, ix.t);
@@ -215,8 +223,8 @@ __global__ void __launch_bounds__(256) k_col_forward(Geometry g, const cd* __res
   }
 }
 
-// forward column DFT of one real line (the Raman response)
-template <int L, int CB>
+// forward column DFT of one real line (the Raman response), FP64 only
+template <int L, int CB, class W>
 __global__ void __launch_bounds__(256) k_col_forward_real(Geometry g, const cd* __restrict__ coltw,
                                                           const cd* __restrict__ fourtw, const double* __restrict__ in,
                                                           cd* __restrict__ out) {
@@ -245,33 +253,35 @@ __global__ void __launch_bounds__(256) k_col_forward_real(Geometry g, const cd* 
 }
 
 // pair products of the time-domain fields U, then forward column DFT
-template <int L, int CB>
-__global__ void __launch_bounds__(256) k_col_pairs_forward(Geometry g, const cd* __restrict__ coltw,
-                                                           const cd* __restrict__ fourtw, const cd* __restrict__ U,
-                                                           cd* __restrict__ out, int lines, PairLineArgs pl) {
+template <int L, int CB, class W>
+__global__ void __launch_bounds__(256) k_col_pairs_forward(Geometry g, const cpx<W>* __restrict__ coltw,
+                                                           const cpx<W>* __restrict__ fourtw,
+                                                           const cpx<W>* __restrict__ U, cpx<W>* __restrict__ out,
+                                                           int lines, PairLineArgs pl) {
   using C = ColCfg<L, CB>;
-  __shared__ cd smem[C::G * C::SLOTS * CB];
+  using cw = cpx<W>;
+  __shared__ cw smem[C::G * C::SLOTS * CB];
   const ColIdx ix = col_index<L, CB>(g.N2 / CB);
   const int q = ix.lg * C::G + ix.gl;
   const int n2 = ix.cb * CB + ix.c;
   const bool valid = q < lines;
   const int qq = valid ? q : 0;
   const uint32_t la = pl.line_a[qq], lb = pl.line_b[qq], self = pl.line_self[qq];
-  cd* sm = smem + ix.gl * C::SLOTS * CB + ix.c;
-  LineTw<L> lt;
+  cw* sm = smem + ix.gl * C::SLOTS * CB + ix.c;
+  LineTw<L, W> lt;
   load_line_tw<L>(lt, coltw, ix.t);
-  cd v[8], w4[8];
+  cw v[8], w4[8];
 #pragma unroll
   for (int i = 0; i < 8; i++) {
     const int n1 = ix.t + i * C::T;
     const size_t tt = (size_t)n1 * g.N2 + n2;
-    cd x = {0., 0.};
+    cw x = {0, 0};
     if (valid) {
-      const cd ua = U[(size_t)la * g.N + tt];
+      const cw ua = U[(size_t)la * g.N + tt];
       if (self) {
         x.re = ua.re * ua.re + ua.im * ua.im;
         if (lb != 0xffffffffu) {
-          const cd ub = U[(size_t)lb * g.N + tt];
+          const cw ub = U[(size_t)lb * g.N + tt];
           x.im = ub.re * ub.re + ub.im * ub.im;
         }
       } else {
@@ -350,29 +360,32 @@ __global__ void __launch_bounds__(64) k_row_inverse_phase(Geometry g, const cd* 
 }
 
 // Raman convolution in the spectral rows: DFT, * H * hscale, inverse DFT (in place)
-template <int L>
-__global__ void __launch_bounds__(64) k_row_raman(Geometry g, const cd* __restrict__ rowtw, cd* __restrict__ data,
-                                                  int lines, const cd* __restrict__ hspec, double hscale) {
+template <int L, class W>
+__global__ void __launch_bounds__(64) k_row_raman(Geometry g, const cpx<W>* __restrict__ rowtw,
+                                                  cpx<W>* __restrict__ data, int lines,
+                                                  const cpx<W>* __restrict__ hspec, double hscale) {
   using C = RowCfg<L>;
-  __shared__ cd smem[C::G * C::SLOTS];
+  using cw = cpx<W>;
+  __shared__ cw smem[C::G * C::SLOTS];
   const int tid = threadIdx.x;
   const int gl = tid / C::T, t = tid % C::T;
   const int row = blockIdx.x * C::G + gl;
   const bool valid = row < lines * g.N1;
   const size_t base = valid ? (size_t)row * g.N2 : 0;
   const size_t hbase = valid ? (size_t)(row % g.N1) * g.N2 : 0;
-  cd* sm = smem + gl * C::SLOTS;
-  LineTw<L> lt;
+  cw* sm = smem + gl * C::SLOTS;
+  LineTw<L, W> lt;
   load_line_tw<L>(lt, rowtw, t);
-  cd v[8], hs[8];
+  cw v[8], hs[8];
 #pragma unroll
   for (int i = 0; i < 8; i++) {
-    v[i] = valid ? data[base + t + i * C::T] : cd{0., 0.};
+    v[i] = valid ? data[base + t + i * C::T] : cw{0, 0};
     hs[i] = hspec[hbase + t + i * C::T];
   }
   line_fft<L, -1, 1, 1>(v, sm, lt, t);
+  const W hsc = (W)hscale;
 #pragma unroll
-  for (int i = 0; i < 8; i++) v[i] = cscale(cmul(v[i], hs[i]), hscale);
+  for (int i = 0; i < 8; i++) v[i] = cscale(cmul(v[i], hs[i]), hsc);
   line_fft<L, 1, 1, 1>(v, sm, lt, t);
   if (valid) {
 #pragma unroll
@@ -393,24 +406,38 @@ __global__ void __launch_bounds__(64) k_row_raman(Geometry g, const cd* __restri
 // `stride` values. Lane l loads nonzero eb + l; v_readlane broadcasts it, eight at a
 // time, so the sixteen LDS reads of a batch are in flight together. The products are
 // added in nonzero order.
-__device__ __forceinline__ cd gather_sum(cd sum, const uint32_t* __restrict__ kslot, const double* __restrict__ q,
-                                         uint32_t e0, uint32_t e1, int s0, const cd* Xs, const cd* Us, int stride,
-                                         int lane) {
+// Broadcast lane j's coefficient (kept as its bit pattern in one or two ints) to the wave.
+struct QBits {
+  int lo, hi;
+};
+__device__ __forceinline__ QBits q_bits(double v) { return {__double2loint(v), __double2hiint(v)}; }
+__device__ __forceinline__ QBits q_bits(float v) { return {__float_as_int(v), 0}; }
+template <class W>
+__device__ __forceinline__ W q_lane(QBits b, int j) {
+  if constexpr (sizeof(W) == 8)
+    return __hiloint2double(__builtin_amdgcn_readlane(b.hi, j), __builtin_amdgcn_readlane(b.lo, j));
+  else
+    return __int_as_float(__builtin_amdgcn_readlane(b.lo, j));
+}
+
+template <class W>
+__device__ __forceinline__ cpx<W> gather_sum(cpx<W> sum, const uint32_t* __restrict__ kslot, const W* __restrict__ q,
+                                             uint32_t e0, uint32_t e1, int s0, const cpx<W>* Xs, const cpx<W>* Us,
+                                             int stride, int lane) {
   constexpr int B = 8;
   for (uint32_t eb = e0; eb < e1; eb += 64) {
     const uint32_t mine = eb + lane;
     const uint32_t ks = mine < e1 ? kslot[mine] : 0u;
-    const double qv = mine < e1 ? q[mine] : 0.;
-    const int qlo = __double2loint(qv), qhi = __double2hiint(qv);
+    const QBits qb = q_bits(mine < e1 ? q[mine] : (W)0);
     const int cnt = (int)min(64u, e1 - eb);
     int j = 0;
     for (; j + B <= cnt; j += B) {
-      cd xs[B], us[B];
-      double qs[B];
+      cpx<W> xs[B], us[B];
+      W qs[B];
 #pragma unroll
       for (int u = 0; u < B; u++) {
         const uint32_t ksj = (uint32_t)__builtin_amdgcn_readlane((int)ks, j + u);
-        qs[u] = __hiloint2double(__builtin_amdgcn_readlane(qhi, j + u), __builtin_amdgcn_readlane(qlo, j + u));
+        qs[u] = q_lane<W>(qb, j + u);
         xs[u] = Xs[((int)(ksj >> 8) - s0) * stride];
         us[u] = Us[(int)(ksj & 255u) * stride];
       }
@@ -419,7 +446,7 @@ __device__ __forceinline__ cd gather_sum(cd sum, const uint32_t* __restrict__ ks
     }
     for (; j < cnt; j++) {
       const uint32_t ksj = (uint32_t)__builtin_amdgcn_readlane((int)ks, j);
-      const double qj = __hiloint2double(__builtin_amdgcn_readlane(qhi, j), __builtin_amdgcn_readlane(qlo, j));
+      const W qj = q_lane<W>(qb, j);
       sum = cadd(sum, cscale(cmul(Xs[((int)(ksj >> 8) - s0) * stride], Us[(int)(ksj & 255u) * stride]), qj));
     }
   }
@@ -429,16 +456,20 @@ __device__ __forceinline__ cd gather_sum(cd sum, const uint32_t* __restrict__ ks
 // Slot mixes of a 64-sample tile into LDS: X[s - s0][j] = kv U_a conj(U_b) + kr VR_s[t0 + j]
 // for slots [s0, s0 + ns). Every Raman value a lane needs is loaded before any is used, so
 // the loads overlap instead of each waiting for the previous one.
-__device__ __forceinline__ void slot_mix_tile(const GatherArgs& a, const Geometry& g, const cd* __restrict__ VR,
-                                              const cd* Ush, cd* Xsh, int s0, int ns, int t0, int tid, int NTHR) {
+template <class W>
+__device__ __forceinline__ void slot_mix_tile(const GatherArgs& a, const Geometry& g, const cpx<W>* __restrict__ VR,
+                                              const cpx<W>* Ush, cpx<W>* Xsh, int s0, int ns, int t0, int tid,
+                                              int NTHR) {
+  using cw = cpx<W>;
   constexpr int TS = 64, R = 16;
   const int total = ns * TS;
+  const W kv = (W)a.kv, kr = (W)a.kr;
   for (int base = 0; base < total; base += R * NTHR) {
-    cd r[R];
+    cw r[R];
 #pragma unroll
     for (int i = 0; i < R; i++) {
       const int e = base + tid + i * NTHR;
-      r[i] = {0., 0.};
+      r[i] = {0, 0};
       // e / 64 is the same for the whole wavefront (e = base + 64 * wave + lane + i * NTHR)
       const int sl = __builtin_amdgcn_readfirstlane(s0 + e / TS);
       if (a.raman && e < total) r[i] = VR[(size_t)a.slot_line[sl] * g.N + t0 + e % TS];
@@ -448,17 +479,17 @@ __device__ __forceinline__ void slot_mix_tile(const GatherArgs& a, const Geometr
       const int e = base + tid + i * NTHR;
       if (e >= total) break;
       const int sl = __builtin_amdgcn_readfirstlane(s0 + e / TS), j = e % TS;
-      const cd V = cmulc(Ush[a.slot_a[sl] * TS + j], Ush[a.slot_b[sl] * TS + j]);
-      cd X = cscale(V, a.kv);
+      const cw V = cmulc(Ush[a.slot_a[sl] * TS + j], Ush[a.slot_b[sl] * TS + j]);
+      cw X = cscale(V, kv);
       if (a.raman) {
-        cd vr;
+        cw vr;
         switch (a.slot_kind[sl]) {
           case kDirect: vr = r[i]; break;
           case kConj: vr = cconj(r[i]); break;
-          case kRe: vr = {r[i].re, 0.}; break;
-          default: vr = {r[i].im, 0.}; break;
+          case kRe: vr = {r[i].re, 0}; break;
+          default: vr = {r[i].im, 0}; break;
         }
-        X = cadd(X, cscale(vr, a.kr));
+        X = cadd(X, cscale(vr, kr));
       }
       Xsh[e] = X;
     }
@@ -483,23 +514,26 @@ void gather_plan(int M, int P, int& cap, int& chunk) {
   }
 }
 
-template <int CAP>
-__global__ void __launch_bounds__(512) k_gather(Geometry g, const cd* __restrict__ U, const cd* __restrict__ VR,
-                                                cd* __restrict__ sigma, GatherArgs a) {
-  __shared__ cd sh[CAP];
+template <int CAP, class W>
+__global__ void __launch_bounds__(512) k_gather(Geometry g, const cpx<W>* __restrict__ U,
+                                                const cpx<W>* __restrict__ VR, cpx<W>* __restrict__ sigma,
+                                                GatherArgs a) {
+  using cw = cpx<W>;
+  __shared__ cw sh[CAP];
   constexpr int TS = kGatherTS;
   const int M = g.M;
-  cd* Ush = sh;            // [M][TS]
-  cd* Xsh = sh + M * TS;   // [chunk][TS]
+  const W* __restrict__ qv = static_cast<const W*>(a.q);
+  cw* Ush = sh;            // [M][TS]
+  cw* Xsh = sh + M * TS;   // [chunk][TS]
   const int t0 = blockIdx.x * TS;
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int lane = tid % TS, W = nthr / TS;
+  const int lane = tid % TS, nwaves = nthr / TS;
   const int wave = __builtin_amdgcn_readfirstlane(tid / TS);  // wave-uniform for the compiler
 #pragma unroll 4
   for (int e = tid; e < M * TS; e += nthr) Ush[e] = U[(size_t)(e / TS) * g.N + t0 + e % TS];
-  cd acc[kGatherMaxPW];
+  cw acc[kGatherMaxPW];
 #pragma unroll
-  for (int r = 0; r < kGatherMaxPW; r++) acc[r] = {0., 0.};
+  for (int r = 0; r < kGatherMaxPW; r++) acc[r] = {0, 0};
   for (int c = 0; c < a.n_chunks; c++) {
     const int s0 = c * a.chunk;
     const int ns = min(a.chunk, a.n_slots - s0);
@@ -508,15 +542,15 @@ __global__ void __launch_bounds__(512) k_gather(Geometry g, const cd* __restrict
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < kGatherMaxPW; r++) {
-      const int p = wave + r * W;
+      const int p = wave + r * nwaves;
       if (p >= M) break;
       const uint32_t* co = a.coff + (size_t)p * (a.n_chunks + 1) + c;
-      acc[r] = gather_sum(acc[r], a.kslot, a.q, co[0], co[1], s0, Xsh + lane, Ush + lane, TS, lane);
+      acc[r] = gather_sum(acc[r], a.kslot, qv, co[0], co[1], s0, Xsh + lane, Ush + lane, TS, lane);
     }
   }
 #pragma unroll
   for (int r = 0; r < kGatherMaxPW; r++) {
-    const int p = wave + r * W;
+    const int p = wave + r * nwaves;
     if (p >= M) break;
     sigma[(size_t)p * g.N + t0 + lane] = acc[r];
   }
@@ -686,7 +720,8 @@ __global__ void __launch_bounds__(256) k_col_raman_gather(Geometry g, const cd* 
       const int p = pg + r * F::PG;
       if (p >= M) break;
       const uint32_t* co = fa.coff + (size_t)p * (fa.n_chunks + 1) + c;
-      acc[r] = gather_sum(acc[r], a.kslot, a.q, co[0], co[1], s0, Xsh + n1g, Ush + n1g, L, lane);
+      acc[r] = gather_sum(acc[r], a.kslot, static_cast<const double*>(a.q), co[0], co[1], s0, Xsh + n1g, Ush + n1g,
+                          L, lane);
     }
   }
   __syncthreads();  // every lane done reading Ush and Xsh
@@ -727,25 +762,26 @@ int dense_gather_chunk(int M) {
   return room >= 4 * M ? room : 0;
 }
 
-template <int M>
-__global__ void __launch_bounds__(256) k_gather_dense(Geometry g, const cd* __restrict__ U, const cd* __restrict__ VR,
-                                                      cd* __restrict__ sigma, GatherArgs a, const double* __restrict__ Qd,
-                                                      int chunk) {
-  __shared__ cd sh[kDenseCap];
+template <int M, class W>
+__global__ void __launch_bounds__(256) k_gather_dense(Geometry g, const cpx<W>* __restrict__ U,
+                                                      const cpx<W>* __restrict__ VR, cpx<W>* __restrict__ sigma,
+                                                      GatherArgs a, const W* __restrict__ Qd, int chunk) {
+  using cw = cpx<W>;
+  __shared__ cw sh[kDenseCap];
   constexpr int TS = 64;
-  cd* Ush = sh;           // [M][TS]
-  cd* Xsh = sh + M * TS;  // [chunk][TS]; reused for the partial sums
+  cw* Ush = sh;           // [M][TS]
+  cw* Xsh = sh + M * TS;  // [chunk][TS]; reused for the partial sums
   const int t0 = blockIdx.x * TS;
   const int tid = threadIdx.x, lane = tid % TS;
   const int wave = __builtin_amdgcn_readfirstlane(tid / TS);
 #pragma unroll 4
   for (int e = tid; e < M * TS; e += 256) Ush[e] = U[(size_t)(e / TS) * g.N + t0 + e % TS];
   __syncthreads();
-  cd u[M], acc[M];
+  cw u[M], acc[M];
 #pragma unroll
   for (int k = 0; k < M; k++) {
     u[k] = Ush[k * TS + lane];
-    acc[k] = {0., 0.};
+    acc[k] = {0, 0};
   }
   for (int s0 = 0; s0 < a.n_slots; s0 += chunk) {
     const int ns = min(chunk, a.n_slots - s0);
@@ -753,14 +789,14 @@ __global__ void __launch_bounds__(256) k_gather_dense(Geometry g, const cd* __re
     slot_mix_tile(a, g, VR, Ush, Xsh, s0, ns, t0, tid, 256);
     __syncthreads();
     for (int sl = s0 + wave; sl < s0 + ns; sl += 4) {
-      const cd X = Xsh[(sl - s0) * TS + lane];
-      const double* qb = Qd + (size_t)sl * M * M;
-      cd y[M];
+      const cw X = Xsh[(sl - s0) * TS + lane];
+      const W* qb = Qd + (size_t)sl * M * M;
+      cw y[M];
 #pragma unroll
       for (int k = 0; k < M; k++) y[k] = cmul(X, u[k]);
 #pragma unroll
       for (int p = 0; p < M; p++) {
-        cd t = acc[p];
+        cw t = acc[p];
 #pragma unroll
         for (int k = 0; k < M; k++) t = cadd(t, cscale(y[k], qb[p * M + k]));
         acc[p] = t;
@@ -773,7 +809,7 @@ __global__ void __launch_bounds__(256) k_gather_dense(Geometry g, const cd* __re
   __syncthreads();
   for (int e = tid; e < M * TS; e += 256) {
     const int p = e / TS, j = e % TS;
-    cd t = Xsh[p * TS + j];
+    cw t = Xsh[p * TS + j];
 #pragma unroll
     for (int w = 1; w < 4; w++) t = cadd(t, Xsh[(w * M + p) * TS + j]);
     sigma[(size_t)p * g.N + t0 + j] = t;
@@ -820,16 +856,28 @@ __global__ void __launch_bounds__(256) k_energy(Geometry g, const cd* __restrict
     default: return hipErrorInvalidValue;                  \
   }
 
-template <int L>
+#define MMP_DISPATCH_LW(Lv, FN, W, ...)                    \
+  switch (Lv) {                                            \
+    case 8: return FN<8, W>(__VA_ARGS__);                  \
+    case 16: return FN<16, W>(__VA_ARGS__);                \
+    case 32: return FN<32, W>(__VA_ARGS__);                \
+    case 64: return FN<64, W>(__VA_ARGS__);                \
+    case 128: return FN<128, W>(__VA_ARGS__);              \
+    case 256: return FN<256, W>(__VA_ARGS__);              \
+    default: return hipErrorInvalidValue;                  \
+  }
+
+template <int L, class W>
 static hipError_t row_head_impl(const Geometry& g, const Twiddles& tw, const HeadArgs& a, hipStream_t s) {
   using C = RowCfg<L>;
   const int rows = g.M * g.N1;
   dim3 grid((rows + C::G - 1) / C::G);
-  hipLaunchKernelGGL(k_row_head<L>, grid, dim3(C::THREADS), 0, s, g, tw.rowtw, a);
+  hipLaunchKernelGGL((k_row_head<L, W>), grid, dim3(C::THREADS), 0, s, g, TwOf<W>::row(tw), a);
   return hipGetLastError();
 }
+template <class W>
 hipError_t launch_row_head(const Geometry& g, const Twiddles& tw, const HeadArgs& a, hipStream_t s) {
-  MMP_DISPATCH_L(g.N2, row_head_impl, g, tw, a, s);
+  MMP_DISPATCH_LW(g.N2, row_head_impl, W, g, tw, a, s);
 }
 
 // Column-block width: 8 when that already gives >= 256 workgroups, else 4, else 2.
@@ -853,53 +901,56 @@ static dim3 col_grid(const Geometry& g, int lines) {
 
 #define MMP_COL_LAUNCH(KERNEL, LINES, ...)                                                                      \
   switch (pick_cb<L>(g, LINES)) {                                                                               \
-    case 8: hipLaunchKernelGGL((KERNEL<L, 8>), (col_grid<L, 8>(g, LINES)), (dim3(ColCfg<L, 8>::THREADS)), 0, s,    \
+    case 8: hipLaunchKernelGGL((KERNEL<L, 8, W>), (col_grid<L, 8>(g, LINES)), (dim3(ColCfg<L, 8>::THREADS)), 0, s, \
                                __VA_ARGS__); break;                                                            \
-    case 4: hipLaunchKernelGGL((KERNEL<L, 4>), (col_grid<L, 4>(g, LINES)), (dim3(ColCfg<L, 4>::THREADS)), 0, s,    \
+    case 4: hipLaunchKernelGGL((KERNEL<L, 4, W>), (col_grid<L, 4>(g, LINES)), (dim3(ColCfg<L, 4>::THREADS)), 0, s, \
                                __VA_ARGS__); break;                                                            \
-    default: hipLaunchKernelGGL((KERNEL<L, 2>), (col_grid<L, 2>(g, LINES)), (dim3(ColCfg<L, 2>::THREADS)), 0, s,   \
-                                __VA_ARGS__); break;                                                           \
+    default: hipLaunchKernelGGL((KERNEL<L, 2, W>), (col_grid<L, 2>(g, LINES)), (dim3(ColCfg<L, 2>::THREADS)), 0,  \
+                                s, __VA_ARGS__); break;                                                        \
   }                                                                                                             \
   return hipGetLastError();
 
-template <int L>
-static hipError_t col_inverse_impl(const Geometry& g, const Twiddles& tw, const cd* in, cd* out, int lines,
+template <int L, class W>
+static hipError_t col_inverse_impl(const Geometry& g, const Twiddles& tw, const cpx<W>* in, cpx<W>* out, int lines,
                                    double scale, double divisor, hipStream_t s) {
-  MMP_COL_LAUNCH(k_col_inverse, lines, g, tw.coltw, tw.fourtw, in, out, lines, scale, divisor)
+  MMP_COL_LAUNCH(k_col_inverse, lines, g, TwOf<W>::col(tw), TwOf<W>::four(tw), in, out, lines, scale, divisor)
 }
-hipError_t launch_col_inverse(const Geometry& g, const Twiddles& tw, const cd* in, cd* out, int lines, double scale,
-                              double divisor, hipStream_t s) {
-  MMP_DISPATCH_L(g.N1, col_inverse_impl, g, tw, in, out, lines, scale, divisor, s);
+template <class W>
+hipError_t launch_col_inverse(const Geometry& g, const Twiddles& tw, const cpx<W>* in, cpx<W>* out, int lines,
+                              double scale, double divisor, hipStream_t s) {
+  MMP_DISPATCH_LW(g.N1, col_inverse_impl, W, g, tw, in, out, lines, scale, divisor, s);
 }
 
-template <int L>
-static hipError_t col_forward_impl(const Geometry& g, const Twiddles& tw, const cd* in, cd* out, int lines,
+template <int L, class W>
+static hipError_t col_forward_impl(const Geometry& g, const Twiddles& tw, const cpx<W>* in, cpx<W>* out, int lines,
                                    hipStream_t s) {
-  MMP_COL_LAUNCH(k_col_forward, lines, g, tw.coltw, tw.fourtw, in, out, lines)
+  MMP_COL_LAUNCH(k_col_forward, lines, g, TwOf<W>::col(tw), TwOf<W>::four(tw), in, out, lines)
 }
-hipError_t launch_col_forward(const Geometry& g, const Twiddles& tw, const cd* in, cd* out, int lines,
+template <class W>
+hipError_t launch_col_forward(const Geometry& g, const Twiddles& tw, const cpx<W>* in, cpx<W>* out, int lines,
                               hipStream_t s) {
-  MMP_DISPATCH_L(g.N1, col_forward_impl, g, tw, in, out, lines, s);
+  MMP_DISPATCH_LW(g.N1, col_forward_impl, W, g, tw, in, out, lines, s);
 }
 
-template <int L>
+template <int L, class W>
 static hipError_t col_forward_real_impl(const Geometry& g, const Twiddles& tw, const double* in, cd* out,
                                         hipStream_t s) {
   MMP_COL_LAUNCH(k_col_forward_real, 1, g, tw.coltw, tw.fourtw, in, out)
 }
 hipError_t launch_col_forward_real(const Geometry& g, const Twiddles& tw, const double* in, cd* out,
                                    hipStream_t s) {
-  MMP_DISPATCH_L(g.N1, col_forward_real_impl, g, tw, in, out, s);
+  MMP_DISPATCH_LW(g.N1, col_forward_real_impl, double, g, tw, in, out, s);
 }
 
-template <int L>
-static hipError_t col_pairs_forward_impl(const Geometry& g, const Twiddles& tw, const cd* U, cd* out, int lines,
-                                         const PairLineArgs& pl, hipStream_t s) {
-  MMP_COL_LAUNCH(k_col_pairs_forward, lines, g, tw.coltw, tw.fourtw, U, out, lines, pl)
+template <int L, class W>
+static hipError_t col_pairs_forward_impl(const Geometry& g, const Twiddles& tw, const cpx<W>* U, cpx<W>* out,
+                                         int lines, const PairLineArgs& pl, hipStream_t s) {
+  MMP_COL_LAUNCH(k_col_pairs_forward, lines, g, TwOf<W>::col(tw), TwOf<W>::four(tw), U, out, lines, pl)
 }
-hipError_t launch_col_pairs_forward(const Geometry& g, const Twiddles& tw, const cd* U, cd* out, int lines,
+template <class W>
+hipError_t launch_col_pairs_forward(const Geometry& g, const Twiddles& tw, const cpx<W>* U, cpx<W>* out, int lines,
                                     const PairLineArgs& pl, hipStream_t s) {
-  MMP_DISPATCH_L(g.N1, col_pairs_forward_impl, g, tw, U, out, lines, pl, s);
+  MMP_DISPATCH_LW(g.N1, col_pairs_forward_impl, W, g, tw, U, out, lines, pl, s);
 }
 
 template <int L>
@@ -928,28 +979,30 @@ hipError_t launch_row_inverse_phase(const Geometry& g, const Twiddles& tw, const
   MMP_DISPATCH_L(g.N2, row_inverse_phase_impl, g, tw, in, out, imL, st, use_phase, s);
 }
 
-template <int L>
-static hipError_t row_raman_impl(const Geometry& g, const Twiddles& tw, cd* data, int lines, const cd* hspec,
+template <int L, class W>
+static hipError_t row_raman_impl(const Geometry& g, const Twiddles& tw, cpx<W>* data, int lines, const cpx<W>* hspec,
                                  double hscale, hipStream_t s) {
   using C = RowCfg<L>;
   const int rows = lines * g.N1;
-  hipLaunchKernelGGL(k_row_raman<L>, dim3((rows + C::G - 1) / C::G), dim3(C::THREADS), 0, s, g, tw.rowtw, data,
-                     lines, hspec, hscale);
+  hipLaunchKernelGGL((k_row_raman<L, W>), dim3((rows + C::G - 1) / C::G), dim3(C::THREADS), 0, s, g,
+                     TwOf<W>::row(tw), data, lines, hspec, hscale);
   return hipGetLastError();
 }
-hipError_t launch_row_raman(const Geometry& g, const Twiddles& tw, cd* data, int lines, const cd* hspec,
+template <class W>
+hipError_t launch_row_raman(const Geometry& g, const Twiddles& tw, cpx<W>* data, int lines, const cpx<W>* hspec,
                             double hscale, hipStream_t s) {
-  MMP_DISPATCH_L(g.N2, row_raman_impl, g, tw, data, lines, hspec, hscale, s);
+  MMP_DISPATCH_LW(g.N2, row_raman_impl, W, g, tw, data, lines, hspec, hscale, s);
 }
 
-hipError_t launch_gather(const Geometry& g, const cd* U, const cd* VR, cd* sigma, const GatherArgs& a,
+template <class W>
+hipError_t launch_gather(const Geometry& g, const cpx<W>* U, const cpx<W>* VR, cpx<W>* sigma, const GatherArgs& a,
                          hipStream_t s) {
   if (g.N % kGatherTS) return hipErrorInvalidValue;
   int waves = g.M < 8 ? g.M : 8;
   if ((g.M + waves - 1) / waves > kGatherMaxPW) return hipErrorInvalidValue;
   dim3 grid(g.N / kGatherTS), block(kGatherTS * waves);
-  if (a.cap == 4096) hipLaunchKernelGGL(k_gather<4096>, grid, block, 0, s, g, U, VR, sigma, a);
-  else if (a.cap == 8192) hipLaunchKernelGGL(k_gather<8192>, grid, block, 0, s, g, U, VR, sigma, a);
+  if (a.cap == 4096) hipLaunchKernelGGL((k_gather<4096, W>), grid, block, 0, s, g, U, VR, sigma, a);
+  else if (a.cap == 8192) hipLaunchKernelGGL((k_gather<8192, W>), grid, block, 0, s, g, U, VR, sigma, a);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
@@ -993,13 +1046,14 @@ hipError_t launch_col_raman_gather(const Geometry& g, const Twiddles& tw, const 
   }
 }
 
-#define MMP_DENSE_CASE(MV)                                                                                  \
-  case MV:                                                                                                   \
-    hipLaunchKernelGGL(k_gather_dense<MV>, dim3(g.N / 64), dim3(256), 0, s, g, U, VR, sigma, a, Qd, chunk); \
+#define MMP_DENSE_CASE(MV)                                                                                     \
+  case MV:                                                                                                      \
+    hipLaunchKernelGGL((k_gather_dense<MV, W>), dim3(g.N / 64), dim3(256), 0, s, g, U, VR, sigma, a, Qd, chunk); \
     break;
 
-hipError_t launch_gather_dense(const Geometry& g, const cd* U, const cd* VR, cd* sigma, const GatherArgs& a,
-                               const double* Qd, hipStream_t s) {
+template <class W>
+hipError_t launch_gather_dense(const Geometry& g, const cpx<W>* U, const cpx<W>* VR, cpx<W>* sigma,
+                               const GatherArgs& a, const W* Qd, hipStream_t s) {
   const int chunk = dense_gather_chunk(g.M);
   if (!chunk || g.N % 64) return hipErrorInvalidValue;
   switch (g.M) {
@@ -1008,6 +1062,18 @@ hipError_t launch_gather_dense(const Geometry& g, const cd* U, const cd* VR, cd*
     MMP_DENSE_CASE(13) MMP_DENSE_CASE(14) MMP_DENSE_CASE(15) MMP_DENSE_CASE(16)
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) k_narrow(const cd* __restrict__ in, cf* __restrict__ out, size_t n) {
+  for (size_t j = (size_t)blockIdx.x * 256 + threadIdx.x; j < n; j += (size_t)gridDim.x * 256)
+    out[j] = narrow<float>(in[j]);
+}
+
+hipError_t launch_narrow(const cd* in, cf* out, size_t n, hipStream_t s) {
+  const size_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(k_narrow, dim3((unsigned)(blocks < 4096 ? (blocks ? blocks : 1) : 4096)), dim3(256), 0, s, in,
+                     out, n);
   return hipGetLastError();
 }
 
@@ -1020,5 +1086,22 @@ hipError_t launch_energy(const Geometry& g, const cd* fields, double* out, doubl
   hipLaunchKernelGGL(k_energy, dim3(g.M), dim3(256), 0, s, g, fields, out, dt);
   return hipGetLastError();
 }
+
+#define MMP_INSTANTIATE_W(W)                                                                                        \
+  template hipError_t launch_row_head<W>(const Geometry&, const Twiddles&, const HeadArgs&, hipStream_t);         \
+  template hipError_t launch_col_inverse<W>(const Geometry&, const Twiddles&, const cpx<W>*, cpx<W>*, int, double, \
+                                            double, hipStream_t);                                                 \
+  template hipError_t launch_col_forward<W>(const Geometry&, const Twiddles&, const cpx<W>*, cpx<W>*, int,         \
+                                            hipStream_t);                                                         \
+  template hipError_t launch_col_pairs_forward<W>(const Geometry&, const Twiddles&, const cpx<W>*, cpx<W>*, int,   \
+                                                  const PairLineArgs&, hipStream_t);                              \
+  template hipError_t launch_row_raman<W>(const Geometry&, const Twiddles&, cpx<W>*, int, const cpx<W>*, double,   \
+                                          hipStream_t);                                                           \
+  template hipError_t launch_gather<W>(const Geometry&, const cpx<W>*, const cpx<W>*, cpx<W>*, const GatherArgs&,  \
+                                       hipStream_t);                                                              \
+  template hipError_t launch_gather_dense<W>(const Geometry&, const cpx<W>*, const cpx<W>*, cpx<W>*,               \
+                                             const GatherArgs&, const W*, hipStream_t);
+MMP_INSTANTIATE_W(double)
+MMP_INSTANTIATE_W(float)
 
 }  // namespace mmp

@@ -138,7 +138,14 @@ mmp_status prepare_problem(const mmp_params* in, const mmp_options& opt, Problem
   }
   if (in->n_steps == 0) { err = "n_steps must be >= 1"; return MMP_ERR_INVALID_ARGUMENT; }
   if (in->raman && !in->raman_response) { err = "raman_response is required with Raman on"; return MMP_ERR_INVALID_ARGUMENT; }
-  if (opt.precision != MMP_PRECISION_FP64) { err = "unknown precision"; return MMP_ERR_UNSUPPORTED; }
+  if (opt.precision != MMP_PRECISION_FP64 && opt.precision != MMP_PRECISION_MIXED) {
+    err = "unknown precision";
+    return MMP_ERR_UNSUPPORTED;
+  }
+  if (opt.precision == MMP_PRECISION_MIXED && (opt.flags & MMP_FLAG_FUSED_COLUMNS)) {
+    err = "the fused column passes are FP64 only";
+    return MMP_ERR_UNSUPPORTED;
+  }
   if (opt.pair_decode != MMP_PAIRS_REFERENCE_CPU && opt.pair_decode != MMP_PAIRS_PHYSICAL) {
     err = "unknown pair_decode";
     return MMP_ERR_INVALID_ARGUMENT;

@@ -88,6 +88,15 @@ struct mmp_solver {
   FusedArgs fused{};
   bool use_fused = false;
   double* qdense = nullptr;  // [P][M][M] dense coupling blocks for the register gather (M <= 12)
+  // work arrays of the step in the work precision (the FP64 buffers above in FP64 mode)
+  bool mixed = false;
+  size_t wbytes = 16;        // bytes per complex work value
+  void* ybuf_w = nullptr;
+  void* U_w = nullptr;
+  void* conv_w = nullptr;
+  void* hspec_w = nullptr;
+  const void* q_w = nullptr;
+  const void* qdense_w = nullptr;
   double z_host = 0;
   uint32_t steps_taken = 0;
   bool graphs = true;
@@ -179,55 +188,82 @@ mmp_status run_kernel(mmp_solver* s, const char* name, double bytes, F&& fn) {
   return MMP_OK;
 }
 
-mmp_status enqueue_step(mmp_solver* s) {
+template <class W>
+mmp_status enqueue_step_w(mmp_solver* s) {
+  using cw = cpx<W>;
   const Problem& P = s->P;
   const Geometry& g = s->g;
   hipStream_t st = s->ctx->stream;
-  const double cb = 16.0;  // bytes per complex FP64
+  cw* ybuf = static_cast<cw*>(s->ybuf_w);
+  cw* U = static_cast<cw*>(s->U_w);
+  cw* conv = static_cast<cw*>(s->conv_w);
+  const cw* hspec = static_cast<const cw*>(s->hspec_w);
+  const W* qdense = static_cast<const W*>(s->qdense_w);
+  const double cb = sizeof(cw);  // bytes per complex work value
+  const double cs = 16.0;        // bytes per complex FP64 state value
   const double MN = (double)P.M * P.N;
   const double LN = (double)s->n_lines * P.N;
   const double invN = 1.0 / P.N;
   for (int i = 0; i <= P.rk.s; i++) {
     const HeadArgs& h = s->heads[i];
-    const double hb = cb * MN * ((h.has_prev ? 1 : 0) + 1 + (h.has_prev && !h.prev_is_first ? 1 : 0) +
-                                 (h.final_stage ? 0 : 1) + (h.final_stage ? 0 : h.n_a)) +
-                      8.0 * MN * ((h.has_prev ? 1 : 0) + (h.final_stage ? 0 : 1));
+    // algorithmic bytes: every array the head reads or writes, once
+    int state_arrays = 0, work_arrays = 0;
+    if (h.has_prev) {
+      work_arrays += 1;                                  // sigma
+      state_arrays += 2;                                 // E_{i-1}, acc or psi_hat written
+      if (!h.prev_is_first) state_arrays += 1;           // acc read
+      if (h.update_prev) state_arrays += 2;              // E1 read, E_{i-1} written
+      if (h.stash_prev) state_arrays += 1;               // N_{i-1} stashed
+    }
+    if (h.prev_is_first || !h.final_stage) state_arrays += 1;  // psi_hat read
+    if (!h.final_stage) {
+      state_arrays += 1;                                 // E_i
+      work_arrays += 1;                                  // row-inverse output
+      for (int e = 0; e < h.n_a; e++)
+        if (h.a_idx[e] != h.prev_stage) state_arrays += 1;  // stashed N_j
+    }
+    const double hb = MN * (cs * state_arrays + cb * work_arrays);
     MMP_TRY(run_kernel(s, i == P.rk.s ? "row_close_step" : "row_stage_head", hb,
-                       [&] { return launch_row_head(g, s->tw, h, st); }));
+                       [&] { return launch_row_head<W>(g, s->tw, h, st); }));
     if (i == P.rk.s) break;
-    if (s->use_fused) {
-      MMP_TRY(run_kernel(s, "col_fields_pairs", cb * (3 * MN + LN), [&] {
-        return launch_col_fields_pairs(g, s->tw, s->ybuf, s->U, s->conv, s->n_lines, s->pairs, invN, st);
-      }));
-      MMP_TRY(run_kernel(s, "row_raman", cb * (2 * LN + P.N),
-                         [&] { return launch_row_raman(g, s->tw, s->conv, s->n_lines, s->hspec, P.delta_t, st); }));
-      MMP_TRY(run_kernel(s, "col_raman_gather", cb * (2 * MN + LN), [&] {
-        return launch_col_raman_gather(g, s->tw, s->U, s->conv, s->ybuf, s->gather, s->fused, invN, st);
-      }));
-      continue;
+    if constexpr (sizeof(W) == 8) {
+      if (s->use_fused) {
+        MMP_TRY(run_kernel(s, "col_fields_pairs", cb * (3 * MN + LN), [&] {
+          return launch_col_fields_pairs(g, s->tw, s->ybuf, s->U, s->conv, s->n_lines, s->pairs, invN, st);
+        }));
+        MMP_TRY(run_kernel(s, "row_raman", cb * (2 * LN + P.N), [&] {
+          return launch_row_raman<double>(g, s->tw, s->conv, s->n_lines, s->hspec, P.delta_t, st);
+        }));
+        MMP_TRY(run_kernel(s, "col_raman_gather", cb * (2 * MN + LN), [&] {
+          return launch_col_raman_gather(g, s->tw, s->U, s->conv, s->ybuf, s->gather, s->fused, invN, st);
+        }));
+        continue;
+      }
     }
     MMP_TRY(run_kernel(s, "col_inverse_fields", 2 * cb * MN,
-                       [&] { return launch_col_inverse(g, s->tw, s->ybuf, s->U, P.M, invN, 1.0, st); }));
+                       [&] { return launch_col_inverse<W>(g, s->tw, ybuf, U, P.M, invN, 1.0, st); }));
     if (P.raman) {
       MMP_TRY(run_kernel(s, "col_pairs_forward", cb * (MN + LN), [&] {
-        return launch_col_pairs_forward(g, s->tw, s->U, s->conv, s->n_lines, s->pairs, st);
+        return launch_col_pairs_forward<W>(g, s->tw, U, conv, s->n_lines, s->pairs, st);
       }));
       MMP_TRY(run_kernel(s, "row_raman", cb * (2 * LN + P.N),
-                         [&] { return launch_row_raman(g, s->tw, s->conv, s->n_lines, s->hspec, P.delta_t, st); }));
+                         [&] { return launch_row_raman<W>(g, s->tw, conv, s->n_lines, hspec, P.delta_t, st); }));
       MMP_TRY(run_kernel(s, "col_inverse_raman", 2 * cb * LN, [&] {
-        return launch_col_inverse(g, s->tw, s->conv, s->conv, s->n_lines, invN, 1.0, st);
+        return launch_col_inverse<W>(g, s->tw, conv, conv, s->n_lines, invN, 1.0, st);
       }));
     }
     MMP_TRY(run_kernel(s, "gather_kerr_raman", cb * (2 * MN + LN), [&] {
-      return s->qdense ? launch_gather_dense(g, s->U, s->conv, s->ybuf, s->gather, s->qdense, st)
-                       : launch_gather(g, s->U, s->conv, s->ybuf, s->gather, st);
+      return qdense ? launch_gather_dense<W>(g, U, conv, ybuf, s->gather, qdense, st)
+                    : launch_gather<W>(g, U, conv, ybuf, s->gather, st);
     }));
     MMP_TRY(run_kernel(s, "col_forward_sigma", 2 * cb * MN,
-                       [&] { return launch_col_forward(g, s->tw, s->ybuf, s->ybuf, P.M, st); }));
+                       [&] { return launch_col_forward<W>(g, s->tw, ybuf, ybuf, P.M, st); }));
   }
   MMP_TRY(run_kernel(s, "advance_z", 0.0, [&] { return launch_advance_z(s->st, P.delta_z, st); }));
   return MMP_OK;
 }
+
+mmp_status enqueue_step(mmp_solver* s) { return s->mixed ? enqueue_step_w<float>(s) : enqueue_step_w<double>(s); }
 
 mmp_status build_graph(mmp_solver* s) {
   hipStream_t st = s->ctx->stream;
@@ -270,7 +306,7 @@ mmp_status reset_z(mmp_solver* s) {
 // psi_hat = F(psi) for a time-domain field already in s->U
 mmp_status forward_state_from_U(mmp_solver* s) {
   hipStream_t st = s->ctx->stream;
-  HIP_TRY(launch_col_forward(s->g, s->tw, s->U, s->psi_hat, s->P.M, st), "col_forward");
+  HIP_TRY(launch_col_forward<double>(s->g, s->tw, s->U, s->psi_hat, s->P.M, st), "col_forward");
   HIP_TRY(launch_row_forward(s->g, s->tw, s->psi_hat, s->P.M, st), "row_forward");
   return MMP_OK;
 }
@@ -280,7 +316,7 @@ mmp_status inverse_state_to_U(mmp_solver* s, bool with_phase) {
   hipStream_t st = s->ctx->stream;
   HIP_TRY(launch_row_inverse_phase(s->g, s->tw, s->psi_hat, s->ybuf, s->imL, s->st, with_phase ? 1 : 0, st),
           "row_inverse_phase");
-  HIP_TRY(launch_col_inverse(s->g, s->tw, s->ybuf, s->U, s->P.M, 1.0 / s->P.N,
+  HIP_TRY(launch_col_inverse<double>(s->g, s->tw, s->ybuf, s->U, s->P.M, 1.0 / s->P.N,
                              with_phase ? s->P.conversion_factor : 1.0, st),
           "col_inverse");
   return MMP_OK;
@@ -302,22 +338,28 @@ void release(mmp_solver* s) {
 }
 
 // Device bytes create_impl allocates for problem P (kept in step with its dalloc calls).
-size_t planned_device_bytes(const Problem& P) {
+size_t planned_device_bytes(const Problem& P, const mmp_options& opt) {
   auto c = [](size_t n) { return n ? n : (size_t)1; };
+  const bool mixed = opt.precision == MMP_PRECISION_MIXED;
   const size_t MN = (size_t)P.M * P.N, L = P.line_a.size();
   bool any_stash = false;
   for (int j = 0; j < P.rk.s; j++) any_stash = any_stash || P.stash_needed[j];
   size_t b = 16 * (4 * MN) + 8 * MN + 16 * (P.N2 + P.N1 + P.N) + sizeof(StepState) + 8 * c(P.M);
   b += 16 * MN * (P.c_unique.size() + 1);
   if (any_stash) b += 16 * (size_t)P.rk.s * MN;
-  if (P.raman) b += 16 * c(L) * P.N + 16 * (size_t)P.N + 8 * (size_t)P.N;
+  if (P.raman) b += (mixed ? 8 : 16) * c(L) * P.N + 16 * (size_t)P.N + 8 * (size_t)P.N;
   b += 4 * c(P.coff.size()) + 4 * c(P.nnz) + 8 * c(P.nnz) + 4 * 4 * c(P.P) + 4 * 3 * c(L);
-  if (P.fused_chunks > 0) b += 4 * (P.line_slot0.size() + P.fused_slot.size() + P.fused_coff.size());
-  if (dense_gather_chunk((int)P.M) > 0) b += 8 * (size_t)P.P * P.M * P.M;
+  if (mixed) {
+    b += 8 * (2 * MN) + 8 * (P.N2 + P.N1 + P.N) + 4 * c(P.nnz);
+    if (P.raman) b += 8 * (size_t)P.N;
+  }
+  if (P.fused_chunks > 0 && (opt.flags & MMP_FLAG_FUSED_COLUMNS))
+    b += 4 * (P.line_slot0.size() + P.fused_slot.size() + P.fused_coff.size());
+  if (dense_gather_chunk((int)P.M) > 0) b += (mixed ? 4 : 8) * (size_t)P.P * P.M * P.M;
   return b;
 }
 
-void fill_info(const Problem& P, mmp_info* o) {
+void fill_info(const Problem& P, const mmp_options& opt, mmp_info* o) {
   std::memset(o, 0, sizeof *o);
   o->n_modes = P.M;
   o->nt = P.N;
@@ -337,7 +379,7 @@ void fill_info(const Problem& P, mmp_info* o) {
   o->ld = P.ld;
   o->max_beta2 = P.max_beta2;
   o->max_q = P.max_q;
-  o->device_bytes = planned_device_bytes(P);
+  o->device_bytes = planned_device_bytes(P, opt);
 }
 
 mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options* options, mmp_solver** out) {
@@ -356,6 +398,8 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
     return fail(stt, err);
   }
   s->time_window = params->time_window;
+  s->mixed = opt.precision == MMP_PRECISION_MIXED;
+  s->wbytes = s->mixed ? sizeof(cf) : sizeof(cd);
   s->profile = (opt.flags & MMP_FLAG_PROFILE) != 0;
   s->graphs = !s->profile && (opt.flags & MMP_FLAG_NO_GRAPH) == 0;
   Problem& P = s->P;
@@ -389,9 +433,32 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
   for (int j = 0; j < P.rk.s; j++) any_stash = any_stash || P.stash_needed[j];
   if (any_stash) A(dalloc(s, &s->stash, (size_t)P.rk.s * MN));
   if (P.raman) {
-    A(dalloc(s, &s->conv, (size_t)(s->n_lines ? s->n_lines : 1) * P.N));
     A(dalloc(s, &s->hspec, P.N));
     A(dalloc(s, &s->hR, P.N));
+  }
+  cf *rowtw32 = nullptr, *coltw32 = nullptr, *fourtw32 = nullptr;
+  const size_t conv_n = (size_t)(s->n_lines ? s->n_lines : 1) * P.N;
+  if (s->mixed) {
+    cf *yw, *uw, *cw = nullptr, *hw = nullptr;
+    A(dalloc(s, &yw, MN));
+    A(dalloc(s, &uw, MN));
+    if (P.raman) {
+      A(dalloc(s, &cw, conv_n));
+      A(dalloc(s, &hw, P.N));
+    }
+    A(dalloc(s, &rowtw32, P.N2));
+    A(dalloc(s, &coltw32, P.N1));
+    A(dalloc(s, &fourtw32, P.N));
+    s->ybuf_w = yw;
+    s->U_w = uw;
+    s->conv_w = cw;
+    s->hspec_w = hw;
+  } else {
+    if (P.raman) A(dalloc(s, &s->conv, conv_n));
+    s->ybuf_w = s->ybuf;
+    s->U_w = s->U;
+    s->conv_w = s->conv;
+    s->hspec_w = s->hspec;
   }
   uint32_t *d_coff, *d_k, *d_sa, *d_sb, *d_sl, *d_sk, *d_la, *d_lb, *d_ls;
   double* d_q;
@@ -413,6 +480,13 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
   A(upload(s, rowtw, tr.data(), tr.size() * 8));
   A(upload(s, coltw, tc.data(), tc.size() * 8));
   A(upload(s, fourtw, t4.data(), t4.size() * 8));
+  // FP32 twiddles: the FP64 values rounded once
+  std::vector<float> tr32(tr.begin(), tr.end()), tc32(tc.begin(), tc.end()), t432(t4.begin(), t4.end());
+  if (s->mixed) {
+    A(upload(s, rowtw32, tr32.data(), tr32.size() * 4));
+    A(upload(s, coltw32, tc32.data(), tc32.size() * 4));
+    A(upload(s, fourtw32, t432.data(), t432.size() * 4));
+  }
   A(upload(s, s->imL, P.imL_fourstep.data(), MN * 8));
   A(upload(s, d_coff, P.coff.data(), P.coff.size() * 4));
   A(upload(s, s->e1, P.e1.data(), P.e1.size() * sizeof(double)));
@@ -432,19 +506,37 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
     A(upload(s, d_lb, P.line_b.data(), s->n_lines * 4));
     A(upload(s, d_ls, P.line_self.data(), s->n_lines * 4));
   }
+  std::vector<float> q32(P.q.begin(), P.q.end());
+  float* d_q32 = nullptr;
+  if (s->mixed) {
+    A(dalloc(s, &d_q32, nnz1));
+    if (P.nnz) A(upload(s, d_q32, q32.data(), P.nnz * 4));
+  }
   if (st != MMP_OK) return cleanup(st);
-  s->tw = Twiddles{rowtw, coltw, fourtw};
-  s->gather = GatherArgs{d_coff, d_k, d_q, d_sa, d_sb, d_sl, d_sk, (int)P.P, P.gather_chunk,
+  s->tw = Twiddles{rowtw, coltw, fourtw, rowtw32, coltw32, fourtw32};
+  s->q_w = s->mixed ? (const void*)d_q32 : (const void*)d_q;
+  s->gather = GatherArgs{d_coff, d_k, s->q_w, d_sa, d_sb, d_sl, d_sk, (int)P.P, P.gather_chunk,
                          P.gather_chunks, P.gather_cap, P.raman, P.kv, P.kr};
   s->pairs = PairLineArgs{d_la, d_lb, d_ls};
   std::vector<double> qd;
+  std::vector<float> qd32;
   if (dense_gather_chunk((int)P.M) > 0) {
     qd.assign((size_t)P.P * P.M * P.M, 0.);
     for (uint32_t p = 0; p < P.M; p++)
       for (uint32_t e = P.off[p]; e < P.off[p + 1]; e++) qd[((size_t)P.slot[e] * P.M + p) * P.M + P.k[e]] += P.q[e];
-    A(dalloc(s, &s->qdense, qd.size()));
-    if (st != MMP_OK) return cleanup(st);
-    A(upload(s, s->qdense, qd.data(), qd.size() * sizeof(double)));
+    if (s->mixed) {
+      float* d;
+      qd32.assign(qd.begin(), qd.end());
+      A(dalloc(s, &d, qd32.size()));
+      if (st != MMP_OK) return cleanup(st);
+      A(upload(s, d, qd32.data(), qd32.size() * sizeof(float)));
+      s->qdense_w = d;
+    } else {
+      A(dalloc(s, &s->qdense, qd.size()));
+      if (st != MMP_OK) return cleanup(st);
+      A(upload(s, s->qdense, qd.data(), qd.size() * sizeof(double)));
+      s->qdense_w = s->qdense;
+    }
   }
   if (P.fused_chunks > 0 && (opt.flags & MMP_FLAG_FUSED_COLUMNS)) {
     uint32_t *d_ls0, *d_cs, *d_fc;
@@ -462,11 +554,11 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
   const double h = P.delta_z;
   for (int i = 0; i <= P.rk.s; i++) {
     HeadArgs a{};
-    a.ysig = s->ybuf;
+    a.ysig = s->ybuf_w;
     a.psi_hat = s->psi_hat;
     a.acc = s->acc;
     a.stash = s->stash;
-    a.yout = s->ybuf;
+    a.yout = s->ybuf_w;
     a.e1 = s->e1;
     a.sgn = P.sign_gamma;
     a.scale_cur = 1.0;
@@ -501,6 +593,10 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
     if (e != hipSuccess) A(hip_fail(e, "col_forward_real"));
     e = launch_row_forward(s->g, s->tw, s->hspec, 1, ctx->stream);
     if (e != hipSuccess) A(hip_fail(e, "row_forward"));
+    if (s->mixed) {
+      e = launch_narrow(s->hspec, static_cast<cf*>(s->hspec_w), P.N, ctx->stream);
+      if (e != hipSuccess) A(hip_fail(e, "narrow"));
+    }
   }
   A(reset_z(s));
   if (st == MMP_OK) {
@@ -683,7 +779,7 @@ mmp_status mmp_energy(mmp_solver* s, double* energy_out) {
 
 mmp_status mmp_solver_info(mmp_solver* s, mmp_info* o) {
   if (!s || !o) return fail(MMP_ERR_INVALID_ARGUMENT, "solver and info_out are required");
-  fill_info(s->P, o);
+  fill_info(s->P, s->opt, o);
   o->steps_taken = s->steps_taken;
   o->z = s->z_host;
   o->device_bytes = s->device_bytes;
@@ -711,7 +807,7 @@ mmp_status mmp_plan(const mmp_params* params, const mmp_options* options, mmp_in
     std::string err;
     const mmp_status st = prepare_problem(params, opt, P, err);
     if (st != MMP_OK) return fail(st, err);
-    fill_info(P, o);
+    fill_info(P, opt, o);
     return MMP_OK;
   } catch (const std::bad_alloc&) {
     return fail(MMP_ERR_OUT_OF_MEMORY, "host allocation failed");

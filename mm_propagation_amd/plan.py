@@ -14,16 +14,17 @@ _DP = ctypes.POINTER(ctypes.c_double)
 _UP = ctypes.POINTER(ctypes.c_uint32)
 
 
-def _opts(pair_decode: int, precision: int = _capi.MMP_PRECISION_FP64):
-    return _capi.MmpOptions(precision=precision, pair_decode=pair_decode, flags=0, reserved=0)
+def _opts(pair_decode: int, precision: int = _capi.MMP_PRECISION_FP64, flags: int = 0):
+    return _capi.MmpOptions(precision=precision, pair_decode=pair_decode, flags=flags, reserved=0)
 
 
-def plan(sp: SimulationParameters, raman: bool = True, pair_decode: int = 0) -> Dict[str, float]:
+def plan(sp: SimulationParameters, raman: bool = True, pair_decode: int = 0,
+         precision: int = _capi.MMP_PRECISION_FP64, flags: int = 0) -> Dict[str, float]:
     """Scalars and device bytes a solver for `sp` would have; raises MmpError on invalid input."""
     lib = _capi.load_library()
     cp = CParams(sp, raman)
     info = _capi.MmpInfo()
-    o = _opts(pair_decode)
+    o = _opts(pair_decode, precision, flags)
     _capi.check(lib.mmp_plan(ctypes.byref(cp.c), ctypes.byref(o), ctypes.byref(info)), "mmp_plan")
     return {name: getattr(info, name) for name, _ in _capi.MmpInfo._fields_}
 

@@ -58,12 +58,12 @@ class MultimodePropagationGPU:
     raman = True
 
     def __init__(self, params: SimulationParameters, pair_decode: int = _capi.MMP_PAIRS_REFERENCE_CPU,
-                 flags: int = 0, ctx: Optional[Context] = None):
+                 flags: int = 0, ctx: Optional[Context] = None, precision: int = _capi.MMP_PRECISION_FP64):
         self.ctx = ctx or default_context()
         self.lib = self.ctx.lib
         self.params = params
         self._cp = CParams(params, self.raman)
-        opt = _capi.MmpOptions(precision=_capi.MMP_PRECISION_FP64, pair_decode=pair_decode, flags=flags, reserved=0)
+        opt = _capi.MmpOptions(precision=precision, pair_decode=pair_decode, flags=flags, reserved=0)
         h = ctypes.c_void_p()
         _capi.check(self.lib.mmp_solver_create(self.ctx.h, ctypes.byref(self._cp.c), ctypes.byref(opt),
                                                ctypes.byref(h)), "mmp_solver_create")

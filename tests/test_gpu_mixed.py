@@ -1,0 +1,89 @@
+"""Mixed precision (MMP_PRECISION_MIXED): FFTs, pair products, Raman convolutions and the
+Kerr sum in FP32; spectral state, Lawson phases, stage combination and energy in FP64.
+Compared with the FP64 oracle on the same inputs. Tolerances (rel-L2 per mode):
+  MIXED_SHORT = 2e-6 after 50 steps (FP32 rounding of the nonlinear term only: the state
+  and the dispersion phases stay FP64, so the error grows with the number of steps, not
+  with the accumulated phase)."""
+import numpy as np
+import pytest
+
+import oracle_binding as ob
+from gpu_util import note, require_device
+from mm_propagation_amd import _capi, configs
+from mm_propagation_amd.solver import MultimodePropagationGPU_RamanOFF, MultimodePropagationGPU_RamanON
+
+pytestmark = pytest.mark.gpu
+
+MIXED_SHORT = 2e-6
+MIXED = _capi.MMP_PRECISION_MIXED
+
+
+def run_mixed(sp, steps, raman=True, decode=0, flags=0):
+    cls = MultimodePropagationGPU_RamanON if raman else MultimodePropagationGPU_RamanOFF
+    note(f"case mixed M={sp.n_modes} NT={sp.nt} steps={steps} raman={raman} decode={decode}")
+    s = cls(sp, pair_decode=decode, precision=MIXED, flags=flags)
+    s.computeLawsonRK(steps)
+    got = s.getResult()
+    energy = s.energy()
+    s.close()
+    o = ob.Oracle(sp, raman=raman, pair_decode=decode)
+    o.compute_lawson_rk(steps)
+    ref = o.get_result()
+    o.close()
+    return configs.rel_l2_per_mode(got, ref), got, energy
+
+
+@pytest.mark.parametrize("raman", [True, False])
+def test_two_mode_mixed_short(raman):
+    require_device()
+    err, _, _ = run_mixed(configs.two_mode_gaussian(nt=4096), 50, raman=raman)
+    print("rel-L2", err)
+    assert np.all(err <= MIXED_SHORT), err
+
+
+@pytest.mark.parametrize("M,nt", [(3, 4096), (6, 16384), (10, 8192)])
+def test_grin_mixed_short(M, nt):
+    require_device()
+    sp = configs.grin_fiber(M=M, nt=nt, symmetric=True)
+    err, _, _ = run_mixed(sp, 50, decode=1)
+    print("rel-L2", err)
+    assert np.all(err <= MIXED_SHORT), err
+
+
+@pytest.mark.parametrize("order", [6, 8])
+def test_higher_order_tableaux_mixed(order):
+    require_device()
+    err, _, _ = run_mixed(configs.two_mode_gaussian(nt=1024, method_order=order), 20)
+    print("rel-L2", err)
+    assert np.all(err <= MIXED_SHORT), err
+
+
+def test_mixed_sparse_gather_path():
+    """M = 15 exceeds the dense register gather: the sparse LDS gather runs in FP32."""
+    require_device()
+    sp = configs.grin_fiber(M=15, nt=1024, symmetric=True)
+    err, _, _ = run_mixed(sp, 10, decode=1)
+    print("rel-L2", err)
+    assert np.all(err <= MIXED_SHORT), err
+
+
+def test_mixed_graph_replays_equal_eager():
+    require_device()
+    sp = configs.grin_fiber(M=6, nt=4096, symmetric=True)
+    note("case mixed graph vs eager")
+    a = MultimodePropagationGPU_RamanON(sp, pair_decode=1, precision=MIXED)
+    b = MultimodePropagationGPU_RamanON(sp, pair_decode=1, precision=MIXED, flags=_capi.MMP_FLAG_NO_GRAPH)
+    a.computeLawsonRK(48)
+    b.computeLawsonRK(48)
+    ra, rb = a.getResult(), b.getResult()
+    a.close()
+    b.close()
+    assert np.array_equal(ra, rb)
+
+
+def test_fused_columns_are_rejected_in_mixed_precision():
+    require_device()
+    with pytest.raises(_capi.MmpError) as ei:
+        MultimodePropagationGPU_RamanON(configs.two_mode_gaussian(nt=1024), precision=MIXED,
+                                        flags=_capi.MMP_FLAG_FUSED_COLUMNS)
+    assert ei.value.status == _capi.MMP_ERR_UNSUPPORTED

@@ -9,7 +9,7 @@ import pytest
 
 import oracle_binding as ob
 from gpu_util import note, require_device
-from mm_propagation_amd import _capi, configs
+from mm_propagation_amd import _capi, configs, plan
 from mm_propagation_amd.solver import MultimodePropagationGPU_RamanON, make_solver
 
 pytestmark = pytest.mark.gpu
@@ -112,3 +112,16 @@ def test_invalid_inputs_rejected_on_device_path(mutate, status):
     with pytest.raises(_capi.MmpError) as ei:
         MultimodePropagationGPU_RamanON(sp)
     assert ei.value.status == status
+
+
+@pytest.mark.parametrize("precision,flags", [(0, 0), (1, 0), (0, _capi.MMP_FLAG_FUSED_COLUMNS)])
+def test_planned_bytes_equal_allocated_bytes(precision, flags):
+    """mmp_plan's device_bytes (host only) is what a solver with the same options allocates."""
+    require_device()
+    sp = configs.grin_fiber(M=6, nt=4096, symmetric=True)
+    note(f"case planned bytes precision={precision} flags={flags}")
+    planned = plan.plan(sp, pair_decode=1, precision=precision, flags=flags)["device_bytes"]
+    s = MultimodePropagationGPU_RamanON(sp, pair_decode=1, precision=precision, flags=flags)
+    got = s.info()["device_bytes"]
+    s.close()
+    assert planned == got
