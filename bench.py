@@ -68,6 +68,11 @@ def main() -> int:
                     help="launch kernels one by one instead of replaying HIP graphs (for kernel tracers)")
     ap.add_argument("--fused-columns", action="store_true",
                     help="fused column passes (4 per stage instead of 7)")
+    ap.add_argument("--mixed", action="store_true",
+                    help="mixed precision (FP32 transforms and sums, FP64 state); the bench line is FP64 by default")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="after the bench, copy this many MiB device to device once with the L2 evicted first "
+                         "(the reference dispatch for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -81,7 +86,9 @@ def main() -> int:
     M, NT = sp.n_modes, sp.nt
     ctx = Context(0)
     flags = (_capi.MMP_FLAG_NO_GRAPH if args.no_graph else 0) | (_capi.MMP_FLAG_FUSED_COLUMNS if args.fused_columns else 0)
-    s = MultimodePropagationGPU_RamanON(sp, pair_decode=_capi.MMP_PAIRS_PHYSICAL, flags=flags, ctx=ctx)
+    prec = _capi.MMP_PRECISION_MIXED if args.mixed else _capi.MMP_PRECISION_FP64
+    s = MultimodePropagationGPU_RamanON(sp, pair_decode=_capi.MMP_PAIRS_PHYSICAL, flags=flags, ctx=ctx,
+                                        precision=prec)
     psi0, _ = s.get_state()
     if args.warmup:
         s.advance(args.warmup)
@@ -102,7 +109,8 @@ def main() -> int:
 
     # per-kernel HIP-event timings on the solver's stream over a profiled run of the same workload
     ps = MultimodePropagationGPU_RamanON(sp, pair_decode=_capi.MMP_PAIRS_PHYSICAL,
-                                         flags=_capi.MMP_FLAG_PROFILE | (flags & _capi.MMP_FLAG_FUSED_COLUMNS), ctx=ctx)
+                                         flags=_capi.MMP_FLAG_PROFILE | (flags & _capi.MMP_FLAG_FUSED_COLUMNS), ctx=ctx,
+                                         precision=prec)
     ps.advance(8)
     ps.kernel_times()
     ps.advance(args.profile_steps)
@@ -126,7 +134,8 @@ def main() -> int:
     if not args.no_cpu_baseline and args.cpu_steps > 0:
         cpu, ref = cpu_baseline(sp, args.cpu_steps)
         # the same solve prefix on the device, compared field for field with the oracle
-        vs = MultimodePropagationGPU_RamanON(sp, pair_decode=_capi.MMP_PAIRS_PHYSICAL, flags=flags, ctx=ctx)
+        vs = MultimodePropagationGPU_RamanON(sp, pair_decode=_capi.MMP_PAIRS_PHYSICAL, flags=flags, ctx=ctx,
+                                             precision=prec)
         vs.computeLawsonRK(args.cpu_steps)
         err = configs.rel_l2_per_mode(vs.getResult(), ref)
         vs.close()
@@ -139,7 +148,7 @@ def main() -> int:
     out = {
         "metric": METRIC, "value": value, "unit": "M·NT·steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if args.mixed else "f64", "data": "synthetic",
         "config": {"workload": "grin6_fp64: M=6, NT=16384, T_w=20 ps, L=1 m, n_steps=20000, RK4, Raman on, "
                                "symmetric GRIN coupling nnz=648, physical pair decode",
                    "global_batch": 1, "seq_len": NT, "parallelism": "single GPU"},
@@ -147,7 +156,24 @@ def main() -> int:
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+    if args.copy_calibration_mib > 0:
+        copy_calibration(args.copy_calibration_mib)
     return 0
+
+
+def copy_calibration(mib: int) -> None:
+    """One device-to-device copy of `mib` MiB after writing 512 MiB elsewhere (evicting the
+    L2s), so FETCH_SIZE and WRITE_SIZE of that dispatch should each read `mib` MiB."""
+    import torch
+    n = mib * 1024 * 1024 // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    evict = torch.empty(512 * 1024 * 1024 // 8, dtype=torch.float64, device="cuda")
+    evict.fill_(2.0)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_mib": mib}), flush=True)
 
 
 if __name__ == "__main__":

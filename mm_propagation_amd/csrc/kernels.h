@@ -111,19 +111,17 @@ struct PairLineArgs {
   const uint32_t* line_self;  // [n_lines] 1 = packed self pairs, 0 = cross pair a, b
 };
 
-// Fused column passes (one workgroup per column, Raman on, N1 >= 64).
+// Column-stage kernels (one workgroup per column, Raman on, N1 = 64, M <= 12): Raman lines
+// are processed in chunks of chunk_lines lines; chunk c covers slots [chunk_slot[c], chunk_slot[c+1]).
 struct FusedArgs {
   const uint32_t* line_slot0;  // [n_lines + 1] first device slot of each Raman line
   const uint32_t* chunk_slot;  // [n_chunks + 1] first slot of each chunk of Raman lines
-  const uint32_t* coff;        // [M][n_chunks + 1] nonzero ranges per chunk
   int n_lines, chunk_lines, n_chunks;
 };
 
-// LDS budget of the fused column kernels (complex values), the part the line transforms
-// use, and the largest mode count the gather lanes can hold for column length L.
-int fused_lds_capacity();
-int fused_fft_slots();
-int fused_max_modes(int L);
+// Largest mode count of the column-stage kernels and the slot mixes one LDS chunk holds.
+int colstage_max_modes();
+int colstage_slots_per_chunk();
 
 // Launchers (return hipSuccess or the launch error). Those templated on W run in the
 // work precision W (double, or float for the mixed-precision step); the others are FP64.
@@ -149,10 +147,14 @@ hipError_t launch_row_raman(const Geometry& g, const Twiddles& tw, cpx<W>* data,
 template <class W>
 hipError_t launch_gather(const Geometry& g, const cpx<W>* U, const cpx<W>* VR, cpx<W>* sigma, const GatherArgs& a,
                          hipStream_t s);
-hipError_t launch_col_fields_pairs(const Geometry& g, const Twiddles& tw, const cd* yin, cd* U, cd* conv,
-                                   int n_lines, const PairLineArgs& pl, double scale, hipStream_t s);
-hipError_t launch_col_raman_gather(const Geometry& g, const Twiddles& tw, const cd* U, const cd* conv, cd* yout,
-                                   const GatherArgs& ga, const FusedArgs& fa, double scale, hipStream_t s);
+template <class W>
+hipError_t launch_colstage_fields_pairs(const Geometry& g, const Twiddles& tw, const cpx<W>* yin, cpx<W>* U,
+                                        cpx<W>* conv, int n_lines, const PairLineArgs& pl, double scale,
+                                        hipStream_t s);
+template <class W>
+hipError_t launch_colstage_raman_gather(const Geometry& g, const Twiddles& tw, const cpx<W>* U, const cpx<W>* conv,
+                                        cpx<W>* yout, const GatherArgs& ga, const FusedArgs& fa, const W* Qd,
+                                        double scale, hipStream_t s);
 // Dense-block gather (M <= 16): Qd is [P][M][M] with Qd[s][p][k] = sum of Q/max|Q| over the
 // nonzeros (k, slot s) of Q^(p). dense_gather_chunk(M) is 0 when M is out of range.
 int dense_gather_chunk(int M);

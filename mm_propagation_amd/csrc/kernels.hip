@@ -557,60 +557,63 @@ __global__ void __launch_bounds__(512) k_gather(Geometry g, const cpx<W>* __rest
 }
 
 // ---------------------------------------------------------------------------
-// Fused column passes. One workgroup of 256 lanes owns one column n2; its
-// lanes form 2048/L line slots of L/8 lanes that transform one line each per
-// round. The column of every mode field stays in LDS between the transforms.
+// Column-stage kernels (Raman on, N1 = 64, M <= 12). One workgroup of 256 lanes
+// owns one column n2 and does every column-side step of a stage there, with the
+// column of each field and Raman line held in LDS between the steps:
+//   C1  inverse column DFT of the M fields -> U (HBM and LDS) -> pair products ->
+//       forward column DFT of the Raman lines -> conv
+//   C2  inverse column DFT of the Raman lines -> slot mixes X_s in LDS -> dense
+//       Kerr + Raman gather of the 64 samples -> forward column DFT of sigma -> yout
+// Lanes form 32 line slots of 8 lanes; a round transforms up to 32 lines.
 // ---------------------------------------------------------------------------
-constexpr int kFusedCap = 10240;  // complex LDS values (160 KB)
-constexpr int kFusedThreads = 256;
-constexpr int kFusedMaxPW = 16;   // modes per wavefront in the gather
+constexpr int kColStageL = 64;       // column length the column-stage kernels take
+constexpr int kColStageMaxM = 12;    // modes kept in registers by the gather
+constexpr int kColStageXCap = 3072;  // complex LDS values for slot mixes / partial sums
 
-template <int L>
-struct FusedCfg {
-  static constexpr int T = L / 8;
-  static constexpr int NL = kFusedThreads / T;  // line slots per round
-  static constexpr int SLOTS = LineSlots<L, 1>::value;
-  static constexpr int FFT = NL * SLOTS;        // 2304 for every L
-  static constexpr int SG = L >= 64 ? L / 64 : 1;  // sample groups of 64 in a column
-  static constexpr int PG = 4 / SG;             // mode groups (4 wavefronts)
-  static_assert(L >= 64 && L <= 256, "fused column passes need 64 <= N1 <= 256");
+struct ColStageCfg {
+  static constexpr int L = kColStageL;
+  static constexpr int T = L / 8;                      // 8 lanes per line
+  static constexpr int NL = 256 / T;                   // 32 line slots per round
+  static constexpr int SLOTS = LineSlots<L, 1>::value;  // 72
+  static constexpr int FFT = NL * SLOTS;               // 2304
 };
 
-int fused_lds_capacity() { return kFusedCap; }
-int fused_fft_slots() { return FusedCfg<64>::FFT; }
-int fused_max_modes(int L) { return kFusedMaxPW * (L >= 256 ? 1 : (L >= 128 ? 2 : 4)); }
+int colstage_max_modes() { return kColStageMaxM; }
+int colstage_slots_per_chunk() { return kColStageXCap / kColStageL; }
 
-// C1: inverse column DFT of the M mode lines -> U (global and LDS), then the Raman
-// pair products and their forward column DFT -> conv.
-template <int L>
-__global__ void __launch_bounds__(256) k_col_fields_pairs(Geometry g, const cd* __restrict__ coltw,
-                                                          const cd* __restrict__ fourtw, const cd* __restrict__ yin,
-                                                          cd* __restrict__ U, cd* __restrict__ conv, int n_lines,
-                                                          PairLineArgs pl, double scale) {
-  using F = FusedCfg<L>;
-  __shared__ cd sh[kFusedCap];
-  cd* Ush = sh + F::FFT;  // [M][L]
+template <class W>
+__global__ void __launch_bounds__(256) k_colstage_fields_pairs(Geometry g, const cpx<W>* __restrict__ coltw,
+                                                               const cpx<W>* __restrict__ fourtw,
+                                                               const cpx<W>* __restrict__ yin, cpx<W>* __restrict__ U,
+                                                               cpx<W>* __restrict__ conv, int n_lines,
+                                                               PairLineArgs pl, double scale) {
+  using F = ColStageCfg;
+  using cw = cpx<W>;
+  constexpr int L = F::L;
+  __shared__ cw sh[F::FFT + kColStageMaxM * L];
+  cw* Ush = sh + F::FFT;  // [M][L]
   const int tid = threadIdx.x;
   const int job = tid / F::T, t = tid % F::T;
   const int n2 = xcd_block(blockIdx.x, gridDim.x);
-  cd* fsm = sh + job * F::SLOTS;
-  LineTw<L> lt;
+  cw* fsm = sh + job * F::SLOTS;
+  LineTw<L, W> lt;
   load_line_tw<L>(lt, coltw, t);
-  cd w4[8], v[8];
+  cw w4[8], v[8];
 #pragma unroll
   for (int i = 0; i < 8; i++) w4[i] = fourtw[n2 * (t + i * F::T)];
+  const W sc = (W)scale;
   for (int r0 = 0; r0 < g.M; r0 += F::NL) {
     const int q = r0 + job;
     const bool valid = q < g.M;
     const size_t base = (size_t)(valid ? q : 0) * g.N + n2;
 #pragma unroll
-    for (int i = 0; i < 8; i++) v[i] = valid ? cmulc(yin[base + (size_t)(t + i * F::T) * g.N2], w4[i]) : cd{0., 0.};
+    for (int i = 0; i < 8; i++) v[i] = valid ? cmulc(yin[base + (size_t)(t + i * F::T) * g.N2], w4[i]) : cw{0, 0};
     line_fft<L, 1, 1, 1>(v, fsm, lt, t);
     if (valid) {
 #pragma unroll
       for (int i = 0; i < 8; i++) {
         const int n1 = t + i * F::T;
-        const cd x = cscale(v[i], scale);
+        const cw x = cscale(v[i], sc);
         U[base + (size_t)n1 * g.N2] = x;
         Ush[q * L + n1] = x;
       }
@@ -625,13 +628,13 @@ __global__ void __launch_bounds__(256) k_col_fields_pairs(Geometry g, const cd* 
 #pragma unroll
     for (int i = 0; i < 8; i++) {
       const int n1 = t + i * F::T;
-      cd x = {0., 0.};
+      cw x = {0, 0};
       if (valid) {
-        const cd ua = Ush[la * L + n1];
+        const cw ua = Ush[la * L + n1];
         if (self) {
           x.re = ua.re * ua.re + ua.im * ua.im;
           if (lb != 0xffffffffu) {
-            const cd ub = Ush[lb * L + n1];
+            const cw ub = Ush[lb * L + n1];
             x.im = ub.re * ub.re + ub.im * ub.im;
           }
         } else {
@@ -649,41 +652,42 @@ __global__ void __launch_bounds__(256) k_col_fields_pairs(Geometry g, const cd* 
   }
 }
 
-// C2: inverse column DFT of the Raman lines (chunk by chunk) -> slot mixes X in LDS ->
-// Kerr + Raman gather for the column's samples -> sigma in LDS -> forward column DFT.
-template <int L>
-__global__ void __launch_bounds__(256) k_col_raman_gather(Geometry g, const cd* __restrict__ coltw,
-                                                          const cd* __restrict__ fourtw, const cd* __restrict__ U,
-                                                          const cd* __restrict__ conv, cd* __restrict__ yout,
-                                                          GatherArgs a, FusedArgs fa, double scale) {
-  using F = FusedCfg<L>;
-  __shared__ cd sh[kFusedCap];
-  const int M = g.M;
-  cd* Ush = sh + F::FFT;     // [M][L]; holds sigma after the gather
-  cd* Xsh = Ush + M * L;     // [slots of a chunk][L]
+template <int M, class W>
+__global__ void __launch_bounds__(256) k_colstage_raman_gather(Geometry g, const cpx<W>* __restrict__ coltw,
+                                                               const cpx<W>* __restrict__ fourtw,
+                                                               const cpx<W>* __restrict__ U,
+                                                               const cpx<W>* __restrict__ conv,
+                                                               cpx<W>* __restrict__ yout, GatherArgs a, FusedArgs fa,
+                                                               const W* __restrict__ Qd, double scale) {
+  using F = ColStageCfg;
+  using cw = cpx<W>;
+  constexpr int L = F::L;
+  static_assert(M >= 1 && M <= kColStageMaxM && 4 * M * L <= kColStageXCap, "modes out of range");
+  __shared__ cw sh[F::FFT + M * L + kColStageXCap];
+  cw* Ush = sh + F::FFT;  // [M][L] fields; sigma after the gather
+  cw* Xsh = Ush + M * L;  // [slots of a chunk][L]; partial sums after the gather
   const int tid = threadIdx.x;
   const int job = tid / F::T, t = tid % F::T;
   const int n2 = xcd_block(blockIdx.x, gridDim.x);
-  cd* fsm = sh + job * F::SLOTS;
-  LineTw<L> lt;
+  cw* fsm = sh + job * F::SLOTS;
+  LineTw<L, W> lt;
   load_line_tw<L>(lt, coltw, t);
-  cd w4[8], v[8];
+  cw w4[8], v[8];
 #pragma unroll
   for (int i = 0; i < 8; i++) w4[i] = fourtw[n2 * (t + i * F::T)];
 #pragma unroll 4
-  for (int e = tid; e < M * L; e += kFusedThreads) Ush[e] = U[(size_t)(e / L) * g.N + (size_t)(e % L) * g.N2 + n2];
-  // gather lanes: wavefront w covers samples sg*64 + lane of modes pg, pg + PG, ...
+  for (int e = tid; e < M * L; e += 256) Ush[e] = U[(size_t)(e / L) * g.N + (size_t)(e % L) * g.N2 + n2];
+  const W sc = (W)scale, kv = (W)a.kv, kr = (W)a.kr;
+  // gather lanes: sample `lane`, slots s0 + wave, s0 + wave + 4, ...
   const int lane = tid % 64;
-  const int wave = __builtin_amdgcn_readfirstlane(tid / 64);  // wave-uniform for the compiler
-  const int sg = wave % F::SG, pg = wave / F::SG;
-  const int n1g = sg * 64 + lane;
-  cd acc[kFusedMaxPW];
+  const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
+  cw u[M], acc[M];
 #pragma unroll
-  for (int r = 0; r < kFusedMaxPW; r++) acc[r] = {0., 0.};
+  for (int p = 0; p < M; p++) acc[p] = {0, 0};
   for (int c = 0; c < fa.n_chunks; c++) {
     const int l0 = c * fa.chunk_lines;
     const int l1 = min(fa.n_lines, l0 + fa.chunk_lines);
-    const int s0 = (int)fa.chunk_slot[c];
+    const int s0 = (int)fa.chunk_slot[c], s1 = (int)fa.chunk_slot[c + 1];
     __syncthreads();  // Ush loaded / Xsh of the previous chunk consumed
     for (int r0 = l0; r0 < l1; r0 += F::NL) {
       const int q = r0 + job;
@@ -691,7 +695,7 @@ __global__ void __launch_bounds__(256) k_col_raman_gather(Geometry g, const cd* 
       const size_t base = (size_t)(valid ? q : 0) * g.N + n2;
 #pragma unroll
       for (int i = 0; i < 8; i++)
-        v[i] = valid ? cmulc(conv[base + (size_t)(t + i * F::T) * g.N2], w4[i]) : cd{0., 0.};
+        v[i] = valid ? cmulc(conv[base + (size_t)(t + i * F::T) * g.N2], w4[i]) : cw{0, 0};
       line_fft<L, 1, 1, 1>(v, fsm, lt, t);
       if (valid) {
         const int sa = (int)fa.line_slot0[q], sb = (int)fa.line_slot0[q + 1];
@@ -700,43 +704,57 @@ __global__ void __launch_bounds__(256) k_col_raman_gather(Geometry g, const cd* 
 #pragma unroll
           for (int i = 0; i < 8; i++) {
             const int n1 = t + i * F::T;
-            const cd r = cscale(v[i], scale);
-            cd vr;
+            const cw r = cscale(v[i], sc);
+            cw vr;
             switch (kind) {
               case kDirect: vr = r; break;
               case kConj: vr = cconj(r); break;
-              case kRe: vr = {r.re, 0.}; break;
-              default: vr = {r.im, 0.}; break;
+              case kRe: vr = {r.re, 0}; break;
+              default: vr = {r.im, 0}; break;
             }
-            const cd V = cmulc(Ush[pa * L + n1], Ush[pb * L + n1]);
-            Xsh[(sl - s0) * L + n1] = cadd(cscale(V, a.kv), cscale(vr, a.kr));
+            const cw V = cmulc(Ush[pa * L + n1], Ush[pb * L + n1]);
+            Xsh[(sl - s0) * L + n1] = cadd(cscale(V, kv), cscale(vr, kr));
           }
         }
       }
     }
     __syncthreads();
+    if (c == 0) {
 #pragma unroll
-    for (int r = 0; r < kFusedMaxPW; r++) {
-      const int p = pg + r * F::PG;
-      if (p >= M) break;
-      const uint32_t* co = fa.coff + (size_t)p * (fa.n_chunks + 1) + c;
-      acc[r] = gather_sum(acc[r], a.kslot, static_cast<const double*>(a.q), co[0], co[1], s0, Xsh + n1g, Ush + n1g,
-                          L, lane);
+      for (int k = 0; k < M; k++) u[k] = Ush[k * L + lane];
+    }
+    for (int sl = s0 + wave; sl < s1; sl += 4) {
+      const cw X = Xsh[(sl - s0) * L + lane];
+      const W* qb = Qd + (size_t)sl * M * M;
+      cw y[M];
+#pragma unroll
+      for (int k = 0; k < M; k++) y[k] = cmul(X, u[k]);
+#pragma unroll
+      for (int p = 0; p < M; p++) {
+        cw tsum = acc[p];
+#pragma unroll
+        for (int k = 0; k < M; k++) tsum = cadd(tsum, cscale(y[k], qb[p * M + k]));
+        acc[p] = tsum;
+      }
     }
   }
-  __syncthreads();  // every lane done reading Ush and Xsh
+  __syncthreads();  // every lane done with Ush and Xsh
 #pragma unroll
-  for (int r = 0; r < kFusedMaxPW; r++) {
-    const int p = pg + r * F::PG;
-    if (p >= M) break;
-    Ush[p * L + n1g] = acc[r];
+  for (int p = 0; p < M; p++) Xsh[(wave * M + p) * L + lane] = acc[p];
+  __syncthreads();
+  for (int e = tid; e < M * L; e += 256) {
+    const int p = e / L, j = e % L;
+    cw tsum = Xsh[p * L + j];
+#pragma unroll
+    for (int w = 1; w < 4; w++) tsum = cadd(tsum, Xsh[(w * M + p) * L + j]);
+    Ush[e] = tsum;
   }
   __syncthreads();
   for (int r0 = 0; r0 < M; r0 += F::NL) {
     const int q = r0 + job;
     const bool valid = q < M;
 #pragma unroll
-    for (int i = 0; i < 8; i++) v[i] = valid ? Ush[q * L + t + i * F::T] : cd{0., 0.};
+    for (int i = 0; i < 8; i++) v[i] = valid ? Ush[q * L + t + i * F::T] : cw{0, 0};
     line_fft<L, -1, 1, 1>(v, fsm, lt, t);
     if (valid) {
       const size_t base = (size_t)q * g.N + n2;
@@ -1007,43 +1025,36 @@ hipError_t launch_gather(const Geometry& g, const cpx<W>* U, const cpx<W>* VR, c
   return hipGetLastError();
 }
 
-template <int L>
-static hipError_t fields_pairs_impl(const Geometry& g, const Twiddles& tw, const cd* yin, cd* U, cd* conv,
-                                    int n_lines, const PairLineArgs& pl, double scale, hipStream_t s) {
-  if (FusedCfg<L>::FFT + g.M * L > kFusedCap) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_col_fields_pairs<L>, dim3(g.N2), dim3(kFusedThreads), 0, s, g, tw.coltw, tw.fourtw, yin, U,
-                     conv, n_lines, pl, scale);
+template <class W>
+hipError_t launch_colstage_fields_pairs(const Geometry& g, const Twiddles& tw, const cpx<W>* yin, cpx<W>* U,
+                                        cpx<W>* conv, int n_lines, const PairLineArgs& pl, double scale,
+                                        hipStream_t s) {
+  if (g.N1 != kColStageL || g.M > kColStageMaxM) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_colstage_fields_pairs<W>), dim3(g.N2), dim3(256), 0, s, g, TwOf<W>::col(tw),
+                     TwOf<W>::four(tw), yin, U, conv, n_lines, pl, scale);
   return hipGetLastError();
-}
-hipError_t launch_col_fields_pairs(const Geometry& g, const Twiddles& tw, const cd* yin, cd* U, cd* conv,
-                                   int n_lines, const PairLineArgs& pl, double scale, hipStream_t s) {
-  switch (g.N1) {
-    case 64: return fields_pairs_impl<64>(g, tw, yin, U, conv, n_lines, pl, scale, s);
-    case 128: return fields_pairs_impl<128>(g, tw, yin, U, conv, n_lines, pl, scale, s);
-    case 256: return fields_pairs_impl<256>(g, tw, yin, U, conv, n_lines, pl, scale, s);
-    default: return hipErrorInvalidValue;
-  }
 }
 
-template <int L>
-static hipError_t raman_gather_impl(const Geometry& g, const Twiddles& tw, const cd* U, const cd* conv, cd* yout,
-                                    const GatherArgs& ga, const FusedArgs& fa, double scale, hipStream_t s) {
-  using F = FusedCfg<L>;
-  // the largest chunk holds at most two slots per line
-  if (F::FFT + g.M * L + 2 * fa.chunk_lines * L > kFusedCap) return hipErrorInvalidValue;
-  if ((g.M + F::PG - 1) / F::PG > kFusedMaxPW) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_col_raman_gather<L>, dim3(g.N2), dim3(kFusedThreads), 0, s, g, tw.coltw, tw.fourtw, U, conv,
-                     yout, ga, fa, scale);
-  return hipGetLastError();
-}
-hipError_t launch_col_raman_gather(const Geometry& g, const Twiddles& tw, const cd* U, const cd* conv, cd* yout,
-                                   const GatherArgs& ga, const FusedArgs& fa, double scale, hipStream_t s) {
-  switch (g.N1) {
-    case 64: return raman_gather_impl<64>(g, tw, U, conv, yout, ga, fa, scale, s);
-    case 128: return raman_gather_impl<128>(g, tw, U, conv, yout, ga, fa, scale, s);
-    case 256: return raman_gather_impl<256>(g, tw, U, conv, yout, ga, fa, scale, s);
+#define MMP_COLSTAGE_CASE(MV)                                                                                     \
+  case MV:                                                                                                         \
+    hipLaunchKernelGGL((k_colstage_raman_gather<MV, W>), dim3(g.N2), dim3(256), 0, s, g, TwOf<W>::col(tw),         \
+                       TwOf<W>::four(tw), U, conv, yout, ga, fa, Qd, scale);                                      \
+    break;
+
+template <class W>
+hipError_t launch_colstage_raman_gather(const Geometry& g, const Twiddles& tw, const cpx<W>* U, const cpx<W>* conv,
+                                        cpx<W>* yout, const GatherArgs& ga, const FusedArgs& fa, const W* Qd,
+                                        double scale, hipStream_t s) {
+  if (g.N1 != kColStageL) return hipErrorInvalidValue;
+  if (fa.n_chunks < 1 || fa.chunk_lines < 1 || 2 * fa.chunk_lines > colstage_slots_per_chunk())
+    return hipErrorInvalidValue;
+  switch (g.M) {
+    MMP_COLSTAGE_CASE(1) MMP_COLSTAGE_CASE(2) MMP_COLSTAGE_CASE(3) MMP_COLSTAGE_CASE(4) MMP_COLSTAGE_CASE(5)
+    MMP_COLSTAGE_CASE(6) MMP_COLSTAGE_CASE(7) MMP_COLSTAGE_CASE(8) MMP_COLSTAGE_CASE(9) MMP_COLSTAGE_CASE(10)
+    MMP_COLSTAGE_CASE(11) MMP_COLSTAGE_CASE(12)
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
 }
 
 #define MMP_DENSE_CASE(MV)                                                                                     \
@@ -1100,7 +1111,12 @@ hipError_t launch_energy(const Geometry& g, const cd* fields, double* out, doubl
   template hipError_t launch_gather<W>(const Geometry&, const cpx<W>*, const cpx<W>*, cpx<W>*, const GatherArgs&,  \
                                        hipStream_t);                                                              \
   template hipError_t launch_gather_dense<W>(const Geometry&, const cpx<W>*, const cpx<W>*, cpx<W>*,               \
-                                             const GatherArgs&, const W*, hipStream_t);
+                                             const GatherArgs&, const W*, hipStream_t);                            \
+  template hipError_t launch_colstage_fields_pairs<W>(const Geometry&, const Twiddles&, const cpx<W>*, cpx<W>*,     \
+                                                      cpx<W>*, int, const PairLineArgs&, double, hipStream_t);     \
+  template hipError_t launch_colstage_raman_gather<W>(const Geometry&, const Twiddles&, const cpx<W>*,             \
+                                                      const cpx<W>*, cpx<W>*, const GatherArgs&, const FusedArgs&, \
+                                                      const W*, double, hipStream_t);
 MMP_INSTANTIATE_W(double)
 MMP_INSTANTIATE_W(float)
 

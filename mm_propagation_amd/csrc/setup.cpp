@@ -94,7 +94,10 @@ bool split_length(uint32_t nt, uint32_t& n1, uint32_t& n2) {
   if (nt < 64 || nt > 65536 || (nt & (nt - 1)) != 0) return false;
   uint32_t lg = 0;
   while ((1u << lg) < nt) lg++;
-  n1 = 1u << (lg / 2);
+  // columns of 64 samples while rows stay within 256 (more columns = more parallel column
+  // workgroups, and a 64-sample column of every field and Raman line fits in LDS)
+  n1 = std::min(1u << (lg / 2), 64u);
+  if (nt / n1 > 256) n1 = nt / 256;
   n2 = nt / n1;
   return true;
 }
@@ -140,10 +143,6 @@ mmp_status prepare_problem(const mmp_params* in, const mmp_options& opt, Problem
   if (in->raman && !in->raman_response) { err = "raman_response is required with Raman on"; return MMP_ERR_INVALID_ARGUMENT; }
   if (opt.precision != MMP_PRECISION_FP64 && opt.precision != MMP_PRECISION_MIXED) {
     err = "unknown precision";
-    return MMP_ERR_UNSUPPORTED;
-  }
-  if (opt.precision == MMP_PRECISION_MIXED && (opt.flags & MMP_FLAG_FUSED_COLUMNS)) {
-    err = "the fused column passes are FP64 only";
     return MMP_ERR_UNSUPPORTED;
   }
   if (opt.pair_decode != MMP_PAIRS_REFERENCE_CPU && opt.pair_decode != MMP_PAIRS_PHYSICAL) {
@@ -389,28 +388,19 @@ mmp_status prepare_problem(const mmp_params* in, const mmp_options& opt, Problem
   }
   P.kv = P.raman ? 1 - P.fR : 1.;
   P.kr = P.raman ? P.fR * P.T0 : 0.;
-  // Fused column passes (Raman on, column length N1 >= 64): Raman lines are processed in
-  // chunks of fused_lines; chunk c covers slots [fused_slot[c], fused_slot[c+1]).
+  // Column-stage kernels (Raman on, N1 = 64, dense gather, M <= 12): Raman lines are
+  // processed in chunks of fused_lines (at most two slots each); chunk c covers slots
+  // [fused_slot[c], fused_slot[c+1]).
   P.fused_lines = 0;
   P.fused_chunks = 0;
-  if (P.raman && P.N1 >= 64) {
-    const int nl = (int)P.line_a.size();
-    const int room = fused_lds_capacity() - fused_fft_slots() - (int)(M * P.N1);
-    const int cl = room > 0 ? room / (2 * (int)P.N1) : 0;
-    if (cl >= 1 && (int)M <= fused_max_modes((int)P.N1) && nl > 0) {
-      P.fused_lines = cl < nl ? cl : nl;
-      P.fused_chunks = (nl + P.fused_lines - 1) / P.fused_lines;
-      P.fused_slot.assign(P.fused_chunks + 1, P.P);
-      for (int c = 0; c < P.fused_chunks; c++) P.fused_slot[c] = P.line_slot0[c * P.fused_lines];
-      P.fused_coff.assign((size_t)M * (P.fused_chunks + 1), 0);
-      for (uint32_t p = 0; p < M; p++) {
-        uint32_t e = P.off[p];
-        for (int c = 0; c <= P.fused_chunks; c++) {
-          while (e < P.off[p + 1] && P.slot[e] < P.fused_slot[c]) e++;
-          P.fused_coff[(size_t)p * (P.fused_chunks + 1) + c] = (c == P.fused_chunks) ? P.off[p + 1] : e;
-        }
-      }
-    }
+  const int nl = (int)P.line_a.size();
+  if (P.raman && P.N1 == 64 && (int)M <= colstage_max_modes() && dense_gather_chunk((int)M) > 0 && nl > 0 &&
+      P.line_slot0.size() == (size_t)nl + 1 && P.line_slot0[nl] == P.P) {
+    const int per_chunk = colstage_slots_per_chunk() / 2;
+    P.fused_lines = per_chunk < nl ? per_chunk : nl;
+    P.fused_chunks = (nl + P.fused_lines - 1) / P.fused_lines;
+    P.fused_slot.assign(P.fused_chunks + 1, P.P);
+    for (int c = 0; c < P.fused_chunks; c++) P.fused_slot[c] = P.line_slot0[c * P.fused_lines];
   }
   // a stage result N_j needs a stash slot when a stage i > j + 1 reads it
   P.stash_needed.assign(P.rk.s, 0);

@@ -51,10 +51,9 @@ struct Problem {
   // Raman convolution lines
   std::vector<uint32_t> line_a, line_b, line_self;  // [n_lines]
   std::vector<uint32_t> line_slot0;                 // [n_lines + 1] first device slot of each line
-  // fused column passes: Raman lines per LDS chunk, chunk slot bounds, per-chunk nonzero ranges
+  // column-stage kernels: Raman lines per LDS chunk and the chunks' slot bounds (0 chunks: not eligible)
   int fused_lines = 0, fused_chunks = 0;
   std::vector<uint32_t> fused_slot;                 // [fused_chunks + 1]
-  std::vector<uint32_t> fused_coff;                 // [M][fused_chunks + 1]
   // Kerr gather: nonzeros of each Q^(p) sorted by (slot, k), with per-chunk ranges
   std::vector<uint32_t> off, k, slot;  // [M+1], [nnz], [nnz]
   int gather_cap = 0, gather_chunk = 0, gather_chunks = 0;
@@ -68,7 +67,8 @@ struct Problem {
 // Validates and prepares. On failure returns a status and fills err.
 mmp_status prepare_problem(const mmp_params* in, const mmp_options& opt, Problem& out, std::string& err);
 
-// Four-step split NT = N1 * N2 with N1 <= N2 <= 2*N1; false if NT is unsupported.
+// Four-step split NT = N1 * N2: N1 = min(2^floor(log2(NT)/2), 64), raised to NT/256 when the
+// rows would exceed 256 samples; false if NT is unsupported.
 bool split_length(uint32_t nt, uint32_t& n1, uint32_t& n2);
 
 // exp(-2*pi*i*j/n), j < n, evaluated in long double and rounded once.

@@ -226,19 +226,16 @@ mmp_status enqueue_step_w(mmp_solver* s) {
     MMP_TRY(run_kernel(s, i == P.rk.s ? "row_close_step" : "row_stage_head", hb,
                        [&] { return launch_row_head<W>(g, s->tw, h, st); }));
     if (i == P.rk.s) break;
-    if constexpr (sizeof(W) == 8) {
-      if (s->use_fused) {
-        MMP_TRY(run_kernel(s, "col_fields_pairs", cb * (3 * MN + LN), [&] {
-          return launch_col_fields_pairs(g, s->tw, s->ybuf, s->U, s->conv, s->n_lines, s->pairs, invN, st);
-        }));
-        MMP_TRY(run_kernel(s, "row_raman", cb * (2 * LN + P.N), [&] {
-          return launch_row_raman<double>(g, s->tw, s->conv, s->n_lines, s->hspec, P.delta_t, st);
-        }));
-        MMP_TRY(run_kernel(s, "col_raman_gather", cb * (2 * MN + LN), [&] {
-          return launch_col_raman_gather(g, s->tw, s->U, s->conv, s->ybuf, s->gather, s->fused, invN, st);
-        }));
-        continue;
-      }
+    if (s->use_fused) {
+      MMP_TRY(run_kernel(s, "col_fields_pairs", cb * (2 * MN + LN), [&] {
+        return launch_colstage_fields_pairs<W>(g, s->tw, ybuf, U, conv, s->n_lines, s->pairs, invN, st);
+      }));
+      MMP_TRY(run_kernel(s, "row_raman", cb * (2 * LN + P.N),
+                         [&] { return launch_row_raman<W>(g, s->tw, conv, s->n_lines, hspec, P.delta_t, st); }));
+      MMP_TRY(run_kernel(s, "col_raman_gather", cb * (2 * MN + LN), [&] {
+        return launch_colstage_raman_gather<W>(g, s->tw, U, conv, ybuf, s->gather, s->fused, qdense, invN, st);
+      }));
+      continue;
     }
     MMP_TRY(run_kernel(s, "col_inverse_fields", 2 * cb * MN,
                        [&] { return launch_col_inverse<W>(g, s->tw, ybuf, U, P.M, invN, 1.0, st); }));
@@ -354,7 +351,7 @@ size_t planned_device_bytes(const Problem& P, const mmp_options& opt) {
     if (P.raman) b += 8 * (size_t)P.N;
   }
   if (P.fused_chunks > 0 && (opt.flags & MMP_FLAG_FUSED_COLUMNS))
-    b += 4 * (P.line_slot0.size() + P.fused_slot.size() + P.fused_coff.size());
+    b += 4 * (P.line_slot0.size() + P.fused_slot.size());
   if (dense_gather_chunk((int)P.M) > 0) b += (mixed ? 4 : 8) * (size_t)P.P * P.M * P.M;
   return b;
 }
@@ -539,15 +536,13 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
     }
   }
   if (P.fused_chunks > 0 && (opt.flags & MMP_FLAG_FUSED_COLUMNS)) {
-    uint32_t *d_ls0, *d_cs, *d_fc;
+    uint32_t *d_ls0, *d_cs;
     A(dalloc(s, &d_ls0, P.line_slot0.size()));
     A(dalloc(s, &d_cs, P.fused_slot.size()));
-    A(dalloc(s, &d_fc, P.fused_coff.size()));
     if (st != MMP_OK) return cleanup(st);
     A(upload(s, d_ls0, P.line_slot0.data(), P.line_slot0.size() * 4));
     A(upload(s, d_cs, P.fused_slot.data(), P.fused_slot.size() * 4));
-    A(upload(s, d_fc, P.fused_coff.data(), P.fused_coff.size() * 4));
-    s->fused = FusedArgs{d_ls0, d_cs, d_fc, s->n_lines, P.fused_lines, P.fused_chunks};
+    s->fused = FusedArgs{d_ls0, d_cs, s->n_lines, P.fused_lines, P.fused_chunks};
     s->use_fused = true;
   }
   // stage heads

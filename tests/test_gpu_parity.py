@@ -87,10 +87,11 @@ def test_two_mode_gaussian_full_length():
     assert np.all(err <= TOL_FULL), err
 
 
-@pytest.mark.parametrize("M,nt", [(2, 4096), (6, 16384), (10, 65536), (15, 8192)])
+@pytest.mark.parametrize("M,nt", [(2, 4096), (6, 16384), (10, 8192), (12, 4096), (15, 8192)])
 def test_fused_column_passes_match_separate_passes(M, nt):
-    """The fused column passes (4 per stage) against the separate ones (7 per stage), and
-    both against the oracle; 10 and 15 modes run the Raman lines in several LDS chunks."""
+    """The column-stage kernels (4 passes per stage) against the separate passes (7 per stage),
+    and both against the oracle; 10 and 12 modes run the Raman lines in several LDS chunks;
+    15 modes are beyond the column-stage kernels and fall back to the separate passes."""
     require_device()
     sp = configs.two_mode_gaussian(nt=nt) if M == 2 else configs.grin_fiber(M=M, nt=nt, symmetric=True)
     dec = 0 if M == 2 else 1
@@ -106,3 +107,17 @@ def test_fused_column_passes_match_separate_passes(M, nt):
     o = ob.Oracle(sp, pair_decode=dec)
     o.compute_lawson_rk(5)
     assert np.all(configs.rel_l2_per_mode(ra, o.get_result()) <= TOL_SHORT)
+
+
+@pytest.mark.parametrize("order", [4, 6, 8])
+@pytest.mark.parametrize("decode", [0, 1])
+def test_column_stage_kernels_tableaux_and_decode(order, decode):
+    sp = configs.two_mode_gaussian(nt=4096, method_order=order)
+    err, _, _ = run_pair(sp, 25, decode=decode, flags=_capi.MMP_FLAG_FUSED_COLUMNS)
+    assert np.all(err <= TOL_SHORT), err
+
+
+def test_column_stage_kernels_grin_k_le_l_reference_decode():
+    sp = configs.grin_fiber(M=6, nt=16384)
+    err, _, _ = run_pair(sp, 20, decode=0, flags=_capi.MMP_FLAG_FUSED_COLUMNS)
+    assert np.all(err <= TOL_SHORT), err

@@ -44,11 +44,19 @@ def test_prepared_arrays_match_reference_setup_bitwise(name):
     assert np.array_equal(got["e"], r["E"])
 
 
+@pytest.mark.parametrize("nt,n1", [(64, 8), (128, 8), (256, 16), (1024, 32), (2048, 32), (4096, 64), (8192, 64),
+                                   (16384, 64), (32768, 128), (65536, 256)])
+def test_four_step_split(nt, n1):
+    """Columns of at most 64 samples while rows stay within 256 samples."""
+    i = plan.plan(configs.two_mode_gaussian(nt=nt))
+    assert (i["n1"], i["n2"]) == (n1, nt // n1)
+
+
 def test_plan_sizes_and_memory():
     sp = configs.grin_fiber(M=6, nt=16384, symmetric=True)
     i = plan.plan(sp, pair_decode=1)
     assert (i["n_modes"], i["nt"], i["stages"], i["n_pairs"], i["nnz"]) == (6, 16384, 4, 36, 648)
-    assert i["n1"] * i["n2"] == 16384 and i["n1"] == 128
+    assert (i["n1"], i["n2"]) == (64, 256)
     # 21 Raman lines: 15 cross pairs + 3 lines of packed self pairs
     assert i["n_conv_lines"] == 18
     assert 10e6 < i["device_bytes"] < 40e6
