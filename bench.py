@@ -66,8 +66,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels one by one instead of replaying HIP graphs (for kernel tracers)")
-    ap.add_argument("--fused-columns", action="store_true",
-                    help="fused column passes (4 per stage instead of 7)")
+    ap.add_argument("--separate-columns", action="store_true",
+                    help="every column pass a kernel of its own (7 per stage instead of 4)")
     ap.add_argument("--mixed", action="store_true",
                     help="mixed precision (FP32 transforms and sums, FP64 state); the bench line is FP64 by default")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -85,7 +85,7 @@ def main() -> int:
     sp = workload()
     M, NT = sp.n_modes, sp.nt
     ctx = Context(0)
-    flags = (_capi.MMP_FLAG_NO_GRAPH if args.no_graph else 0) | (_capi.MMP_FLAG_FUSED_COLUMNS if args.fused_columns else 0)
+    flags = (_capi.MMP_FLAG_NO_GRAPH if args.no_graph else 0) | (_capi.MMP_FLAG_SEPARATE_COLUMNS if args.separate_columns else 0)
     prec = _capi.MMP_PRECISION_MIXED if args.mixed else _capi.MMP_PRECISION_FP64
     s = MultimodePropagationGPU_RamanON(sp, pair_decode=_capi.MMP_PAIRS_PHYSICAL, flags=flags, ctx=ctx,
                                         precision=prec)
@@ -109,7 +109,7 @@ def main() -> int:
 
     # per-kernel HIP-event timings on the solver's stream over a profiled run of the same workload
     ps = MultimodePropagationGPU_RamanON(sp, pair_decode=_capi.MMP_PAIRS_PHYSICAL,
-                                         flags=_capi.MMP_FLAG_PROFILE | (flags & _capi.MMP_FLAG_FUSED_COLUMNS), ctx=ctx,
+                                         flags=_capi.MMP_FLAG_PROFILE | (flags & _capi.MMP_FLAG_SEPARATE_COLUMNS), ctx=ctx,
                                          precision=prec)
     ps.advance(8)
     ps.kernel_times()

@@ -67,8 +67,9 @@ enum {
   MMP_FLAG_NONE = 0,
   MMP_FLAG_PROFILE = 1u << 0,     /* record per-kernel HIP event times (see mmp_kernel_times) */
   MMP_FLAG_NO_GRAPH = 1u << 1,    /* launch step kernels one by one instead of replaying a HIP graph */
-  MMP_FLAG_FUSED_COLUMNS = 1u << 2 /* fuse the column passes around the Raman row pass (4 passes per stage
-                                      instead of 7; Raman on, N1 >= 64); separate passes are the default */
+  MMP_FLAG_SEPARATE_COLUMNS = 1u << 2 /* keep every column pass a kernel of its own (7 passes per stage)
+                                         where the column-stage kernels would run (4 passes per stage:
+                                         Raman on, 64-sample columns, n_modes <= 12) */
 };
 
 /*

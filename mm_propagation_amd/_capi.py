@@ -35,7 +35,7 @@ MMP_PAIRS_REFERENCE_CPU = 0
 MMP_PAIRS_PHYSICAL = 1
 MMP_FLAG_PROFILE = 1 << 0
 MMP_FLAG_NO_GRAPH = 1 << 1
-MMP_FLAG_FUSED_COLUMNS = 1 << 2
+MMP_FLAG_SEPARATE_COLUMNS = 1 << 2
 
 
 class MmpParams(ctypes.Structure):

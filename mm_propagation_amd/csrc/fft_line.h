@@ -181,6 +181,23 @@ __device__ __forceinline__ void load_line_tw(LineTw<L, T>& lt, const cpx<T>* __r
   }
 }
 
+// Orders the LDS accesses of one wavefront around a data exchange between its lanes.
+// LDS operations of a single wavefront are performed in order, so no wait is needed;
+// the fences only keep the compiler from moving accesses across this point.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Exchange barrier of a line transform: the whole workgroup, or only the wavefront when
+// every line lies inside one wavefront (L/8 lanes per line, lines aligned to it).
+template <bool WAVE>
+__device__ __forceinline__ void line_sync() {
+  if constexpr (WAVE) wave_lds_sync();
+  else __syncthreads();
+}
+
 template <int SIGN, class T>
 __device__ __forceinline__ cpx<T> tw_dir(cpx<T> w) {
   return SIGN < 0 ? w : cpx<T>{w.re, -w.im};
@@ -188,8 +205,9 @@ __device__ __forceinline__ cpx<T> tw_dir(cpx<T> w) {
 
 // In-place transform of the line held as v[i] = x[t + i*T]; sm points at the
 // line's first LDS slot and element pos lives at sm[lds_pos<STRIDE, PAD>(pos)].
-// All lanes of the workgroup must call this together.
-template <int L, int SIGN, int STRIDE, int PAD, class T>
+// All lanes of the workgroup (WAVE = false) or of the wavefront (WAVE = true) must
+// call this together.
+template <int L, int SIGN, int STRIDE, int PAD, class T, bool WAVE = false>
 __device__ __forceinline__ void line_fft(cpx<T>* v, cpx<T>* sm, const LineTw<L, T>& lt, int t) {
   using P = LinePlan<L>;
   constexpr int LT = P::T;
@@ -201,10 +219,10 @@ __device__ __forceinline__ void line_fft(cpx<T>* v, cpx<T>* sm, const LineTw<L, 
     (void)t;
     return;
   } else {
-    __syncthreads();  // the slice may still be read by a previous use
+    line_sync<WAVE>();  // the slice may still be read by a previous use
 #pragma unroll
     for (int q = 0; q < 8; q++) sm[lds_pos<STRIDE, PAD>(8 * t + q)] = v[q];
-    __syncthreads();
+    line_sync<WAVE>();
 #pragma unroll
     for (int i = 0; i < 8; i++) v[i] = sm[lds_pos<STRIDE, PAD>(t + i * LT)];
     if constexpr (P::n8 == 2) {
@@ -213,19 +231,19 @@ __device__ __forceinline__ void line_fft(cpx<T>* v, cpx<T>* sm, const LineTw<L, 
       for (int m = 1; m < 8; m++) v[m] = cmul(v[m], tw_dir<SIGN>(lt.w[m - 1]));
       dft8<SIGN>(v);
       if constexpr (P::tail == 1) {
-        __syncthreads();
+        line_sync<WAVE>();
         return;  // outputs at positions t + q*LT
       }
-      __syncthreads();
+      line_sync<WAVE>();
       const int k = t % 8;
       const int base = (t / 8) * 64 + k;
 #pragma unroll
       for (int q = 0; q < 8; q++) sm[lds_pos<STRIDE, PAD>(base + q * 8)] = v[q];
-      __syncthreads();
+      line_sync<WAVE>();
 #pragma unroll
       for (int i = 0; i < 8; i++) v[i] = sm[lds_pos<STRIDE, PAD>(t + i * LT)];
     }
-    __syncthreads();
+    line_sync<WAVE>();
     constexpr int b0 = LineTw<L, T>::base;
     if constexpr (P::tail == 4) {
 #pragma unroll

@@ -112,16 +112,21 @@ struct PairLineArgs {
 };
 
 // Column-stage kernels (one workgroup per column, Raman on, N1 = 64, M <= 12): Raman lines
-// are processed in chunks of chunk_lines lines; chunk c covers slots [chunk_slot[c], chunk_slot[c+1]).
+// are processed in chunks; chunk c covers lines [chunk_line[c], chunk_line[c+1]) and their
+// slots [chunk_slot[c], chunk_slot[c+1]) (slots are numbered by line).
 struct FusedArgs {
-  const uint32_t* line_slot0;  // [n_lines + 1] first device slot of each Raman line
-  const uint32_t* chunk_slot;  // [n_chunks + 1] first slot of each chunk of Raman lines
-  int n_lines, chunk_lines, n_chunks;
+  const uint32_t* chunk_line;  // [n_chunks + 1]
+  const uint32_t* chunk_slot;  // [n_chunks + 1]
+  const uint32_t* slot_meta;   // [P] a | b << 8 | kind << 16 | line << 18 of each slot
+  int n_chunks, max_chunk_lines, max_chunk_slots;
 };
 
-// Largest mode count of the column-stage kernels and the slot mixes one LDS chunk holds.
+// Limits of one column-stage chunk: Raman lines, slots, and coupling coefficients
+// (slots x M x M) held in LDS; and the largest mode count.
 int colstage_max_modes();
+int colstage_lines_per_chunk();
 int colstage_slots_per_chunk();
+int colstage_coefficients_per_chunk();
 
 // Launchers (return hipSuccess or the launch error). Those templated on W run in the
 // work precision W (double, or float for the mixed-precision step); the others are FP64.

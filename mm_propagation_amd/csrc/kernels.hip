@@ -562,13 +562,19 @@ __global__ void __launch_bounds__(512) k_gather(Geometry g, const cpx<W>* __rest
 // column of each field and Raman line held in LDS between the steps:
 //   C1  inverse column DFT of the M fields -> U (HBM and LDS) -> pair products ->
 //       forward column DFT of the Raman lines -> conv
-//   C2  inverse column DFT of the Raman lines -> slot mixes X_s in LDS -> dense
-//       Kerr + Raman gather of the 64 samples -> forward column DFT of sigma -> yout
+//   C2  inverse column DFT of the Raman lines (kept in LDS) -> dense Kerr + Raman
+//       gather of the 64 samples, forming each slot mix X_s on the way -> forward
+//       column DFT of sigma -> yout
 // Lanes form 32 line slots of 8 lanes; a round transforms up to 32 lines.
 // ---------------------------------------------------------------------------
 constexpr int kColStageL = 64;       // column length the column-stage kernels take
 constexpr int kColStageMaxM = 12;    // modes kept in registers by the gather
-constexpr int kColStageXCap = 3072;  // complex LDS values for slot mixes / partial sums
+constexpr int kColStageXCap = 3072;  // complex LDS values for Raman lines / partial sums
+constexpr int kColStageQCap = 2048;  // coupling coefficients of one chunk's slots held in LDS
+constexpr int kColStageSCap = 128;   // slots of one chunk
+constexpr int kColStageLines = 40;   // Raman lines of one chunk in the gather kernel
+constexpr int kColStageRS = LineSlots<kColStageL, 1>::value;  // LDS row stride of a line (72)
+constexpr int kColStageRRows = 48;   // LDS rows for the lines of a chunk (lines rounded up to 8)
 
 struct ColStageCfg {
   static constexpr int L = kColStageL;
@@ -579,7 +585,9 @@ struct ColStageCfg {
 };
 
 int colstage_max_modes() { return kColStageMaxM; }
-int colstage_slots_per_chunk() { return kColStageXCap / kColStageL; }
+int colstage_lines_per_chunk() { return kColStageLines; }
+int colstage_coefficients_per_chunk() { return kColStageQCap; }
+int colstage_slots_per_chunk() { return kColStageSCap; }
 
 template <class W>
 __global__ void __launch_bounds__(256) k_colstage_fields_pairs(Geometry g, const cpx<W>* __restrict__ coltw,
@@ -596,19 +604,32 @@ __global__ void __launch_bounds__(256) k_colstage_fields_pairs(Geometry g, const
   const int job = tid / F::T, t = tid % F::T;
   const int n2 = xcd_block(blockIdx.x, gridDim.x);
   cw* fsm = sh + job * F::SLOTS;
+  cw w4[8], v[8];
+  // the first round's field samples are requested before the twiddles
+  {
+    const bool valid = job < g.M;
+    const size_t base = (size_t)(valid ? job : 0) * g.N + n2;
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = valid ? yin[base + (size_t)(t + i * F::T) * g.N2] : cw{0, 0};
+  }
   LineTw<L, W> lt;
   load_line_tw<L>(lt, coltw, t);
-  cw w4[8], v[8];
 #pragma unroll
   for (int i = 0; i < 8; i++) w4[i] = fourtw[n2 * (t + i * F::T)];
   const W sc = (W)scale;
+  const int wave_job0 = (tid / 64) * (64 / F::T);  // first line slot of this wavefront
   for (int r0 = 0; r0 < g.M; r0 += F::NL) {
     const int q = r0 + job;
     const bool valid = q < g.M;
     const size_t base = (size_t)(valid ? q : 0) * g.N + n2;
+    if (r0 + wave_job0 >= g.M) continue;  // no field of this wavefront in the round
+    if (r0 > 0) {
 #pragma unroll
-    for (int i = 0; i < 8; i++) v[i] = valid ? cmulc(yin[base + (size_t)(t + i * F::T) * g.N2], w4[i]) : cw{0, 0};
-    line_fft<L, 1, 1, 1>(v, fsm, lt, t);
+      for (int i = 0; i < 8; i++) v[i] = valid ? yin[base + (size_t)(t + i * F::T) * g.N2] : cw{0, 0};
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = cmulc(v[i], w4[i]);
+    line_fft<L, 1, 1, 1, W, true>(v, fsm, lt, t);
     if (valid) {
 #pragma unroll
       for (int i = 0; i < 8; i++) {
@@ -624,26 +645,21 @@ __global__ void __launch_bounds__(256) k_colstage_fields_pairs(Geometry g, const
     const int q = r0 + job;
     const bool valid = q < n_lines;
     const int qq = valid ? q : 0;
+    if (r0 + wave_job0 >= n_lines) continue;  // no line of this wavefront in the round
+    // cross line: U_a conj(U_b); self line: |U_a|^2 + i |U_b|^2 (b absent: 0)
     const uint32_t la = pl.line_a[qq], lb = pl.line_b[qq], self = pl.line_self[qq];
+    const bool has_b = lb != 0xffffffffu;
+    const int ia = (int)la * L, ib = (int)(has_b ? lb : la) * L;
+    const W bmask = has_b ? (W)1 : (W)0;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
       const int n1 = t + i * F::T;
-      cw x = {0, 0};
-      if (valid) {
-        const cw ua = Ush[la * L + n1];
-        if (self) {
-          x.re = ua.re * ua.re + ua.im * ua.im;
-          if (lb != 0xffffffffu) {
-            const cw ub = Ush[lb * L + n1];
-            x.im = ub.re * ub.re + ub.im * ub.im;
-          }
-        } else {
-          x = cmulc(ua, Ush[lb * L + n1]);
-        }
-      }
-      v[i] = x;
+      const cw ua = Ush[ia + n1], ub = Ush[ib + n1];
+      const cw cross = cmulc(ua, ub);
+      const cw selfv = {ua.re * ua.re + ua.im * ua.im, bmask * (ub.re * ub.re + ub.im * ub.im)};
+      v[i] = !valid ? cw{0, 0} : (self ? selfv : cross);
     }
-    line_fft<L, -1, 1, 1>(v, fsm, lt, t);
+    line_fft<L, -1, 1, 1, W, true>(v, fsm, lt, t);
     if (valid) {
       const size_t base = (size_t)q * g.N + n2;
 #pragma unroll
@@ -652,80 +668,128 @@ __global__ void __launch_bounds__(256) k_colstage_fields_pairs(Geometry g, const
   }
 }
 
-template <int M, class W>
-__global__ void __launch_bounds__(256) k_colstage_raman_gather(Geometry g, const cpx<W>* __restrict__ coltw,
-                                                               const cpx<W>* __restrict__ fourtw,
-                                                               const cpx<W>* __restrict__ U,
-                                                               const cpx<W>* __restrict__ conv,
-                                                               cpx<W>* __restrict__ yout, GatherArgs a, FusedArgs fa,
-                                                               const W* __restrict__ Qd, double scale) {
-  using F = ColStageCfg;
+// NW wavefronts (4, or 8 when M <= 6); each line transform runs inside one wavefront,
+// in place in its padded LDS row, and wavefronts without a line skip it.
+template <int M, int NW, class W>
+__global__ void __launch_bounds__(NW * 64) k_colstage_raman_gather(Geometry g, const cpx<W>* __restrict__ coltw,
+                                                                   const cpx<W>* __restrict__ fourtw,
+                                                                   const cpx<W>* __restrict__ U,
+                                                                   const cpx<W>* __restrict__ conv,
+                                                                   cpx<W>* __restrict__ yout, GatherArgs a,
+                                                                   FusedArgs fa, const W* __restrict__ Qd,
+                                                                   double scale) {
   using cw = cpx<W>;
-  constexpr int L = F::L;
-  static_assert(M >= 1 && M <= kColStageMaxM && 4 * M * L <= kColStageXCap, "modes out of range");
-  __shared__ cw sh[F::FFT + M * L + kColStageXCap];
-  cw* Ush = sh + F::FFT;  // [M][L] fields; sigma after the gather
-  cw* Xsh = Ush + M * L;  // [slots of a chunk][L]; partial sums after the gather
+  constexpr int L = kColStageL, T = L / 8, RS = kColStageRS;
+  constexpr int NT = NW * 64;
+  constexpr int JOBS = NT / T;  // line slots per round
+  constexpr int MM = M * M;
+  constexpr int UR = (M + 7) / 8 * 8;  // field rows, rounded up to whole wavefronts of lines
+  constexpr int UIN = (M * L + NT - 1) / NT;
+  constexpr int RCAP = kColStageRRows * RS;
+  static_assert(M >= 1 && M <= kColStageMaxM && NW * M * L <= RCAP, "modes out of range");
+  __shared__ cw Ush[UR * RS];   // field rows; sigma rows after the gather
+  __shared__ cw Rsh[RCAP];      // Raman line rows of the chunk; partial sums after the gather
+  __shared__ W Qsh[kColStageQCap];         // [slots of the chunk][M][M] coupling blocks
+  __shared__ uint32_t Msh[kColStageSCap];  // slot metadata of the chunk
   const int tid = threadIdx.x;
-  const int job = tid / F::T, t = tid % F::T;
-  const int n2 = xcd_block(blockIdx.x, gridDim.x);
-  cw* fsm = sh + job * F::SLOTS;
-  LineTw<L, W> lt;
-  load_line_tw<L>(lt, coltw, t);
-  cw w4[8], v[8];
-#pragma unroll
-  for (int i = 0; i < 8; i++) w4[i] = fourtw[n2 * (t + i * F::T)];
-#pragma unroll 4
-  for (int e = tid; e < M * L; e += 256) Ush[e] = U[(size_t)(e / L) * g.N + (size_t)(e % L) * g.N2 + n2];
-  const W sc = (W)scale, kv = (W)a.kv, kr = (W)a.kr;
-  // gather lanes: sample `lane`, slots s0 + wave, s0 + wave + 4, ...
+  const int job = tid / T, t = tid % T;
   const int lane = tid % 64;
   const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
+  const int n2 = xcd_block(blockIdx.x, gridDim.x);
+  // the first chunk's lines, this lane's field samples, coupling blocks and slot metadata are
+  // all requested before anything waits on memory
+  cw v[8], uin[UIN];
+  int l0 = 0, l1 = (int)fa.chunk_line[1], s0 = 0, s1 = (int)fa.chunk_slot[1];
+  {
+    const bool valid = job < l1;
+    const size_t base = (size_t)(valid ? job : 0) * g.N + n2;
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = valid ? conv[base + (size_t)(t + i * T) * g.N2] : cw{0, 0};
+#pragma unroll
+    for (int r = 0; r < UIN; r++) {
+      const int e = tid + r * NT;
+      uin[r] = e < M * L ? U[(size_t)(e / L) * g.N + (size_t)(e % L) * g.N2 + n2] : cw{0, 0};
+    }
+  }
+  constexpr int QIN = kColStageQCap / NT;
+  W qin[QIN];
+  uint32_t min_ = 0;
+#pragma unroll
+  for (int r = 0; r < QIN; r++) {
+    const int e = tid + r * NT;
+    qin[r] = e < (s1 - s0) * MM ? Qd[(size_t)s0 * MM + e] : (W)0;
+  }
+  if (tid < s1 - s0) min_ = fa.slot_meta[s0 + tid];
+  LineTw<L, W> lt;
+  load_line_tw<L>(lt, coltw, t);
+  cw w4[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) w4[i] = fourtw[n2 * (t + i * T)];
+#pragma unroll
+  for (int r = 0; r < UIN; r++) {
+    const int e = tid + r * NT;
+    if (e < M * L) Ush[(e / L) * RS + e % L] = uin[r];
+  }
+  const W sc = (W)scale, kv = (W)a.kv, kr = (W)a.kr;
   cw u[M], acc[M];
 #pragma unroll
   for (int p = 0; p < M; p++) acc[p] = {0, 0};
   for (int c = 0; c < fa.n_chunks; c++) {
-    const int l0 = c * fa.chunk_lines;
-    const int l1 = min(fa.n_lines, l0 + fa.chunk_lines);
-    const int s0 = (int)fa.chunk_slot[c], s1 = (int)fa.chunk_slot[c + 1];
-    __syncthreads();  // Ush loaded / Xsh of the previous chunk consumed
-    for (int r0 = l0; r0 < l1; r0 += F::NL) {
+    if (c > 0) {
+      __syncthreads();  // Rsh, Qsh and Msh of the previous chunk consumed
+      l0 = l1;
+      l1 = (int)fa.chunk_line[c + 1];
+      s0 = s1;
+      s1 = (int)fa.chunk_slot[c + 1];
+#pragma unroll
+      for (int r = 0; r < QIN; r++) {
+        const int e = tid + r * NT;
+        qin[r] = e < (s1 - s0) * MM ? Qd[(size_t)s0 * MM + e] : (W)0;
+      }
+      if (tid < s1 - s0) min_ = fa.slot_meta[s0 + tid];
+    }
+#pragma unroll
+    for (int r = 0; r < QIN; r++) Qsh[tid + r * NT] = qin[r];
+    if (tid < s1 - s0) Msh[tid] = min_;
+    for (int r0 = l0; r0 < l1; r0 += JOBS) {
       const int q = r0 + job;
       const bool valid = q < l1;
-      const size_t base = (size_t)(valid ? q : 0) * g.N + n2;
+      if (r0 + wave * (64 / T) >= l1) continue;  // no line of this wavefront in the round
+      if (c > 0 || r0 > l0) {
+        const size_t base = (size_t)(valid ? q : 0) * g.N + n2;
 #pragma unroll
-      for (int i = 0; i < 8; i++)
-        v[i] = valid ? cmulc(conv[base + (size_t)(t + i * F::T) * g.N2], w4[i]) : cw{0, 0};
-      line_fft<L, 1, 1, 1>(v, fsm, lt, t);
+        for (int i = 0; i < 8; i++) v[i] = valid ? conv[base + (size_t)(t + i * T) * g.N2] : cw{0, 0};
+      }
+#pragma unroll
+      for (int i = 0; i < 8; i++) v[i] = cmulc(v[i], w4[i]);
+      cw* row = Rsh + (q - l0) * RS;
+      line_fft<L, 1, 1, 1, W, true>(v, row, lt, t);
       if (valid) {
-        const int sa = (int)fa.line_slot0[q], sb = (int)fa.line_slot0[q + 1];
-        for (int sl = sa; sl < sb; sl++) {
-          const uint32_t pa = a.slot_a[sl], pb = a.slot_b[sl], kind = a.slot_kind[sl];
 #pragma unroll
-          for (int i = 0; i < 8; i++) {
-            const int n1 = t + i * F::T;
-            const cw r = cscale(v[i], sc);
-            cw vr;
-            switch (kind) {
-              case kDirect: vr = r; break;
-              case kConj: vr = cconj(r); break;
-              case kRe: vr = {r.re, 0}; break;
-              default: vr = {r.im, 0}; break;
-            }
-            const cw V = cmulc(Ush[pa * L + n1], Ush[pb * L + n1]);
-            Xsh[(sl - s0) * L + n1] = cadd(cscale(V, kv), cscale(vr, kr));
-          }
-        }
+        for (int i = 0; i < 8; i++) row[t + i * T] = cscale(v[i], sc);
       }
     }
     __syncthreads();
     if (c == 0) {
 #pragma unroll
-      for (int k = 0; k < M; k++) u[k] = Ush[k * L + lane];
+      for (int k = 0; k < M; k++) u[k] = Ush[k * RS + lane];
     }
-    for (int sl = s0 + wave; sl < s1; sl += 4) {
-      const cw X = Xsh[(sl - s0) * L + lane];
-      const W* qb = Qd + (size_t)sl * M * M;
+    // slot s mixes kv U_a conj(U_b) with its Raman line read as is, conjugated, or by real or
+    // imaginary part; everything but the sample is wave-uniform and read from LDS by broadcast
+    for (int sl = s0 + wave; sl < s1; sl += NW) {
+      const uint32_t meta = Msh[sl - s0];
+      const int pa = meta & 255u, pb = (meta >> 8) & 255u, kind = (meta >> 16) & 3u, line = (int)(meta >> 18);
+      const W* qb = Qsh + (sl - s0) * MM;
+      const cw r = Rsh[(line - l0) * RS + lane];
+      const cw V = cmulc(Ush[pa * RS + lane], Ush[pb * RS + lane]);
+      cw vr;
+      switch (kind) {
+        case kDirect: vr = r; break;
+        case kConj: vr = cconj(r); break;
+        case kRe: vr = {r.re, 0}; break;
+        default: vr = {r.im, 0}; break;
+      }
+      const cw X = cadd(cscale(V, kv), cscale(vr, kr));
       cw y[M];
 #pragma unroll
       for (int k = 0; k < M; k++) y[k] = cmul(X, u[k]);
@@ -738,28 +802,29 @@ __global__ void __launch_bounds__(256) k_colstage_raman_gather(Geometry g, const
       }
     }
   }
-  __syncthreads();  // every lane done with Ush and Xsh
+  __syncthreads();  // every lane done with Ush and Rsh
 #pragma unroll
-  for (int p = 0; p < M; p++) Xsh[(wave * M + p) * L + lane] = acc[p];
+  for (int p = 0; p < M; p++) Rsh[(wave * M + p) * L + lane] = acc[p];
   __syncthreads();
-  for (int e = tid; e < M * L; e += 256) {
+  for (int e = tid; e < M * L; e += NT) {
     const int p = e / L, j = e % L;
-    cw tsum = Xsh[p * L + j];
+    cw tsum = Rsh[p * L + j];
 #pragma unroll
-    for (int w = 1; w < 4; w++) tsum = cadd(tsum, Xsh[(w * M + p) * L + j]);
-    Ush[e] = tsum;
+    for (int w = 1; w < NW; w++) tsum = cadd(tsum, Rsh[(w * M + p) * L + j]);
+    Ush[p * RS + j] = tsum;
   }
   __syncthreads();
-  for (int r0 = 0; r0 < M; r0 += F::NL) {
-    const int q = r0 + job;
+  if (wave * (64 / T) < M) {
+    const int q = job;
     const bool valid = q < M;
+    cw* row = Ush + q * RS;
 #pragma unroll
-    for (int i = 0; i < 8; i++) v[i] = valid ? Ush[q * L + t + i * F::T] : cw{0, 0};
-    line_fft<L, -1, 1, 1>(v, fsm, lt, t);
+    for (int i = 0; i < 8; i++) v[i] = valid ? row[t + i * T] : cw{0, 0};
+    line_fft<L, -1, 1, 1, W, true>(v, row, lt, t);
     if (valid) {
       const size_t base = (size_t)q * g.N + n2;
 #pragma unroll
-      for (int i = 0; i < 8; i++) yout[base + (size_t)(t + i * F::T) * g.N2] = cmul(v[i], w4[i]);
+      for (int i = 0; i < 8; i++) yout[base + (size_t)(t + i * T) * g.N2] = cmul(v[i], w4[i]);
     }
   }
 }
@@ -1035,10 +1100,14 @@ hipError_t launch_colstage_fields_pairs(const Geometry& g, const Twiddles& tw, c
   return hipGetLastError();
 }
 
-#define MMP_COLSTAGE_CASE(MV)                                                                                     \
-  case MV:                                                                                                         \
-    hipLaunchKernelGGL((k_colstage_raman_gather<MV, W>), dim3(g.N2), dim3(256), 0, s, g, TwOf<W>::col(tw),         \
-                       TwOf<W>::four(tw), U, conv, yout, ga, fa, Qd, scale);                                      \
+#define MMP_COLSTAGE_CASE(MV)                                                                                    \
+  case MV:                                                                                                        \
+    if constexpr (MV <= 6)                                                                                        \
+      hipLaunchKernelGGL((k_colstage_raman_gather<MV, 8, W>), dim3(g.N2), dim3(512), 0, s, g, TwOf<W>::col(tw),  \
+                         TwOf<W>::four(tw), U, conv, yout, ga, fa, Qd, scale);                                    \
+    else                                                                                                          \
+      hipLaunchKernelGGL((k_colstage_raman_gather<MV, 4, W>), dim3(g.N2), dim3(256), 0, s, g, TwOf<W>::col(tw),  \
+                         TwOf<W>::four(tw), U, conv, yout, ga, fa, Qd, scale);                                    \
     break;
 
 template <class W>
@@ -1046,7 +1115,9 @@ hipError_t launch_colstage_raman_gather(const Geometry& g, const Twiddles& tw, c
                                         cpx<W>* yout, const GatherArgs& ga, const FusedArgs& fa, const W* Qd,
                                         double scale, hipStream_t s) {
   if (g.N1 != kColStageL) return hipErrorInvalidValue;
-  if (fa.n_chunks < 1 || fa.chunk_lines < 1 || 2 * fa.chunk_lines > colstage_slots_per_chunk())
+  if (fa.n_chunks < 1 || fa.max_chunk_lines < 1 || fa.max_chunk_lines > colstage_lines_per_chunk() ||
+      fa.max_chunk_slots < 1 || fa.max_chunk_slots > colstage_slots_per_chunk() ||
+      fa.max_chunk_slots * g.M * g.M > colstage_coefficients_per_chunk())
     return hipErrorInvalidValue;
   switch (g.M) {
     MMP_COLSTAGE_CASE(1) MMP_COLSTAGE_CASE(2) MMP_COLSTAGE_CASE(3) MMP_COLSTAGE_CASE(4) MMP_COLSTAGE_CASE(5)

@@ -389,18 +389,44 @@ mmp_status prepare_problem(const mmp_params* in, const mmp_options& opt, Problem
   P.kv = P.raman ? 1 - P.fR : 1.;
   P.kr = P.raman ? P.fR * P.T0 : 0.;
   // Column-stage kernels (Raman on, N1 = 64, dense gather, M <= 12): Raman lines are
-  // processed in chunks of fused_lines (at most two slots each); chunk c covers slots
-  // [fused_slot[c], fused_slot[c+1]).
-  P.fused_lines = 0;
+  // processed in chunks that respect the LDS limits on lines, slots and coupling blocks;
+  // chunk c covers lines [fused_line[c], fused_line[c+1]) and slots [fused_slot[c], fused_slot[c+1]).
   P.fused_chunks = 0;
+  P.fused_max_lines = 0;
+  P.fused_max_slots = 0;
+  P.fused_line.clear();
+  P.fused_slot.clear();
   const int nl = (int)P.line_a.size();
   if (P.raman && P.N1 == 64 && (int)M <= colstage_max_modes() && dense_gather_chunk((int)M) > 0 && nl > 0 &&
-      P.line_slot0.size() == (size_t)nl + 1 && P.line_slot0[nl] == P.P) {
-    const int per_chunk = colstage_slots_per_chunk() / 2;
-    P.fused_lines = per_chunk < nl ? per_chunk : nl;
-    P.fused_chunks = (nl + P.fused_lines - 1) / P.fused_lines;
-    P.fused_slot.assign(P.fused_chunks + 1, P.P);
-    for (int c = 0; c < P.fused_chunks; c++) P.fused_slot[c] = P.line_slot0[c * P.fused_lines];
+      P.line_slot0.size() == (size_t)nl + 1 && P.line_slot0[0] == 0 && P.line_slot0[nl] == P.P) {
+    const int max_lines = colstage_lines_per_chunk();
+    const int max_slots = std::min(colstage_slots_per_chunk(), colstage_coefficients_per_chunk() / (int)(M * M));
+    bool ok = max_slots >= 2;
+    P.fused_line.push_back(0);
+    P.fused_slot.push_back(0);
+    int l = 0;
+    while (ok && l < nl) {
+      int lines = 0, slots = 0;
+      while (l < nl && lines < max_lines) {
+        const int ns = (int)(P.line_slot0[l + 1] - P.line_slot0[l]);
+        if (slots + ns > max_slots) break;
+        slots += ns;
+        lines++;
+        l++;
+      }
+      if (lines == 0) ok = false;
+      P.fused_line.push_back((uint32_t)l);
+      P.fused_slot.push_back(P.line_slot0[l]);
+      P.fused_max_lines = std::max(P.fused_max_lines, lines);
+      P.fused_max_slots = std::max(P.fused_max_slots, slots);
+    }
+    if (ok) {
+      P.fused_chunks = (int)P.fused_line.size() - 1;
+    } else {
+      P.fused_line.clear();
+      P.fused_slot.clear();
+      P.fused_max_lines = P.fused_max_slots = 0;
+    }
   }
   // a stage result N_j needs a stash slot when a stage i > j + 1 reads it
   P.stash_needed.assign(P.rk.s, 0);

@@ -51,8 +51,10 @@ struct Problem {
   // Raman convolution lines
   std::vector<uint32_t> line_a, line_b, line_self;  // [n_lines]
   std::vector<uint32_t> line_slot0;                 // [n_lines + 1] first device slot of each line
-  // column-stage kernels: Raman lines per LDS chunk and the chunks' slot bounds (0 chunks: not eligible)
-  int fused_lines = 0, fused_chunks = 0;
+  // column-stage kernels: chunks of Raman lines (0 chunks: not eligible), their line and slot
+  // bounds, and the largest chunk's line and slot counts
+  int fused_chunks = 0, fused_max_lines = 0, fused_max_slots = 0;
+  std::vector<uint32_t> fused_line;                 // [fused_chunks + 1]
   std::vector<uint32_t> fused_slot;                 // [fused_chunks + 1]
   // Kerr gather: nonzeros of each Q^(p) sorted by (slot, k), with per-chunk ranges
   std::vector<uint32_t> off, k, slot;  // [M+1], [nnz], [nnz]

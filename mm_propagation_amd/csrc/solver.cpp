@@ -350,8 +350,8 @@ size_t planned_device_bytes(const Problem& P, const mmp_options& opt) {
     b += 8 * (2 * MN) + 8 * (P.N2 + P.N1 + P.N) + 4 * c(P.nnz);
     if (P.raman) b += 8 * (size_t)P.N;
   }
-  if (P.fused_chunks > 0 && (opt.flags & MMP_FLAG_FUSED_COLUMNS))
-    b += 4 * (P.line_slot0.size() + P.fused_slot.size());
+  if (P.fused_chunks > 0 && !(opt.flags & MMP_FLAG_SEPARATE_COLUMNS))
+    b += 4 * (P.fused_line.size() + P.fused_slot.size() + c(P.P));
   if (dense_gather_chunk((int)P.M) > 0) b += (mixed ? 4 : 8) * (size_t)P.P * P.M * P.M;
   return b;
 }
@@ -535,14 +535,19 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
       s->qdense_w = s->qdense;
     }
   }
-  if (P.fused_chunks > 0 && (opt.flags & MMP_FLAG_FUSED_COLUMNS)) {
-    uint32_t *d_ls0, *d_cs;
-    A(dalloc(s, &d_ls0, P.line_slot0.size()));
+  std::vector<uint32_t> slot_meta(P.P);
+  if (P.fused_chunks > 0 && !(opt.flags & MMP_FLAG_SEPARATE_COLUMNS)) {
+    uint32_t *d_cl, *d_cs, *d_meta;
+    A(dalloc(s, &d_cl, P.fused_line.size()));
     A(dalloc(s, &d_cs, P.fused_slot.size()));
+    A(dalloc(s, &d_meta, P1));
     if (st != MMP_OK) return cleanup(st);
-    A(upload(s, d_ls0, P.line_slot0.data(), P.line_slot0.size() * 4));
+    for (uint32_t j = 0; j < P.P; j++)
+      slot_meta[j] = P.slot_a[j] | (P.slot_b[j] << 8) | (P.slot_kind[j] << 16) | (P.slot_line[j] << 18);
+    A(upload(s, d_cl, P.fused_line.data(), P.fused_line.size() * 4));
     A(upload(s, d_cs, P.fused_slot.data(), P.fused_slot.size() * 4));
-    s->fused = FusedArgs{d_ls0, d_cs, s->n_lines, P.fused_lines, P.fused_chunks};
+    A(upload(s, d_meta, slot_meta.data(), P.P * 4));
+    s->fused = FusedArgs{d_cl, d_cs, d_meta, P.fused_chunks, P.fused_max_lines, P.fused_max_slots};
     s->use_fused = true;
   }
   // stage heads

@@ -82,9 +82,9 @@ def test_mixed_graph_replays_equal_eager():
 
 
 @pytest.mark.parametrize("M,nt", [(6, 16384), (10, 8192)])
-def test_mixed_column_stage_kernels(M, nt):
+def test_mixed_separate_column_passes(M, nt):
     require_device()
     sp = configs.grin_fiber(M=M, nt=nt, symmetric=True)
-    err, _, _ = run_mixed(sp, 50, decode=1, flags=_capi.MMP_FLAG_FUSED_COLUMNS)
+    err, _, _ = run_mixed(sp, 50, decode=1, flags=_capi.MMP_FLAG_SEPARATE_COLUMNS)
     print("rel-L2", err)
     assert np.all(err <= MIXED_SHORT), err

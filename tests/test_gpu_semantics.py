@@ -114,7 +114,7 @@ def test_invalid_inputs_rejected_on_device_path(mutate, status):
     assert ei.value.status == status
 
 
-@pytest.mark.parametrize("precision,flags", [(0, 0), (1, 0), (0, _capi.MMP_FLAG_FUSED_COLUMNS)])
+@pytest.mark.parametrize("precision,flags", [(0, 0), (1, 0), (0, _capi.MMP_FLAG_SEPARATE_COLUMNS)])
 def test_planned_bytes_equal_allocated_bytes(precision, flags):
     """mmp_plan's device_bytes (host only) is what a solver with the same options allocates."""
     require_device()
