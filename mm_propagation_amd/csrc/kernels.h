@@ -81,6 +81,9 @@ struct HeadArgs {
   int stash_prev;      // 1 when N_{i-1} is read by a later stage
   int final_stage;     // 1 when i == s: psi_hat = acc, no row-inverse output
   int update_prev;     // 1 when this head is the last reader of e_prev in the step
+  int chain_next;      // with final_stage: also build stage 0 of the next step (E_0 in e_cur, no a terms)
+  StepState* st;       // with final_stage: z += h, steps += 1 (NULL: left alone)
+  double h;            // the step h
   int n_a;             // number of nonzero a_ij, j < i, in increasing j
   int a_idx[kMaxStages];   // j (N_{i-1} comes from registers, earlier N_j from stash)
   double a_h[kMaxStages];  // a_ij h

@@ -92,7 +92,11 @@ __global__ void __launch_bounds__(64) k_row_head(Geometry g, const cpx<W>* __res
     e1[i] = (valid && a.update_prev) ? a.e1[idx] : zero;
     psi[i] = (valid && (a.prev_is_first || !a.final_stage)) ? a.psi_hat[idx] : zero;
     accin[i] = (valid && a.has_prev && !a.prev_is_first) ? a.acc[idx] : zero;
-    ec[i] = (valid && !a.final_stage) ? a.e_cur[idx] : zero;
+    ec[i] = (valid && (!a.final_stage || a.chain_next)) ? a.e_cur[idx] : zero;
+  }
+  if (a.final_stage && a.st && blockIdx.x == 0 && threadIdx.x == 0) {
+    a.st->z += a.h;  // z += h (repeated FP64 addition)
+    a.st->steps += 1;
   }
   cd nh[8];
   if (a.has_prev) {
@@ -116,11 +120,12 @@ __global__ void __launch_bounds__(64) k_row_head(Geometry g, const cpx<W>* __res
       }
     }
   }
-  if (a.final_stage) return;
+  if (a.final_stage && !a.chain_next) return;
+  // stage i's product, or (chain_next) stage 0 of the next step from the new psi_hat
 #pragma unroll
   for (int i = 0; i < 8; i++) {
     const size_t idx = base + i * C::T;
-    cd x = psi[i];
+    cd x = a.final_stage ? accin[i] : psi[i];
 #pragma unroll
     for (int e = 0; e < kMaxStages; e++) {
       if (e >= a.n_a) break;

@@ -87,6 +87,7 @@ struct mmp_solver {
   PairLineArgs pairs{};
   FusedArgs fused{};
   bool use_fused = false;
+  bool chain = false;        // the step close also issues the next step's stage-0 head
   double* qdense = nullptr;  // [P][M][M] dense coupling blocks for the register gather (M <= 12)
   // work arrays of the step in the work precision (the FP64 buffers above in FP64 mode)
   bool mixed = false;
@@ -188,6 +189,41 @@ mmp_status run_kernel(mmp_solver* s, const char* name, double bytes, F&& fn) {
   return MMP_OK;
 }
 
+// Row stage head i (i == s closes the step; with s->chain it also builds the next step's
+// stage 0 and advances z).
+template <class W>
+mmp_status enqueue_head_w(mmp_solver* s, int i) {
+  const Problem& P = s->P;
+  const HeadArgs& h = s->heads[i];
+  const double cb = sizeof(cpx<W>), cs = 16.0;
+  const double MN = (double)P.M * P.N;
+  // algorithmic bytes: every array the head reads or writes, once
+  int state_arrays = 0, work_arrays = 0;
+  if (h.has_prev) {
+    work_arrays += 1;                                  // sigma
+    state_arrays += 2;                                 // E_{i-1}, acc or psi_hat written
+    if (!h.prev_is_first) state_arrays += 1;           // acc read
+    if (h.update_prev) state_arrays += 2;              // E1 read, E_{i-1} written
+    if (h.stash_prev) state_arrays += 1;               // N_{i-1} stashed
+  }
+  if (h.prev_is_first || !h.final_stage) state_arrays += 1;  // psi_hat read
+  if (!h.final_stage || h.chain_next) {
+    state_arrays += 1;                                 // E_i (E_0 of the next step when chained)
+    work_arrays += 1;                                  // row-inverse output
+    for (int e = 0; e < h.n_a; e++)
+      if (h.a_idx[e] != h.prev_stage) state_arrays += 1;  // stashed N_j
+  }
+  const double hb = MN * (cs * state_arrays + cb * work_arrays);
+  return run_kernel(s, i == P.rk.s ? "row_close_step" : "row_stage_head", hb,
+                    [&] { return launch_row_head<W>(s->g, s->tw, h, s->ctx->stream); });
+}
+
+mmp_status enqueue_head(mmp_solver* s, int i) {
+  return s->mixed ? enqueue_head_w<float>(s, i) : enqueue_head_w<double>(s, i);
+}
+
+// One step. With s->chain the step's stage-0 head was issued by the previous step's close
+// (or by advance_impl for the first step of a call).
 template <class W>
 mmp_status enqueue_step_w(mmp_solver* s) {
   using cw = cpx<W>;
@@ -205,26 +241,7 @@ mmp_status enqueue_step_w(mmp_solver* s) {
   const double LN = (double)s->n_lines * P.N;
   const double invN = 1.0 / P.N;
   for (int i = 0; i <= P.rk.s; i++) {
-    const HeadArgs& h = s->heads[i];
-    // algorithmic bytes: every array the head reads or writes, once
-    int state_arrays = 0, work_arrays = 0;
-    if (h.has_prev) {
-      work_arrays += 1;                                  // sigma
-      state_arrays += 2;                                 // E_{i-1}, acc or psi_hat written
-      if (!h.prev_is_first) state_arrays += 1;           // acc read
-      if (h.update_prev) state_arrays += 2;              // E1 read, E_{i-1} written
-      if (h.stash_prev) state_arrays += 1;               // N_{i-1} stashed
-    }
-    if (h.prev_is_first || !h.final_stage) state_arrays += 1;  // psi_hat read
-    if (!h.final_stage) {
-      state_arrays += 1;                                 // E_i
-      work_arrays += 1;                                  // row-inverse output
-      for (int e = 0; e < h.n_a; e++)
-        if (h.a_idx[e] != h.prev_stage) state_arrays += 1;  // stashed N_j
-    }
-    const double hb = MN * (cs * state_arrays + cb * work_arrays);
-    MMP_TRY(run_kernel(s, i == P.rk.s ? "row_close_step" : "row_stage_head", hb,
-                       [&] { return launch_row_head<W>(g, s->tw, h, st); }));
+    if (i > 0 || !s->chain) MMP_TRY(enqueue_head_w<W>(s, i));
     if (i == P.rk.s) break;
     if (s->use_fused) {
       MMP_TRY(run_kernel(s, "col_fields_pairs", cb * (2 * MN + LN), [&] {
@@ -256,7 +273,6 @@ mmp_status enqueue_step_w(mmp_solver* s) {
     MMP_TRY(run_kernel(s, "col_forward_sigma", 2 * cb * MN,
                        [&] { return launch_col_forward<W>(g, s->tw, ybuf, ybuf, P.M, st); }));
   }
-  MMP_TRY(run_kernel(s, "advance_z", 0.0, [&] { return launch_advance_z(s->st, P.delta_z, st); }));
   return MMP_OK;
 }
 
@@ -552,6 +568,9 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
   }
   // stage heads
   const double h = P.delta_z;
+  // the close head also builds the next step's stage 0 unless the stage-0 Lawson table is
+  // advanced by that same head (no tableau here has c_i = 0 beyond stage 0)
+  s->chain = P.table_last_head[P.stage_table[0]] < P.rk.s;
   for (int i = 0; i <= P.rk.s; i++) {
     HeadArgs a{};
     a.ysig = s->ybuf_w;
@@ -572,6 +591,14 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
       a.update_prev = (P.table_last_head[u] == i);
       a.bh_prev = (P.rk.b[i - 1] != 0.) ? P.rk.b[i - 1] * h : 0.;
       a.stash_prev = P.stash_needed[i - 1];
+    }
+    if (i == P.rk.s) {
+      a.st = s->st;
+      a.h = P.delta_z;
+      if (s->chain) {
+        a.chain_next = 1;
+        a.e_cur = s->etab + (size_t)P.stage_table[0] * MN;
+      }
     }
     if (i < P.rk.s) {
       a.e_cur = s->etab + (size_t)P.stage_table[i] * MN;
@@ -610,6 +637,7 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
 
 mmp_status advance_impl(mmp_solver* s, uint32_t n) {
   MMP_TRY(set_device(s));
+  if (n > 0 && s->chain) MMP_TRY(enqueue_head(s, 0));  // stage 0 of the first step
   uint32_t left = n;
   if (s->graphs && left >= (uint32_t)kGraphSteps) {
     if (!s->graph_exec) {
