@@ -29,6 +29,20 @@ sys.path.insert(0, REPO)
 
 METRIC = "M·NT·n_steps/s at stated rel-L2 vs FP64 oracle; p50/p99 solve latency"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "hbm_traffic_grin6_fp64.md")
+
+
+def measured_traffic():
+    """Bytes per launch of each kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of
+    this workload (FP64 default path), or {} when it is absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return json.loads(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return {}
 
 
 def workload():
@@ -120,8 +134,11 @@ def main() -> int:
     dom = max(kt, key=lambda k: k["total_ms"])
     avg_s = dom["total_ms"] / 1e3 / max(dom["launches"], 1)
     achieved = dom["bytes_per_launch"] / avg_s / 1e9
+    traffic = measured_traffic().get(dom["name"]) if not (args.mixed or args.separate_columns) else None
     roofline = {"bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "traffic_source": "profiles/hbm_traffic_grin6_fp64.md (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, "
+                                  "copy-calibrated)" if traffic else None,
                 "avg_launch_us": round(avg_s * 1e6, 3), "bytes_per_launch": dom["bytes_per_launch"],
                 "share_of_kernel_time": round(dom["total_ms"] / total_ms, 4),
                 "kernels": {k["name"]: {"avg_us": round(k["total_ms"] * 1e3 / max(k["launches"], 1), 3),
