@@ -18,6 +18,13 @@ namespace mmp {
 
 constexpr int kMaxStages = 11;
 
+// Events the next launches record at their dispatch's start and end (both null: none).
+struct LaunchEvents {
+  hipEvent_t start = nullptr;
+  hipEvent_t stop = nullptr;
+};
+extern thread_local LaunchEvents g_launch_events;
+
 // Conv-line kinds: how a pair slot reads its Raman-convolved product.
 enum SlotKind : int { kDirect = 0, kConj = 1, kRe = 2, kIm = 3 };
 
@@ -171,7 +178,6 @@ hipError_t launch_gather_dense(const Geometry& g, const cpx<W>* U, const cpx<W>*
                                const GatherArgs& a, const W* Qd, hipStream_t s);
 // out[j] = in[j] rounded to FP32, j < n
 hipError_t launch_narrow(const cd* in, cf* out, size_t n, hipStream_t s);
-hipError_t launch_advance_z(StepState* st, double h, hipStream_t s);
 hipError_t launch_energy(const Geometry& g, const cd* fields, double* out, double dt, hipStream_t s);
 
 }  // namespace mmp

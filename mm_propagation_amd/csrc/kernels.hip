@@ -15,7 +15,17 @@
 // differ from the reference by rounding only).
 #include "kernels.h"
 
+#include <hip/hip_ext.h>
+
 namespace mmp {
+
+thread_local LaunchEvents g_launch_events;
+
+// Every launch goes through hipExtLaunchKernelGGL so that, when g_launch_events is set, the
+// dispatch itself records its start and stop (no separate event packets around the kernel).
+#define MMP_LAUNCH(KERNEL, GRID, BLOCK, SHMEM, STREAM, ...)                                               \
+  hipExtLaunchKernelGGL(KERNEL, GRID, BLOCK, SHMEM, STREAM, g_launch_events.start, g_launch_events.stop, 0, \
+                        __VA_ARGS__)
 
 // ---------------------------------------------------------------------------
 // configuration of the row and column passes
@@ -904,13 +914,6 @@ __global__ void __launch_bounds__(256) k_gather_dense(Geometry g, const cpx<W>* 
   }
 }
 
-__global__ void k_advance_z(StepState* st, double h) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    st->z += h;
-    st->steps += 1;
-  }
-}
-
 // per-mode energy sum_j |A_p,j|^2 * dt; one workgroup per mode, fixed reduction order
 __global__ void __launch_bounds__(256) k_energy(Geometry g, const cd* __restrict__ f, double* __restrict__ out,
                                                 double dt) {
@@ -960,7 +963,7 @@ static hipError_t row_head_impl(const Geometry& g, const Twiddles& tw, const Hea
   using C = RowCfg<L>;
   const int rows = g.M * g.N1;
   dim3 grid((rows + C::G - 1) / C::G);
-  hipLaunchKernelGGL((k_row_head<L, W>), grid, dim3(C::THREADS), 0, s, g, TwOf<W>::row(tw), a);
+  MMP_LAUNCH((k_row_head<L, W>), grid, dim3(C::THREADS), 0, s, g, TwOf<W>::row(tw), a);
   return hipGetLastError();
 }
 template <class W>
@@ -989,11 +992,11 @@ static dim3 col_grid(const Geometry& g, int lines) {
 
 #define MMP_COL_LAUNCH(KERNEL, LINES, ...)                                                                      \
   switch (pick_cb<L>(g, LINES)) {                                                                               \
-    case 8: hipLaunchKernelGGL((KERNEL<L, 8, W>), (col_grid<L, 8>(g, LINES)), (dim3(ColCfg<L, 8>::THREADS)), 0, s, \
+    case 8: MMP_LAUNCH((KERNEL<L, 8, W>), (col_grid<L, 8>(g, LINES)), (dim3(ColCfg<L, 8>::THREADS)), 0, s, \
                                __VA_ARGS__); break;                                                            \
-    case 4: hipLaunchKernelGGL((KERNEL<L, 4, W>), (col_grid<L, 4>(g, LINES)), (dim3(ColCfg<L, 4>::THREADS)), 0, s, \
+    case 4: MMP_LAUNCH((KERNEL<L, 4, W>), (col_grid<L, 4>(g, LINES)), (dim3(ColCfg<L, 4>::THREADS)), 0, s, \
                                __VA_ARGS__); break;                                                            \
-    default: hipLaunchKernelGGL((KERNEL<L, 2, W>), (col_grid<L, 2>(g, LINES)), (dim3(ColCfg<L, 2>::THREADS)), 0,  \
+    default: MMP_LAUNCH((KERNEL<L, 2, W>), (col_grid<L, 2>(g, LINES)), (dim3(ColCfg<L, 2>::THREADS)), 0,  \
                                 s, __VA_ARGS__); break;                                                        \
   }                                                                                                             \
   return hipGetLastError();
@@ -1045,7 +1048,7 @@ template <int L>
 static hipError_t row_forward_impl(const Geometry& g, const Twiddles& tw, cd* data, int lines, hipStream_t s) {
   using C = RowCfg<L>;
   const int rows = lines * g.N1;
-  hipLaunchKernelGGL(k_row_forward<L>, dim3((rows + C::G - 1) / C::G), dim3(C::THREADS), 0, s, g, tw.rowtw, data,
+  MMP_LAUNCH(k_row_forward<L>, dim3((rows + C::G - 1) / C::G), dim3(C::THREADS), 0, s, g, tw.rowtw, data,
                      lines);
   return hipGetLastError();
 }
@@ -1058,7 +1061,7 @@ static hipError_t row_inverse_phase_impl(const Geometry& g, const Twiddles& tw, 
                                          const double* imL, const StepState* st, int use_phase, hipStream_t s) {
   using C = RowCfg<L>;
   const int rows = g.M * g.N1;
-  hipLaunchKernelGGL(k_row_inverse_phase<L>, dim3((rows + C::G - 1) / C::G), dim3(C::THREADS), 0, s, g, tw.rowtw,
+  MMP_LAUNCH(k_row_inverse_phase<L>, dim3((rows + C::G - 1) / C::G), dim3(C::THREADS), 0, s, g, tw.rowtw,
                      in, out, imL, st, use_phase);
   return hipGetLastError();
 }
@@ -1072,7 +1075,7 @@ static hipError_t row_raman_impl(const Geometry& g, const Twiddles& tw, cpx<W>* 
                                  double hscale, hipStream_t s) {
   using C = RowCfg<L>;
   const int rows = lines * g.N1;
-  hipLaunchKernelGGL((k_row_raman<L, W>), dim3((rows + C::G - 1) / C::G), dim3(C::THREADS), 0, s, g,
+  MMP_LAUNCH((k_row_raman<L, W>), dim3((rows + C::G - 1) / C::G), dim3(C::THREADS), 0, s, g,
                      TwOf<W>::row(tw), data, lines, hspec, hscale);
   return hipGetLastError();
 }
@@ -1089,8 +1092,8 @@ hipError_t launch_gather(const Geometry& g, const cpx<W>* U, const cpx<W>* VR, c
   int waves = g.M < 8 ? g.M : 8;
   if ((g.M + waves - 1) / waves > kGatherMaxPW) return hipErrorInvalidValue;
   dim3 grid(g.N / kGatherTS), block(kGatherTS * waves);
-  if (a.cap == 4096) hipLaunchKernelGGL((k_gather<4096, W>), grid, block, 0, s, g, U, VR, sigma, a);
-  else if (a.cap == 8192) hipLaunchKernelGGL((k_gather<8192, W>), grid, block, 0, s, g, U, VR, sigma, a);
+  if (a.cap == 4096) MMP_LAUNCH((k_gather<4096, W>), grid, block, 0, s, g, U, VR, sigma, a);
+  else if (a.cap == 8192) MMP_LAUNCH((k_gather<8192, W>), grid, block, 0, s, g, U, VR, sigma, a);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
@@ -1100,7 +1103,7 @@ hipError_t launch_colstage_fields_pairs(const Geometry& g, const Twiddles& tw, c
                                         cpx<W>* conv, int n_lines, const PairLineArgs& pl, double scale,
                                         hipStream_t s) {
   if (g.N1 != kColStageL || g.M > kColStageMaxM) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((k_colstage_fields_pairs<W>), dim3(g.N2), dim3(256), 0, s, g, TwOf<W>::col(tw),
+  MMP_LAUNCH((k_colstage_fields_pairs<W>), dim3(g.N2), dim3(256), 0, s, g, TwOf<W>::col(tw),
                      TwOf<W>::four(tw), yin, U, conv, n_lines, pl, scale);
   return hipGetLastError();
 }
@@ -1108,10 +1111,10 @@ hipError_t launch_colstage_fields_pairs(const Geometry& g, const Twiddles& tw, c
 #define MMP_COLSTAGE_CASE(MV)                                                                                    \
   case MV:                                                                                                        \
     if constexpr (MV <= 6)                                                                                        \
-      hipLaunchKernelGGL((k_colstage_raman_gather<MV, 8, W>), dim3(g.N2), dim3(512), 0, s, g, TwOf<W>::col(tw),  \
+      MMP_LAUNCH((k_colstage_raman_gather<MV, 8, W>), dim3(g.N2), dim3(512), 0, s, g, TwOf<W>::col(tw),  \
                          TwOf<W>::four(tw), U, conv, yout, ga, fa, Qd, scale);                                    \
     else                                                                                                          \
-      hipLaunchKernelGGL((k_colstage_raman_gather<MV, 4, W>), dim3(g.N2), dim3(256), 0, s, g, TwOf<W>::col(tw),  \
+      MMP_LAUNCH((k_colstage_raman_gather<MV, 4, W>), dim3(g.N2), dim3(256), 0, s, g, TwOf<W>::col(tw),  \
                          TwOf<W>::four(tw), U, conv, yout, ga, fa, Qd, scale);                                    \
     break;
 
@@ -1135,7 +1138,7 @@ hipError_t launch_colstage_raman_gather(const Geometry& g, const Twiddles& tw, c
 
 #define MMP_DENSE_CASE(MV)                                                                                     \
   case MV:                                                                                                      \
-    hipLaunchKernelGGL((k_gather_dense<MV, W>), dim3(g.N / 64), dim3(256), 0, s, g, U, VR, sigma, a, Qd, chunk); \
+    MMP_LAUNCH((k_gather_dense<MV, W>), dim3(g.N / 64), dim3(256), 0, s, g, U, VR, sigma, a, Qd, chunk); \
     break;
 
 template <class W>
@@ -1159,18 +1162,13 @@ __global__ void __launch_bounds__(256) k_narrow(const cd* __restrict__ in, cf* _
 
 hipError_t launch_narrow(const cd* in, cf* out, size_t n, hipStream_t s) {
   const size_t blocks = (n + 255) / 256;
-  hipLaunchKernelGGL(k_narrow, dim3((unsigned)(blocks < 4096 ? (blocks ? blocks : 1) : 4096)), dim3(256), 0, s, in,
+  MMP_LAUNCH(k_narrow, dim3((unsigned)(blocks < 4096 ? (blocks ? blocks : 1) : 4096)), dim3(256), 0, s, in,
                      out, n);
   return hipGetLastError();
 }
 
-hipError_t launch_advance_z(StepState* st, double h, hipStream_t s) {
-  hipLaunchKernelGGL(k_advance_z, dim3(1), dim3(64), 0, s, st, h);
-  return hipGetLastError();
-}
-
 hipError_t launch_energy(const Geometry& g, const cd* fields, double* out, double dt, hipStream_t s) {
-  hipLaunchKernelGGL(k_energy, dim3(g.M), dim3(256), 0, s, g, fields, out, dt);
+  MMP_LAUNCH(k_energy, dim3(g.M), dim3(256), 0, s, g, fields, out, dt);
   return hipGetLastError();
 }
 

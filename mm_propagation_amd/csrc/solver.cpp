@@ -177,12 +177,12 @@ mmp_status run_kernel(mmp_solver* s, const char* name, double bytes, F&& fn) {
     e0 = take_event(s);
     e1 = take_event(s);
     if (!e0 || !e1) return fail(MMP_ERR_DEVICE, "hipEventCreate failed");
-    HIP_TRY(hipEventRecord(e0, s->ctx->stream), "hipEventRecord");
+    g_launch_events = LaunchEvents{e0, e1};  // recorded by the dispatch itself
   }
   hipError_t e = fn();
+  g_launch_events = LaunchEvents{};
   if (e != hipSuccess) return hip_fail(e, name);
   if (s->profile) {
-    HIP_TRY(hipEventRecord(e1, s->ctx->stream), "hipEventRecord");
     s->pending.push_back({kernel_id(s, name), bytes, e0, e1});
     if (s->pending.size() >= 2048) MMP_TRY(drain_profile(s));
   }
