@@ -90,7 +90,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
-                          "(independent fibers shard across GPUs with no exchange, see DESIGN.md)"}))
+                          "(independent fibers shard across GPUs with no exchange)"}))
         return 2
 
     from mm_propagation_amd import _capi, configs

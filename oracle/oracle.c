@@ -10,8 +10,25 @@
  * radix-2 transform (the reference uses FFTW, absent from this image), so
  * stepped fields agree with the reference to FFT rounding only.
  *
- * Pinning (which check fixes which function) is listed in DESIGN.md; every
- * function here is covered by at least one check under tests/.
+ * What pins each function (tests under tests/):
+ *   fft / orc_dft            brute-force long-double DFT, numpy.fft, single-tone sign,
+ *                            Parseval, non-power-of-two sizes (test_oracle_fft.py)
+ *   setup (scaling, beta~, Q~, conversion factor, unique pairs, L with the +NT/2 Nyquist
+ *   bin, E1, E_i, default step count)
+ *                            bit-exact against the reference's own setup sources compiled
+ *                            as they lie, live and through committed vectors
+ *                            (test_oracle_setup.py, test_golden.py), plus closed forms
+ *   tableaux                 order conditions and literal values (test_oracle_setup.py)
+ *   pair decode              integer brute force, slot-mismatch counts (test_oracle_setup.py)
+ *   nonlinearity             direct summation of the Kerr and Raman sums on tiny inputs
+ *                            (test_oracle_physics.py)
+ *   stage loop, Lawson composition, z, restart
+ *                            linear limit = exact dispersion, CW self-phase modulation vs a
+ *                            scalar RK4 of the same ODE, stationary soliton, convergence
+ *                            orders 4/6/8, energy conservation with a symmetric tensor,
+ *                            z accumulation and restart semantics (test_oracle_physics.py)
+ * The composite field of the FFTW-based reference at z = L is not available (its stepping
+ * sources need FFTW, absent here); every function above is pinned by the checks listed.
  */
 #define _GNU_SOURCE
 #include "oracle.h"

@@ -1,5 +1,5 @@
-"""Reference semantics of the device solver: restart on every computeLawsonRK(nz) (F5),
-z by repeated addition (F3), the default step count (F6), resumable stepping, state
+"""Reference semantics of the device solver: restart on every computeLawsonRK(nz),
+z by repeated addition, the default step count floor(z_final / delta_z), resumable stepping, state
 round trips, energy diagnostics and error statuses."""
 import ctypes
 import math
