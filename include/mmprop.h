@@ -188,6 +188,21 @@ MMP_API mmp_status mmp_compute_lawson_rk_default(mmp_solver* solver);
  * mmp_advance(s, b) equals mmp_compute_lawson_rk(s, a + b). */
 MMP_API mmp_status mmp_advance(mmp_solver* solver, uint32_t n_steps);
 
+/* Snapshot modes of mmp_propagate_snapshots. */
+enum {
+  MMP_SNAPSHOTS_REFERENCE_RESTART = 0, /* each chunk is computeLawsonRK(nz): z and the Lawson tables
+                                          restart per chunk, as matlab/mm_propagation.cpp:61-101 does */
+  MMP_SNAPSHOTS_RESUMABLE = 1          /* one solve from z = 0, paused at each snapshot (mmp_advance) */
+};
+
+/* Replaces the n_save loop of the MEX gateway (matlab/mm_propagation.cpp:61-101): n_steps split
+ * into n_save chunks of n_steps / n_save steps, the last one taking the remainder; after each
+ * chunk the physical field (mmp_get_result) is written to fields_out[i] ([n_save][M][NT] complex)
+ * and the distance Z to z_out[i] in metres, accumulated as Z += nz * (fiber_length / n_steps)
+ * like the gateway. Either output may be NULL. n_save must be in [1, n_steps]. */
+MMP_API mmp_status mmp_propagate_snapshots(mmp_solver* solver, uint32_t n_steps, uint32_t n_save, uint32_t mode,
+                                           double* fields_out, double* z_out);
+
 /* Replaces getResult() (src/MultimodePropagation_CPU.cpp:88-104): the
  * physical field A_p = F^-1[exp(L_p z) F psi_p] / conversion_factor, written
  * to fields_out [M][NT] complex. */

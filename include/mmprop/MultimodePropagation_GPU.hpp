@@ -46,6 +46,16 @@ class MultimodePropagationGPU : public MultimodePropagation {
   std::vector<double> getEnergy();
   double getZ();
 
+  // The MEX gateway's n_save loop (matlab/mm_propagation.cpp:61-101): n_steps in n_save chunks
+  // (the last takes the remainder), the physical field and the distance Z (m) after each.
+  // resumable = false restarts z and the Lawson tables per chunk as the gateway does; true runs
+  // one solve paused at each snapshot.
+  struct Snapshot {
+    ComplexArraysContainerCPU fields;
+    double z;
+  };
+  std::vector<Snapshot> propagateWithSnapshots(unsigned int n_steps, unsigned int n_save, bool resumable = false);
+
  protected:
   MultimodePropagationGPU(const SimulationParameters& in, bool raman, const MultimodePropagationGPUOptions& opt);
 

@@ -31,6 +31,8 @@ STATUS_NAMES = {
 
 MMP_PRECISION_FP64 = 0
 MMP_PRECISION_MIXED = 1
+MMP_SNAPSHOTS_REFERENCE_RESTART = 0
+MMP_SNAPSHOTS_RESUMABLE = 1
 MMP_PAIRS_REFERENCE_CPU = 0
 MMP_PAIRS_PHYSICAL = 1
 MMP_FLAG_PROFILE = 1 << 0
@@ -103,6 +105,8 @@ EXPORTS = [
     ("mmp_compute_lawson_rk_default", c_int, [c_void_p]),
     ("mmp_advance", c_int, [c_void_p, c_uint32]),
     ("mmp_get_result", c_int, [c_void_p, POINTER(c_double)]),
+    ("mmp_propagate_snapshots", c_int, [c_void_p, c_uint32, c_uint32, c_uint32, POINTER(c_double),
+                                        POINTER(c_double)]),
     ("mmp_get_state", c_int, [c_void_p, POINTER(c_double), POINTER(c_double)]),
     ("mmp_set_state", c_int, [c_void_p, POINTER(c_double)]),
     ("mmp_energy", c_int, [c_void_p, POINTER(c_double)]),

@@ -24,7 +24,8 @@ ARCH = "gfx950"
 
 HIP_SOURCES = ["kernels.hip", "solver.cpp", "setup.cpp"]
 DROPIN_SOURCES = ["dropin/ComplexArray.cpp", "dropin/ComplexArraysContainer.cpp", "dropin/Sparse3DArray.cpp",
-                  "dropin/RungeKutta.cpp", "dropin/MultimodePropagation.cpp", "dropin/MultimodePropagation_GPU.cpp"]
+                  "dropin/RungeKutta.cpp", "dropin/MultimodePropagation.cpp", "dropin/MultimodePropagation_GPU.cpp",
+                  "dropin/FieldOutput.cpp"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result", "-fvisibility=hidden",

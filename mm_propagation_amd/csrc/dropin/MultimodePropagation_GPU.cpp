@@ -122,6 +122,27 @@ std::vector<double> MultimodePropagationGPU::getEnergy() {
   return e;
 }
 
+std::vector<MultimodePropagationGPU::Snapshot> MultimodePropagationGPU::propagateWithSnapshots(
+    unsigned int n_steps, unsigned int n_save, bool resumable) {
+  const size_t per = 2 * (size_t)M_ * nt_;
+  std::vector<double> buf(per * (n_save ? n_save : 1)), z(n_save ? n_save : 1);
+  check(mmp_propagate_snapshots(solver_, n_steps, n_save,
+                                resumable ? MMP_SNAPSHOTS_RESUMABLE : MMP_SNAPSHOTS_REFERENCE_RESTART, buf.data(),
+                                z.data()),
+        "mmp_propagate_snapshots");
+  std::vector<Snapshot> out;
+  for (unsigned int i = 0; i < n_save; i++) {
+    Snapshot sn{ComplexArraysContainerCPU(nt_, M_), z[i]};
+    for (unsigned int p = 0; p < M_; p++)
+      for (unsigned int j = 0; j < nt_; j++) {
+        const size_t k = i * per + 2 * ((size_t)p * nt_ + j);
+        sn.fields[p][j] = std::complex<double>(buf[k], buf[k + 1]);
+      }
+    out.push_back(sn);
+  }
+  return out;
+}
+
 double MultimodePropagationGPU::getZ() {
   double z = 0.;
   check(mmp_get_state(solver_, nullptr, &z), "mmp_get_state");

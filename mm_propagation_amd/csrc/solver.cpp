@@ -64,6 +64,7 @@ struct mmp_solver {
   mmp_options opt{};
   Problem P;
   double time_window = 0;
+  double fiber_length = 0;
   Geometry g{};
   Twiddles tw{};
   std::vector<void*> allocations;
@@ -411,6 +412,7 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
     return fail(stt, err);
   }
   s->time_window = params->time_window;
+  s->fiber_length = params->fiber_length;
   s->mixed = opt.precision == MMP_PRECISION_MIXED;
   s->wbytes = s->mixed ? sizeof(cf) : sizeof(cd);
   s->profile = (opt.flags & MMP_FLAG_PROFILE) != 0;
@@ -767,6 +769,27 @@ mmp_status mmp_get_result(mmp_solver* s, double* out) {
   const size_t n = 2 * (size_t)s->P.M * s->P.N;
   for (size_t i = 0; i < n; i++)
     if (!std::isfinite(out[i])) return fail(MMP_ERR_NONFINITE, "the propagated field is not finite");
+  return MMP_OK;
+}
+
+mmp_status mmp_propagate_snapshots(mmp_solver* s, uint32_t n_steps, uint32_t n_save, uint32_t mode,
+                                   double* fields_out, double* z_out) {
+  if (!s) return fail(MMP_ERR_INVALID_ARGUMENT, "solver is NULL");
+  if (n_save == 0 || n_save > n_steps) return fail(MMP_ERR_INVALID_ARGUMENT, "n_save must be in [1, n_steps]");
+  if (mode != MMP_SNAPSHOTS_REFERENCE_RESTART && mode != MMP_SNAPSHOTS_RESUMABLE)
+    return fail(MMP_ERR_INVALID_ARGUMENT, "unknown snapshot mode");
+  const uint32_t nz = n_steps / n_save;
+  const double dZ = s->fiber_length / n_steps;
+  const size_t stride = 2 * (size_t)s->P.M * s->P.N;
+  double Z = 0.;
+  for (uint32_t i = 0; i < n_save; i++) {
+    const uint32_t n = (i + 1 < n_save) ? nz : nz + n_steps % n_save;
+    if (mode == MMP_SNAPSHOTS_REFERENCE_RESTART || i == 0) MMP_TRY(mmp_compute_lawson_rk(s, n));
+    else MMP_TRY(mmp_advance(s, n));
+    Z += n * dZ;
+    if (fields_out) MMP_TRY(mmp_get_result(s, fields_out + i * stride));
+    if (z_out) z_out[i] = Z;
+  }
   return MMP_OK;
 }
 

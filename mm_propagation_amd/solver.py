@@ -87,6 +87,20 @@ class MultimodePropagationGPU:
         return out
 
     # --- extensions ----------------------------------------------------
+    def propagate_snapshots(self, n_steps: int, n_save: int, resumable: bool = False):
+        """The MEX gateway's n_save loop (matlab/mm_propagation.cpp:61-101): returns the physical
+        fields [n_save][M][NT] and the distances Z (m) after each chunk. resumable=False restarts
+        z and the Lawson tables per chunk as the gateway does; True runs one solve paused at each
+        snapshot."""
+        fields = np.empty((n_save, self.M, self.nt), dtype=np.complex128)
+        z = np.empty(n_save, dtype=np.float64)
+        mode = _capi.MMP_SNAPSHOTS_RESUMABLE if resumable else _capi.MMP_SNAPSHOTS_REFERENCE_RESTART
+        _capi.check(self.lib.mmp_propagate_snapshots(self.h, int(n_steps), int(n_save), mode,
+                                                     fields.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                     z.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                    "mmp_propagate_snapshots")
+        return fields, z
+
     def advance(self, n: int) -> None:
         """Resumable stepping from the current z (no restart)."""
         _capi.check(self.lib.mmp_advance(self.h, int(n)), "mmp_advance")

@@ -4,6 +4,9 @@
 #include <cmath>
 #include <cstdio>
 
+#include <sstream>
+
+#include "FieldOutput.hpp"
 #include "MultimodePropagation.hpp"
 
 static int failures = 0;
@@ -69,6 +72,15 @@ int main() {
     }
   }
   EXPECT(rk4.a[1][0] == 0.5 && rk4.b[0] == 1 / 6.);
+  // text output of simulations/main.cpp:101-103 for two modes
+  ComplexArraysContainer fin(3, 2), fout(3, 2);
+  fin[0][1] = Complex(3., 4.);
+  fin[1][1] = Complex(0., 2.);
+  fout[0][1] = Complex(-1., 0.);
+  fout[1][1] = Complex(0.5, 0.);
+  std::ostringstream os;
+  writeFieldMagnitudes(os, fin, fout);
+  EXPECT(os.str() == "0 ; 0 ; 0 ; 0\n5 ; 2 ; 1 ; 0.5\n0 ; 0 ; 0 ; 0\n");
   if (failures) std::fprintf(stderr, "%d failure(s)\n", failures);
   else std::printf("dropin selftest passed\n");
   return failures ? 1 : 0;

@@ -125,3 +125,43 @@ def test_planned_bytes_equal_allocated_bytes(precision, flags):
     got = s.info()["device_bytes"]
     s.close()
     assert planned == got
+
+
+@pytest.mark.parametrize("resumable", [False, True])
+def test_snapshots_match_oracle(resumable):
+    """n_save snapshots: the reference restart semantics of the MEX gateway (each chunk a fresh
+    computeLawsonRK) or one resumable solve; both against the oracle driven the same way,
+    with the remainder steps in the last chunk and Z accumulated in metres."""
+    require_device()
+    sp = configs.two_mode_gaussian(nt=1024, n_steps=100)
+    n_steps, n_save = 50, 4  # chunks of 12, 12, 12, 14
+    note(f"case snapshots resumable={resumable}")
+    s = MultimodePropagationGPU_RamanON(sp)
+    fields, z = s.propagate_snapshots(n_steps, n_save, resumable=resumable)
+    s.close()
+    o = ob.Oracle(sp)
+    Z = 0.0
+    for i, n in enumerate([12, 12, 12, 14]):
+        if resumable and i > 0:
+            o.advance(n)
+        else:
+            o.compute_lawson_rk(n)
+        Z += n * (sp.fiber_length / n_steps)
+        assert z[i] == Z
+        assert np.all(configs.rel_l2_per_mode(fields[i], o.get_result()) <= 1e-12)
+    o.close()
+    if resumable:
+        one = MultimodePropagationGPU_RamanON(sp)
+        one.computeLawsonRK(n_steps)
+        assert np.all(configs.rel_l2_per_mode(fields[-1], one.getResult()) <= 1e-13)
+        one.close()
+
+
+def test_snapshot_arguments_are_checked():
+    require_device()
+    s = MultimodePropagationGPU_RamanON(configs.two_mode_gaussian(nt=256))
+    for n_steps, n_save, mode in ((10, 0, 0), (10, 11, 0), (10, 2, 7)):
+        with pytest.raises(_capi.MmpError) as ei:
+            _capi.check(s.lib.mmp_propagate_snapshots(s.h, n_steps, n_save, mode, None, None), "snapshots")
+        assert ei.value.status == _capi.MMP_ERR_INVALID_ARGUMENT
+    s.close()
