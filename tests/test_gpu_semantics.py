@@ -133,8 +133,8 @@ def test_snapshots_match_oracle(resumable):
     computeLawsonRK) or one resumable solve; both against the oracle driven the same way,
     with the remainder steps in the last chunk and Z accumulated in metres."""
     require_device()
-    sp = configs.two_mode_gaussian(nt=1024, n_steps=100)
     n_steps, n_save = 50, 4  # chunks of 12, 12, 12, 14
+    sp = configs.two_mode_gaussian(nt=1024, n_steps=n_steps, fiber_length=0.005)  # h = 1e-4 m
     note(f"case snapshots resumable={resumable}")
     s = MultimodePropagationGPU_RamanON(sp)
     fields, z = s.propagate_snapshots(n_steps, n_save, resumable=resumable)
