@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MMPROP_ABI_VERSION 1
+#define MMPROP_ABI_VERSION 2
 
 /* Only the entry points below are exported; the library is built with hidden visibility. */
 #if defined(__GNUC__) || defined(__clang__)
@@ -120,7 +120,9 @@ typedef struct mmp_info {
   uint32_t default_steps;    /* floor(z_final / delta_z), src/MultimodePropagation.cpp:213-216 */
   double t_final, delta_t, z_final, delta_z, z;
   double conversion_factor, sign_gamma, ld, max_beta2, max_q;
-  size_t device_bytes;       /* device memory held by the solver */
+  size_t device_bytes;       /* device memory held by the solver (all fibers) */
+  uint32_t n_fibers;         /* fibers in the solver (1 unless made by mmp_batch_create) */
+  uint32_t reserved;
 } mmp_info;
 
 typedef struct mmp_ctx mmp_ctx;
@@ -172,6 +174,27 @@ MMP_API void mmp_ctx_destroy(mmp_ctx* ctx);
 MMP_API mmp_status mmp_solver_create(mmp_ctx* ctx, const mmp_params* params,
                              const mmp_options* options, mmp_solver** out);
 MMP_API void mmp_solver_destroy(mmp_solver* solver);
+
+/* Batched solver (no reference counterpart: the reference runs one fiber per object).
+ * n_fibers independent fibers, params[f] each, that share n_modes, nt, method_order, raman and
+ * the pair-slot structure of their coupling lists (their values, dispersion, lengths, pulses
+ * and Raman responses may all differ). Every stage pass runs all fibers in one launch. Needs
+ * the column-stage kernels (Raman on, nt in {4096, 8192, 16384}, n_modes <= 12). The calls
+ * above act on every fiber (mmp_compute_lawson_rk, mmp_advance, mmp_propagate_snapshots's
+ * stepping) or on fiber 0 (results, state, energy, info); the mmp_fiber_* calls address one
+ * fiber. */
+MMP_API mmp_status mmp_batch_create(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params,
+                                    const mmp_options* options, mmp_solver** out);
+MMP_API mmp_status mmp_fiber_result(mmp_solver* solver, uint32_t fiber, double* fields_out);
+MMP_API mmp_status mmp_fiber_state(mmp_solver* solver, uint32_t fiber, double* psi_out, double* z_out);
+/* Sets fiber f's dimensionless field and restarts that fiber alone at z = 0. */
+MMP_API mmp_status mmp_fiber_set_state(mmp_solver* solver, uint32_t fiber, const double* psi);
+MMP_API mmp_status mmp_fiber_energy(mmp_solver* solver, uint32_t fiber, double* energy_out);
+MMP_API mmp_status mmp_fiber_info(mmp_solver* solver, uint32_t fiber, mmp_info* info_out);
+/* Replaces fiber f by a new problem of the same structure (a serving slot taking a new
+ * request): its inputs are uploaded, it restarts at z = 0; the other fibers keep their state.
+ * Waits for the work already queued on the context's stream. */
+MMP_API mmp_status mmp_fiber_load(mmp_solver* solver, uint32_t fiber, const mmp_params* params);
 
 /* Replaces computeLawsonRK(nz) (include/MultimodePropagation_GPU.cuh:143;
  * semantics of src/MultimodePropagation_CPU.cpp:24-34): restarts at z = 0

@@ -182,6 +182,24 @@ def grin_fiber(M: int = 6, nt: int = 16384, n_steps: int = 20000, fiber_length: 
     return sp
 
 
+BATCH_MODES = (3, 6, 10)
+BATCH_NT = (4096, 8192, 16384)
+
+
+def batched_fiber(instance: int, n_steps: int = 10000) -> SimulationParameters:
+    """One fiber of the batched workload: a default_rng(instance) stream draws, in order, M from
+    {3, 6, 10}, NT from {4096, 8192, 16384}, the length U(0.2, 1) m and T0 U(0.05, 0.3) ps; the
+    GRIN fiber itself (coupling values, powers log-uniform in [10 W, 10 kW], phases) is
+    `grin_fiber(..., seed=instance, symmetric=True)`."""
+    rng = np.random.default_rng(instance)
+    M = int(rng.choice(BATCH_MODES))
+    nt = int(rng.choice(BATCH_NT))
+    length = float(rng.uniform(0.2, 1.0))
+    t0 = float(rng.uniform(0.05, 0.3))
+    return grin_fiber(M=M, nt=nt, n_steps=n_steps, fiber_length=length, seed=instance, t0=t0,
+                      power_range=(10.0, 1.0e4), symmetric=True)
+
+
 def grin2_soliton_pair(nt: int = 2 ** 15, n_steps: int = 1024) -> SimulationParameters:
     """The reference's 2-mode GRIN soliton-pair example (simulations/main.cpp:10-96)."""
     t_final = 160.0

@@ -64,6 +64,32 @@ struct StepState {
   uint32_t pad;
 };
 
+// Per-fiber scalars of a batch of fibers that share M, NT, the tableau and the pair structure.
+struct FiberParams {
+  double h;    // the step h (dimensionless delta z)
+  double sgn;  // sign of gamma
+  double dt;   // delta t, the Raman convolution's quadrature weight
+  double kv;   // slot mix: 1 - fR (Raman on) or 1 (off)
+  double kr;   // slot mix: fR T0 (Raman on) or 0 (off)
+};
+
+// Fiber f of a batch (blockIdx.y in the batched kernels) finds its arrays at base + f * stride.
+// Per fiber, per stage head i: kHeadCoef doubles, [0] = b_{i-1} h, [1 + e] = a_{i, a_idx[e]} h.
+constexpr int kHeadCoef = kMaxStages + 1;
+
+struct Batch {
+  const FiberParams* fp;  // [F]
+  const double* coef;     // [F][nheads][kHeadCoef] stage-head coefficients times the fiber's h
+  size_t nheads;          // s + 1
+  StepState* st;          // [F]
+  size_t sf;              // M x NT field arrays (complex values)
+  size_t sl;              // Raman line arrays (lines x NT)
+  size_t se;              // Lawson table sets (tables x M x NT)
+  size_t ss;              // stage stash (s x M x NT)
+  size_t sq;              // dense coupling blocks (P x M x M coefficients)
+  size_t sh;              // Raman spectra (NT)
+};
+
 // Arguments of the row "stage head" kernel for stage i of the tableau
 // (i == s finishes the step). Lawson tables E_u = exp(c_u h L) * E1^n (one per
 // distinct c_u, four-step order) are advanced by the head that last reads them
@@ -79,9 +105,9 @@ struct HeadArgs {
   cd* e_prev;          // Lawson table of stage i-1 (conjugated for N_{i-1})
   const cd* e_cur;     // Lawson table of stage i
   const cd* e1;        // exp(h L)
-  double bh_prev;      // b_{i-1} h (0 when b_{i-1} == 0)
-  double sgn;          // sign of gamma
-  double scale_cur;    // 1/N folded into the stage-i product (exact: N is a power of two)
+  int head;            // i (selects the fiber's b_{i-1} h and a_ij h in Batch::coef)
+  double b_prev;       // b_{i-1} (0 when b_{i-1} == 0): whether N_{i-1} enters the step result
+  double scale_cur;    // factor folded into the stage-i product (1)
   int has_prev;        // 1 when i > 0
   int prev_stage;      // i - 1
   int prev_is_first;   // 1 when i - 1 == 0 (acc starts from psi_hat)
@@ -89,11 +115,10 @@ struct HeadArgs {
   int final_stage;     // 1 when i == s: psi_hat = acc, no row-inverse output
   int update_prev;     // 1 when this head is the last reader of e_prev in the step
   int chain_next;      // with final_stage: also build stage 0 of the next step (E_0 in e_cur, no a terms)
-  StepState* st;       // with final_stage: z += h, steps += 1 (NULL: left alone)
-  double h;            // the step h
+  int advance_z;       // with final_stage: z += h, steps += 1 on the fiber's StepState
   int n_a;             // number of nonzero a_ij, j < i, in increasing j
-  int a_idx[kMaxStages];   // j (N_{i-1} comes from registers, earlier N_j from stash)
-  double a_h[kMaxStages];  // a_ij h
+  int a_idx[kMaxStages];    // j (N_{i-1} comes from registers, earlier N_j from stash)
+  double a_coef[kMaxStages];  // a_ij (the fiber's a_ij h come from Batch::coef)
 };
 
 struct GatherArgs {
@@ -140,8 +165,11 @@ int colstage_coefficients_per_chunk();
 
 // Launchers (return hipSuccess or the launch error). Those templated on W run in the
 // work precision W (double, or float for the mixed-precision step); the others are FP64.
+// The stage kernels (row head, row Raman, column-stage C1 and C2) run F fibers of a batch
+// in one launch (grid y = fiber); F = 1 for a single solver.
 template <class W>
-hipError_t launch_row_head(const Geometry& g, const Twiddles& tw, const HeadArgs& a, hipStream_t s);
+hipError_t launch_row_head(const Geometry& g, const Twiddles& tw, const HeadArgs& a, const Batch& b, int F,
+                           hipStream_t s);
 template <class W>
 hipError_t launch_col_inverse(const Geometry& g, const Twiddles& tw, const cpx<W>* in, cpx<W>* out, int lines,
                               double scale, double divisor, hipStream_t s);
@@ -158,18 +186,18 @@ hipError_t launch_row_inverse_phase(const Geometry& g, const Twiddles& tw, const
                                     const double* imL, const StepState* st, int use_phase, hipStream_t s);
 template <class W>
 hipError_t launch_row_raman(const Geometry& g, const Twiddles& tw, cpx<W>* data, int lines, const cpx<W>* hspec,
-                            double hscale, hipStream_t s);
+                            const Batch& b, int F, hipStream_t s);
 template <class W>
 hipError_t launch_gather(const Geometry& g, const cpx<W>* U, const cpx<W>* VR, cpx<W>* sigma, const GatherArgs& a,
                          hipStream_t s);
 template <class W>
 hipError_t launch_colstage_fields_pairs(const Geometry& g, const Twiddles& tw, const cpx<W>* yin, cpx<W>* U,
                                         cpx<W>* conv, int n_lines, const PairLineArgs& pl, double scale,
-                                        hipStream_t s);
+                                        const Batch& b, int F, hipStream_t s);
 template <class W>
 hipError_t launch_colstage_raman_gather(const Geometry& g, const Twiddles& tw, const cpx<W>* U, const cpx<W>* conv,
                                         cpx<W>* yout, const GatherArgs& ga, const FusedArgs& fa, const W* Qd,
-                                        double scale, hipStream_t s);
+                                        double scale, const Batch& b, int F, hipStream_t s);
 // Dense-block gather (M <= 16): Qd is [P][M][M] with Qd[s][p][k] = sum of Q/max|Q| over the
 // nonzeros (k, slot s) of Q^(p). dense_gather_chunk(M) is 0 when M is out of range.
 int dense_gather_chunk(int M);

@@ -74,7 +74,7 @@ __device__ __forceinline__ cd cmul_exact(cd a, cd b) {
 }
 
 template <int L, class W>
-__global__ void __launch_bounds__(64) k_row_head(Geometry g, const cpx<W>* __restrict__ rowtw, HeadArgs a) {
+__global__ void __launch_bounds__(64) k_row_head(Geometry g, const cpx<W>* __restrict__ rowtw, HeadArgs a, Batch b) {
   using C = RowCfg<L>;
   using cw = cpx<W>;
   __shared__ cw smem[C::G * C::SLOTS];
@@ -85,8 +85,18 @@ __global__ void __launch_bounds__(64) k_row_head(Geometry g, const cpx<W>* __res
   const size_t base = valid ? (size_t)row * g.N2 + t : t;
   cw* sm = smem + gl * C::SLOTS;
   const size_t MN = (size_t)g.M * g.N;
-  const cw* __restrict__ ysig = static_cast<const cw*>(a.ysig);
-  cw* __restrict__ yout = static_cast<cw*>(a.yout);
+  // fiber blockIdx.y of the batch: its arrays and scalars
+  const int fib = blockIdx.y;
+  const FiberParams fp = b.fp[fib];
+  const double* __restrict__ hc = b.coef + ((size_t)fib * b.nheads + a.head) * kHeadCoef;  // uniform
+  cd* __restrict__ psi_hat = a.psi_hat + fib * b.sf;
+  cd* __restrict__ accp = a.acc + fib * b.sf;
+  cd* __restrict__ stash = a.stash ? a.stash + fib * b.ss : nullptr;
+  cd* __restrict__ e_prev = a.e_prev ? a.e_prev + fib * b.se : nullptr;
+  const cd* __restrict__ e_cur = a.e_cur ? a.e_cur + fib * b.se : nullptr;
+  const cd* __restrict__ e1p = a.e1 + fib * b.sf;
+  const cw* __restrict__ ysig = static_cast<const cw*>(a.ysig) + fib * b.sf;
+  cw* __restrict__ yout = static_cast<cw*>(a.yout) + fib * b.sf;
   LineTw<L, W> lt;
   load_line_tw<L>(lt, rowtw, t);
 
@@ -98,35 +108,36 @@ __global__ void __launch_bounds__(64) k_row_head(Geometry g, const cpx<W>* __res
   for (int i = 0; i < 8; i++) {
     const size_t idx = base + i * C::T;
     v[i] = (valid && a.has_prev) ? ysig[idx] : cw{0, 0};
-    ep[i] = (valid && a.has_prev) ? a.e_prev[idx] : zero;
-    e1[i] = (valid && a.update_prev) ? a.e1[idx] : zero;
-    psi[i] = (valid && (a.prev_is_first || !a.final_stage)) ? a.psi_hat[idx] : zero;
-    accin[i] = (valid && a.has_prev && !a.prev_is_first) ? a.acc[idx] : zero;
-    ec[i] = (valid && (!a.final_stage || a.chain_next)) ? a.e_cur[idx] : zero;
+    ep[i] = (valid && a.has_prev) ? e_prev[idx] : zero;
+    psi[i] = (valid && (a.prev_is_first || !a.final_stage)) ? psi_hat[idx] : zero;
+    accin[i] = (valid && a.has_prev && !a.prev_is_first) ? accp[idx] : zero;
+    e1[i] = (valid && a.update_prev) ? e1p[idx] : zero;
+    ec[i] = (valid && (!a.final_stage || a.chain_next)) ? e_cur[idx] : zero;
   }
-  if (a.final_stage && a.st && blockIdx.x == 0 && threadIdx.x == 0) {
-    a.st->z += a.h;  // z += h (repeated FP64 addition)
-    a.st->steps += 1;
+  if (a.final_stage && a.advance_z && blockIdx.x == 0 && threadIdx.x == 0) {
+    b.st[fib].z += fp.h;  // z += h (repeated FP64 addition)
+    b.st[fib].steps += 1;
   }
   cd nh[8];
+  const double bh = hc[0];
   if (a.has_prev) {
     line_fft<L, -1, 1, 1>(v, sm, lt, t);
 #pragma unroll
     for (int i = 0; i < 8; i++) {
       const cd x = cmulc(widen(v[i]), ep[i]);      // sigma_hat * conj(E_{i-1})
-      nh[i] = {-x.im * a.sgn, x.re * a.sgn};       // * (i sgn)
+      nh[i] = {-x.im * fp.sgn, x.re * fp.sgn};     // * (i sgn)
       cd acc = a.prev_is_first ? psi[i] : accin[i];
-      if (a.bh_prev != 0.) acc = cadd(acc, cscale(nh[i], a.bh_prev));
+      if (a.b_prev != 0.) acc = cadd(acc, cscale(nh[i], bh));
       accin[i] = acc;
     }
     if (valid) {
 #pragma unroll
       for (int i = 0; i < 8; i++) {
         const size_t idx = base + i * C::T;
-        if (a.final_stage) a.psi_hat[idx] = accin[i];
-        else a.acc[idx] = accin[i];
-        if (a.stash_prev) a.stash[(size_t)a.prev_stage * MN + idx] = nh[i];
-        if (a.update_prev) a.e_prev[idx] = cmul_exact(ep[i], e1[i]);
+        if (a.final_stage) psi_hat[idx] = accin[i];
+        else accp[idx] = accin[i];
+        if (a.stash_prev) stash[(size_t)a.prev_stage * MN + idx] = nh[i];
+        if (a.update_prev) e_prev[idx] = cmul_exact(ep[i], e1[i]);
       }
     }
   }
@@ -142,8 +153,8 @@ __global__ void __launch_bounds__(64) k_row_head(Geometry g, const cpx<W>* __res
       const int j = a.a_idx[e];
       cd nj;
       if (j == a.prev_stage) nj = nh[i];
-      else nj = valid ? a.stash[(size_t)j * MN + idx] : zero;
-      x = cadd(x, cscale(nj, a.a_h[e]));
+      else nj = valid ? stash[(size_t)j * MN + idx] : zero;
+      x = cadd(x, cscale(nj, hc[1 + e]));
     }
     v[i] = narrow<W>(cscale(cmul(ec[i], x), a.scale_cur));
   }
@@ -378,10 +389,14 @@ __global__ void __launch_bounds__(64) k_row_inverse_phase(Geometry g, const cd* 
 template <int L, class W>
 __global__ void __launch_bounds__(64) k_row_raman(Geometry g, const cpx<W>* __restrict__ rowtw,
                                                   cpx<W>* __restrict__ data, int lines,
-                                                  const cpx<W>* __restrict__ hspec, double hscale) {
+                                                  const cpx<W>* __restrict__ hspec, Batch b) {
   using C = RowCfg<L>;
   using cw = cpx<W>;
   __shared__ cw smem[C::G * C::SLOTS];
+  const int fib = blockIdx.y;
+  data += fib * b.sl;
+  hspec += fib * b.sh;
+  const double hscale = b.fp[fib].dt;
   const int tid = threadIdx.x;
   const int gl = tid / C::T, t = tid % C::T;
   const int row = blockIdx.x * C::G + gl;
@@ -609,11 +624,15 @@ __global__ void __launch_bounds__(256) k_colstage_fields_pairs(Geometry g, const
                                                                const cpx<W>* __restrict__ fourtw,
                                                                const cpx<W>* __restrict__ yin, cpx<W>* __restrict__ U,
                                                                cpx<W>* __restrict__ conv, int n_lines,
-                                                               PairLineArgs pl, double scale) {
+                                                               PairLineArgs pl, double scale, Batch b) {
   using F = ColStageCfg;
   using cw = cpx<W>;
   constexpr int L = F::L;
   __shared__ cw sh[F::FFT + kColStageMaxM * L];
+  const int fib = blockIdx.y;
+  yin += fib * b.sf;
+  U += fib * b.sf;
+  conv += fib * b.sl;
   cw* Ush = sh + F::FFT;  // [M][L]
   const int tid = threadIdx.x;
   const int job = tid / F::T, t = tid % F::T;
@@ -692,8 +711,14 @@ __global__ void __launch_bounds__(NW * 64) k_colstage_raman_gather(Geometry g, c
                                                                    const cpx<W>* __restrict__ conv,
                                                                    cpx<W>* __restrict__ yout, GatherArgs a,
                                                                    FusedArgs fa, const W* __restrict__ Qd,
-                                                                   double scale) {
+                                                                   double scale, Batch b) {
   using cw = cpx<W>;
+  const int fib = blockIdx.y;
+  const FiberParams fp = b.fp[fib];
+  U += fib * b.sf;
+  yout += fib * b.sf;
+  conv += fib * b.sl;
+  Qd += fib * b.sq;
   constexpr int L = kColStageL, T = L / 8, RS = kColStageRS;
   constexpr int NT = NW * 64;
   constexpr int JOBS = NT / T;  // line slots per round
@@ -745,7 +770,7 @@ __global__ void __launch_bounds__(NW * 64) k_colstage_raman_gather(Geometry g, c
     const int e = tid + r * NT;
     if (e < M * L) Ush[(e / L) * RS + e % L] = uin[r];
   }
-  const W sc = (W)scale, kv = (W)a.kv, kr = (W)a.kr;
+  const W sc = (W)scale, kv = (W)fp.kv, kr = (W)fp.kr;
   cw u[M], acc[M];
 #pragma unroll
   for (int p = 0; p < M; p++) acc[p] = {0, 0};
@@ -959,16 +984,19 @@ __global__ void __launch_bounds__(256) k_energy(Geometry g, const cd* __restrict
   }
 
 template <int L, class W>
-static hipError_t row_head_impl(const Geometry& g, const Twiddles& tw, const HeadArgs& a, hipStream_t s) {
+static hipError_t row_head_impl(const Geometry& g, const Twiddles& tw, const HeadArgs& a, const Batch& b, int F,
+                                hipStream_t s) {
   using C = RowCfg<L>;
   const int rows = g.M * g.N1;
-  dim3 grid((rows + C::G - 1) / C::G);
-  MMP_LAUNCH((k_row_head<L, W>), grid, dim3(C::THREADS), 0, s, g, TwOf<W>::row(tw), a);
+  dim3 grid((rows + C::G - 1) / C::G, F);
+  MMP_LAUNCH((k_row_head<L, W>), grid, dim3(C::THREADS), 0, s, g, TwOf<W>::row(tw), a, b);
   return hipGetLastError();
 }
 template <class W>
-hipError_t launch_row_head(const Geometry& g, const Twiddles& tw, const HeadArgs& a, hipStream_t s) {
-  MMP_DISPATCH_LW(g.N2, row_head_impl, W, g, tw, a, s);
+hipError_t launch_row_head(const Geometry& g, const Twiddles& tw, const HeadArgs& a, const Batch& b, int F,
+                           hipStream_t s) {
+  if (F < 1 || F > 65535) return hipErrorInvalidValue;
+  MMP_DISPATCH_LW(g.N2, row_head_impl, W, g, tw, a, b, F, s);
 }
 
 // Column-block width: 8 when that already gives >= 256 workgroups, else 4, else 2.
@@ -1072,17 +1100,18 @@ hipError_t launch_row_inverse_phase(const Geometry& g, const Twiddles& tw, const
 
 template <int L, class W>
 static hipError_t row_raman_impl(const Geometry& g, const Twiddles& tw, cpx<W>* data, int lines, const cpx<W>* hspec,
-                                 double hscale, hipStream_t s) {
+                                 const Batch& b, int F, hipStream_t s) {
   using C = RowCfg<L>;
   const int rows = lines * g.N1;
-  MMP_LAUNCH((k_row_raman<L, W>), dim3((rows + C::G - 1) / C::G), dim3(C::THREADS), 0, s, g,
-                     TwOf<W>::row(tw), data, lines, hspec, hscale);
+  MMP_LAUNCH((k_row_raman<L, W>), dim3((rows + C::G - 1) / C::G, F), dim3(C::THREADS), 0, s, g,
+                     TwOf<W>::row(tw), data, lines, hspec, b);
   return hipGetLastError();
 }
 template <class W>
 hipError_t launch_row_raman(const Geometry& g, const Twiddles& tw, cpx<W>* data, int lines, const cpx<W>* hspec,
-                            double hscale, hipStream_t s) {
-  MMP_DISPATCH_LW(g.N2, row_raman_impl, W, g, tw, data, lines, hspec, hscale, s);
+                            const Batch& b, int F, hipStream_t s) {
+  if (F < 1 || F > 65535) return hipErrorInvalidValue;
+  MMP_DISPATCH_LW(g.N2, row_raman_impl, W, g, tw, data, lines, hspec, b, F, s);
 }
 
 template <class W>
@@ -1101,28 +1130,28 @@ hipError_t launch_gather(const Geometry& g, const cpx<W>* U, const cpx<W>* VR, c
 template <class W>
 hipError_t launch_colstage_fields_pairs(const Geometry& g, const Twiddles& tw, const cpx<W>* yin, cpx<W>* U,
                                         cpx<W>* conv, int n_lines, const PairLineArgs& pl, double scale,
-                                        hipStream_t s) {
-  if (g.N1 != kColStageL || g.M > kColStageMaxM) return hipErrorInvalidValue;
-  MMP_LAUNCH((k_colstage_fields_pairs<W>), dim3(g.N2), dim3(256), 0, s, g, TwOf<W>::col(tw),
-                     TwOf<W>::four(tw), yin, U, conv, n_lines, pl, scale);
+                                        const Batch& b, int F, hipStream_t s) {
+  if (g.N1 != kColStageL || g.M > kColStageMaxM || F < 1 || F > 65535) return hipErrorInvalidValue;
+  MMP_LAUNCH((k_colstage_fields_pairs<W>), dim3(g.N2, F), dim3(256), 0, s, g, TwOf<W>::col(tw),
+                     TwOf<W>::four(tw), yin, U, conv, n_lines, pl, scale, b);
   return hipGetLastError();
 }
 
 #define MMP_COLSTAGE_CASE(MV)                                                                                    \
   case MV:                                                                                                        \
     if constexpr (MV <= 6)                                                                                        \
-      MMP_LAUNCH((k_colstage_raman_gather<MV, 8, W>), dim3(g.N2), dim3(512), 0, s, g, TwOf<W>::col(tw),  \
-                         TwOf<W>::four(tw), U, conv, yout, ga, fa, Qd, scale);                                    \
+      MMP_LAUNCH((k_colstage_raman_gather<MV, 8, W>), dim3(g.N2, F), dim3(512), 0, s, g, TwOf<W>::col(tw),       \
+                         TwOf<W>::four(tw), U, conv, yout, ga, fa, Qd, scale, b);                                 \
     else                                                                                                          \
-      MMP_LAUNCH((k_colstage_raman_gather<MV, 4, W>), dim3(g.N2), dim3(256), 0, s, g, TwOf<W>::col(tw),  \
-                         TwOf<W>::four(tw), U, conv, yout, ga, fa, Qd, scale);                                    \
+      MMP_LAUNCH((k_colstage_raman_gather<MV, 4, W>), dim3(g.N2, F), dim3(256), 0, s, g, TwOf<W>::col(tw),       \
+                         TwOf<W>::four(tw), U, conv, yout, ga, fa, Qd, scale, b);                                 \
     break;
 
 template <class W>
 hipError_t launch_colstage_raman_gather(const Geometry& g, const Twiddles& tw, const cpx<W>* U, const cpx<W>* conv,
                                         cpx<W>* yout, const GatherArgs& ga, const FusedArgs& fa, const W* Qd,
-                                        double scale, hipStream_t s) {
-  if (g.N1 != kColStageL) return hipErrorInvalidValue;
+                                        double scale, const Batch& b, int F, hipStream_t s) {
+  if (g.N1 != kColStageL || F < 1 || F > 65535) return hipErrorInvalidValue;
   if (fa.n_chunks < 1 || fa.max_chunk_lines < 1 || fa.max_chunk_lines > colstage_lines_per_chunk() ||
       fa.max_chunk_slots < 1 || fa.max_chunk_slots > colstage_slots_per_chunk() ||
       fa.max_chunk_slots * g.M * g.M > colstage_coefficients_per_chunk())
@@ -1173,24 +1202,26 @@ hipError_t launch_energy(const Geometry& g, const cd* fields, double* out, doubl
 }
 
 #define MMP_INSTANTIATE_W(W)                                                                                        \
-  template hipError_t launch_row_head<W>(const Geometry&, const Twiddles&, const HeadArgs&, hipStream_t);         \
+  template hipError_t launch_row_head<W>(const Geometry&, const Twiddles&, const HeadArgs&, const Batch&, int,    \
+                                         hipStream_t);                                                             \
   template hipError_t launch_col_inverse<W>(const Geometry&, const Twiddles&, const cpx<W>*, cpx<W>*, int, double, \
                                             double, hipStream_t);                                                 \
   template hipError_t launch_col_forward<W>(const Geometry&, const Twiddles&, const cpx<W>*, cpx<W>*, int,         \
                                             hipStream_t);                                                         \
   template hipError_t launch_col_pairs_forward<W>(const Geometry&, const Twiddles&, const cpx<W>*, cpx<W>*, int,   \
                                                   const PairLineArgs&, hipStream_t);                              \
-  template hipError_t launch_row_raman<W>(const Geometry&, const Twiddles&, cpx<W>*, int, const cpx<W>*, double,   \
-                                          hipStream_t);                                                           \
+  template hipError_t launch_row_raman<W>(const Geometry&, const Twiddles&, cpx<W>*, int, const cpx<W>*,           \
+                                          const Batch&, int, hipStream_t);                                         \
   template hipError_t launch_gather<W>(const Geometry&, const cpx<W>*, const cpx<W>*, cpx<W>*, const GatherArgs&,  \
                                        hipStream_t);                                                              \
   template hipError_t launch_gather_dense<W>(const Geometry&, const cpx<W>*, const cpx<W>*, cpx<W>*,               \
                                              const GatherArgs&, const W*, hipStream_t);                            \
   template hipError_t launch_colstage_fields_pairs<W>(const Geometry&, const Twiddles&, const cpx<W>*, cpx<W>*,     \
-                                                      cpx<W>*, int, const PairLineArgs&, double, hipStream_t);     \
+                                                      cpx<W>*, int, const PairLineArgs&, double, const Batch&, int, \
+                                                      hipStream_t);                                                \
   template hipError_t launch_colstage_raman_gather<W>(const Geometry&, const Twiddles&, const cpx<W>*,             \
                                                       const cpx<W>*, cpx<W>*, const GatherArgs&, const FusedArgs&, \
-                                                      const W*, double, hipStream_t);
+                                                      const W*, double, const Batch&, int, hipStream_t);
 MMP_INSTANTIATE_W(double)
 MMP_INSTANTIATE_W(float)
 

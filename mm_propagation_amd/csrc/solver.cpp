@@ -2,6 +2,7 @@
 // memory, step scheduling (eager or HIP-graph replay) and results.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -62,9 +63,15 @@ struct KernelRecord {
 struct mmp_solver {
   mmp_ctx* ctx = nullptr;
   mmp_options opt{};
-  Problem P;
-  double time_window = 0;
-  double fiber_length = 0;
+  Problem P;                      // fiber 0; its structure (sizes, tableau, slots) is the batch's
+  uint32_t F = 1;                 // fibers in the batch
+  std::vector<Problem> fib;       // per-fiber scalars (large per-fiber arrays are freed once uploaded)
+  std::vector<double> time_window, fiber_length, z_host;  // per fiber
+  FiberParams* d_fp = nullptr;    // [F]
+  double* d_coef = nullptr;       // [F][s + 1][kHeadCoef] stage-head coefficients times each fiber's h
+  Batch batch{};
+  cd* etab0 = nullptr;            // [F] initial Lawson tables (restart copies them into etab)
+  size_t ntab = 0;                // Lawson tables per fiber
   Geometry g{};
   Twiddles tw{};
   std::vector<void*> allocations;
@@ -99,8 +106,7 @@ struct mmp_solver {
   void* hspec_w = nullptr;
   const void* q_w = nullptr;
   const void* qdense_w = nullptr;
-  double z_host = 0;
-  uint32_t steps_taken = 0;
+  std::vector<uint32_t> steps;  // per fiber, since its last restart
   bool graphs = true;
   hipGraphExec_t graph_exec = nullptr;
   hipGraph_t graph = nullptr;
@@ -215,8 +221,8 @@ mmp_status enqueue_head_w(mmp_solver* s, int i) {
       if (h.a_idx[e] != h.prev_stage) state_arrays += 1;  // stashed N_j
   }
   const double hb = MN * (cs * state_arrays + cb * work_arrays);
-  return run_kernel(s, i == P.rk.s ? "row_close_step" : "row_stage_head", hb,
-                    [&] { return launch_row_head<W>(s->g, s->tw, h, s->ctx->stream); });
+  return run_kernel(s, i == P.rk.s ? "row_close_step" : "row_stage_head", hb * s->F,
+                    [&] { return launch_row_head<W>(s->g, s->tw, h, s->batch, (int)s->F, s->ctx->stream); });
 }
 
 mmp_status enqueue_head(mmp_solver* s, int i) {
@@ -244,14 +250,17 @@ mmp_status enqueue_step_w(mmp_solver* s) {
   for (int i = 0; i <= P.rk.s; i++) {
     if (i > 0 || !s->chain) MMP_TRY(enqueue_head_w<W>(s, i));
     if (i == P.rk.s) break;
+    const int F = (int)s->F;
     if (s->use_fused) {
-      MMP_TRY(run_kernel(s, "col_fields_pairs", cb * (2 * MN + LN), [&] {
-        return launch_colstage_fields_pairs<W>(g, s->tw, ybuf, U, conv, s->n_lines, s->pairs, invN, st);
+      MMP_TRY(run_kernel(s, "col_fields_pairs", cb * (2 * MN + LN) * F, [&] {
+        return launch_colstage_fields_pairs<W>(g, s->tw, ybuf, U, conv, s->n_lines, s->pairs, invN, s->batch, F, st);
       }));
-      MMP_TRY(run_kernel(s, "row_raman", cb * (2 * LN + P.N),
-                         [&] { return launch_row_raman<W>(g, s->tw, conv, s->n_lines, hspec, P.delta_t, st); }));
-      MMP_TRY(run_kernel(s, "col_raman_gather", cb * (2 * MN + LN), [&] {
-        return launch_colstage_raman_gather<W>(g, s->tw, U, conv, ybuf, s->gather, s->fused, qdense, invN, st);
+      MMP_TRY(run_kernel(s, "row_raman", cb * (2 * LN + P.N) * F, [&] {
+        return launch_row_raman<W>(g, s->tw, conv, s->n_lines, hspec, s->batch, F, st);
+      }));
+      MMP_TRY(run_kernel(s, "col_raman_gather", cb * (2 * MN + LN) * F, [&] {
+        return launch_colstage_raman_gather<W>(g, s->tw, U, conv, ybuf, s->gather, s->fused, qdense, invN, s->batch, F,
+                                               st);
       }));
       continue;
     }
@@ -262,7 +271,7 @@ mmp_status enqueue_step_w(mmp_solver* s) {
         return launch_col_pairs_forward<W>(g, s->tw, U, conv, s->n_lines, s->pairs, st);
       }));
       MMP_TRY(run_kernel(s, "row_raman", cb * (2 * LN + P.N),
-                         [&] { return launch_row_raman<W>(g, s->tw, conv, s->n_lines, hspec, P.delta_t, st); }));
+                         [&] { return launch_row_raman<W>(g, s->tw, conv, s->n_lines, hspec, s->batch, 1, st); }));
       MMP_TRY(run_kernel(s, "col_inverse_raman", 2 * cb * LN, [&] {
         return launch_col_inverse<W>(g, s->tw, conv, conv, s->n_lines, invN, 1.0, st);
       }));
@@ -308,30 +317,50 @@ mmp_status set_device(mmp_solver* s) {
 }
 
 // initLawsonRK (src/MultimodePropagation.cpp:158-163): z = 0 and fresh Lawson tables.
+// z = 0 and the initial Lawson tables for every fiber (zeroed on the host: no memset node).
 mmp_status reset_z(mmp_solver* s) {
+  const std::vector<StepState> zero(s->F, StepState{0., 0u, 0u});
+  MMP_TRY(upload(s, s->st, zero.data(), zero.size() * sizeof(StepState)));
+  HIP_TRY(hipMemcpyAsync(s->etab, s->etab0, (size_t)s->F * s->ntab * s->P.M * s->P.N * sizeof(cd),
+                         hipMemcpyDeviceToDevice, s->ctx->stream),
+          "hipMemcpyAsync D2D");
+  HIP_TRY(hipStreamSynchronize(s->ctx->stream), "hipStreamSynchronize");  // `zero` is a host staging copy
+  std::fill(s->z_host.begin(), s->z_host.end(), 0.);
+  std::fill(s->steps.begin(), s->steps.end(), 0u);
+  return MMP_OK;
+}
+
+// z = 0 and the initial Lawson tables for fiber f only.
+mmp_status reset_fiber(mmp_solver* s, uint32_t f) {
   static const StepState zero{0., 0u, 0u};
-  MMP_TRY(upload(s, s->st, &zero, sizeof zero));
-  MMP_TRY(upload(s, s->etab, s->P.etab0.data(), s->P.etab0.size() * sizeof(double)));
-  s->z_host = 0.;
-  s->steps_taken = 0;
+  MMP_TRY(upload(s, s->st + f, &zero, sizeof zero));
+  HIP_TRY(hipMemcpyAsync(s->etab + f * s->batch.se, s->etab0 + f * s->batch.se, s->batch.se * sizeof(cd),
+                         hipMemcpyDeviceToDevice, s->ctx->stream),
+          "hipMemcpyAsync D2D");
+  s->z_host[f] = 0.;
+  s->steps[f] = 0;
   return MMP_OK;
 }
 
-// psi_hat = F(psi) for a time-domain field already in s->U
-mmp_status forward_state_from_U(mmp_solver* s) {
+// psi_hat of fiber f = F(psi) for a time-domain field already in s->U (its first M x NT values)
+mmp_status forward_state_from_U(mmp_solver* s, uint32_t f) {
   hipStream_t st = s->ctx->stream;
-  HIP_TRY(launch_col_forward<double>(s->g, s->tw, s->U, s->psi_hat, s->P.M, st), "col_forward");
-  HIP_TRY(launch_row_forward(s->g, s->tw, s->psi_hat, s->P.M, st), "row_forward");
+  cd* ph = s->psi_hat + (size_t)f * s->batch.sf;
+  HIP_TRY(launch_col_forward<double>(s->g, s->tw, s->U, ph, s->P.M, st), "col_forward");
+  HIP_TRY(launch_row_forward(s->g, s->tw, ph, s->P.M, st), "row_forward");
   return MMP_OK;
 }
 
-// time-domain field of psi_hat into s->U: with_phase -> exp(L z) applied and divided by conv
-mmp_status inverse_state_to_U(mmp_solver* s, bool with_phase) {
+// time-domain field of fiber f's psi_hat into the first M x NT values of s->U: with_phase ->
+// exp(L z) applied and divided by the fiber's conversion factor
+mmp_status inverse_state_to_U(mmp_solver* s, uint32_t f, bool with_phase) {
   hipStream_t st = s->ctx->stream;
-  HIP_TRY(launch_row_inverse_phase(s->g, s->tw, s->psi_hat, s->ybuf, s->imL, s->st, with_phase ? 1 : 0, st),
+  const size_t off = (size_t)f * s->batch.sf;
+  HIP_TRY(launch_row_inverse_phase(s->g, s->tw, s->psi_hat + off, s->ybuf, s->imL + off, s->st + f,
+                                   with_phase ? 1 : 0, st),
           "row_inverse_phase");
   HIP_TRY(launch_col_inverse<double>(s->g, s->tw, s->ybuf, s->U, s->P.M, 1.0 / s->P.N,
-                             with_phase ? s->P.conversion_factor : 1.0, st),
+                                     with_phase ? s->fib[f].conversion_factor : 1.0, st),
           "col_inverse");
   return MMP_OK;
 }
@@ -352,24 +381,26 @@ void release(mmp_solver* s) {
 }
 
 // Device bytes create_impl allocates for problem P (kept in step with its dalloc calls).
-size_t planned_device_bytes(const Problem& P, const mmp_options& opt) {
+// Device bytes create_impl allocates for F fibers of problem P (kept in step with its dalloc calls).
+size_t planned_device_bytes(const Problem& P, const mmp_options& opt, size_t F) {
   auto c = [](size_t n) { return n ? n : (size_t)1; };
   const bool mixed = opt.precision == MMP_PRECISION_MIXED;
-  const size_t MN = (size_t)P.M * P.N, L = P.line_a.size();
+  const size_t MN = (size_t)P.M * P.N, L = P.line_a.size(), Fw = mixed ? 1 : F;
   bool any_stash = false;
   for (int j = 0; j < P.rk.s; j++) any_stash = any_stash || P.stash_needed[j];
-  size_t b = 16 * (4 * MN) + 8 * MN + 16 * (P.N2 + P.N1 + P.N) + sizeof(StepState) + 8 * c(P.M);
-  b += 16 * MN * (P.c_unique.size() + 1);
-  if (any_stash) b += 16 * (size_t)P.rk.s * MN;
-  if (P.raman) b += (mixed ? 8 : 16) * c(L) * P.N + 16 * (size_t)P.N + 8 * (size_t)P.N;
+  size_t b = 16 * MN * (2 * F + 2 * Fw) + 8 * MN * F + 16 * (P.N2 + P.N1 + P.N) + sizeof(StepState) * F +
+             sizeof(FiberParams) * F + 8 * kHeadCoef * ((size_t)P.rk.s + 1) * F + 8 * c(P.M);
+  b += 16 * MN * F * (2 * P.c_unique.size() + 1);
+  if (any_stash) b += 16 * (size_t)P.rk.s * MN * F;
+  if (P.raman) b += (mixed ? 8 : 16) * c(L) * P.N * F + 16 * (size_t)P.N * F + 8 * (size_t)P.N;
   b += 4 * c(P.coff.size()) + 4 * c(P.nnz) + 8 * c(P.nnz) + 4 * 4 * c(P.P) + 4 * 3 * c(L);
   if (mixed) {
-    b += 8 * (2 * MN) + 8 * (P.N2 + P.N1 + P.N) + 4 * c(P.nnz);
-    if (P.raman) b += 8 * (size_t)P.N;
+    b += 8 * (2 * MN) * F + 8 * (P.N2 + P.N1 + P.N) + 4 * c(P.nnz);
+    if (P.raman) b += 8 * (size_t)P.N * F;
   }
   if (P.fused_chunks > 0 && !(opt.flags & MMP_FLAG_SEPARATE_COLUMNS))
     b += 4 * (P.fused_line.size() + P.fused_slot.size() + c(P.P));
-  if (dense_gather_chunk((int)P.M) > 0) b += (mixed ? 4 : 8) * (size_t)P.P * P.M * P.M;
+  if (dense_gather_chunk((int)P.M) > 0) b += (mixed ? 4 : 8) * (size_t)P.P * P.M * P.M * F;
   return b;
 }
 
@@ -393,73 +424,179 @@ void fill_info(const Problem& P, const mmp_options& opt, mmp_info* o) {
   o->ld = P.ld;
   o->max_beta2 = P.max_beta2;
   o->max_q = P.max_q;
-  o->device_bytes = planned_device_bytes(P, opt);
+  o->device_bytes = planned_device_bytes(P, opt, 1);
+  o->n_fibers = 1;
 }
 
-mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options* options, mmp_solver** out) {
-  if (!ctx || !out) return fail(MMP_ERR_INVALID_ARGUMENT, "ctx and out are required");
+// Fibers of one batch must share everything the kernels take from fiber 0's problem.
+bool same_structure(const Problem& a, const Problem& b) {
+  return a.M == b.M && a.N == b.N && a.N1 == b.N1 && a.rk.s == b.rk.s && a.raman == b.raman && a.P == b.P &&
+         a.stage_table == b.stage_table && a.stash_needed == b.stash_needed && a.c_unique == b.c_unique &&
+         a.slot_a == b.slot_a && a.slot_b == b.slot_b && a.slot_line == b.slot_line && a.slot_kind == b.slot_kind &&
+         a.line_a == b.line_a && a.line_b == b.line_b && a.line_self == b.line_self &&
+         a.line_slot0 == b.line_slot0 && a.fused_line == b.fused_line && a.fused_slot == b.fused_slot &&
+         a.fused_chunks == b.fused_chunks;
+}
+
+// Dense coupling blocks Qd[s][p][k] = sum of Q / max|Q| over the nonzeros (k, slot s) of Q^(p).
+std::vector<double> dense_blocks(const Problem& P) {
+  std::vector<double> qd((size_t)P.P * P.M * P.M, 0.);
+  for (uint32_t p = 0; p < P.M; p++)
+    for (uint32_t e = P.off[p]; e < P.off[p + 1]; e++) qd[((size_t)P.slot[e] * P.M + p) * P.M + P.k[e]] += P.q[e];
+  return qd;
+}
+
+// Uploads fiber f's own data (operators, couplings, Raman spectrum, initial state, scalars),
+// resets its z and Lawson tables, then frees the large host arrays of its problem.
+mmp_status load_fiber(mmp_solver* s, uint32_t f, Problem& Pf) {
+  hipStream_t stream = s->ctx->stream;
+  const Batch& b = s->batch;
+  const size_t MN = (size_t)Pf.M * Pf.N;
+  MMP_TRY(upload(s, s->imL + f * b.sf, Pf.imL_fourstep.data(), MN * 8));
+  MMP_TRY(upload(s, s->e1 + f * b.sf, Pf.e1.data(), MN * 16));
+  MMP_TRY(upload(s, s->etab0 + f * b.se, Pf.etab0.data(), Pf.etab0.size() * sizeof(double)));
+  MMP_TRY(upload(s, s->etab + f * b.se, Pf.etab0.data(), Pf.etab0.size() * sizeof(double)));
+  std::vector<double> qd;
+  std::vector<float> qd32;
+  if (s->qdense_w) {
+    qd = dense_blocks(Pf);
+    if (s->mixed) {
+      qd32.assign(qd.begin(), qd.end());
+      MMP_TRY(upload(s, static_cast<float*>(const_cast<void*>(s->qdense_w)) + f * b.sq, qd32.data(), qd32.size() * 4));
+    } else {
+      MMP_TRY(upload(s, s->qdense + f * b.sq, qd.data(), qd.size() * 8));
+    }
+  }
+  const FiberParams fp{Pf.delta_z, Pf.sign_gamma, Pf.delta_t, Pf.kv, Pf.kr};
+  MMP_TRY(upload(s, s->d_fp + f, &fp, sizeof fp));
+  // b_{i-1} h and a_ij h of every head, multiplied in FP64 on the host as the single solver does
+  std::vector<double> coef(b.nheads * kHeadCoef, 0.);
+  for (size_t i = 0; i < b.nheads; i++) {
+    const HeadArgs& a = s->heads[i];
+    coef[i * kHeadCoef] = a.b_prev != 0. ? a.b_prev * Pf.delta_z : 0.;
+    for (int e = 0; e < a.n_a; e++) coef[i * kHeadCoef + 1 + e] = a.a_coef[e] * Pf.delta_z;
+  }
+  MMP_TRY(upload(s, s->d_coef + f * b.nheads * kHeadCoef, coef.data(), coef.size() * sizeof(double)));
+  const StepState zero{0., 0u, 0u};
+  MMP_TRY(upload(s, s->st + f, &zero, sizeof zero));
+  if (Pf.raman) {
+    cd* hs = s->hspec + f * b.sh;
+    MMP_TRY(upload(s, s->hR, Pf.raman_response.data(), Pf.N * 8));
+    HIP_TRY(launch_col_forward_real(s->g, s->tw, s->hR, hs, stream), "col_forward_real");
+    HIP_TRY(launch_row_forward(s->g, s->tw, hs, 1, stream), "row_forward");
+    if (s->mixed) HIP_TRY(launch_narrow(hs, static_cast<cf*>(s->hspec_w) + f * b.sh, Pf.N, stream), "narrow");
+  }
+  MMP_TRY(upload(s, s->U, Pf.psi0.data(), MN * 16));
+  MMP_TRY(forward_state_from_U(s, f));
+  HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");  // host arrays above are staging copies
+  s->z_host[f] = 0.;
+  s->steps[f] = 0;
+  for (auto* v : {&Pf.imL_fourstep, &Pf.e1, &Pf.etab0, &Pf.psi0, &Pf.raman_response}) std::vector<double>().swap(*v);
+  return MMP_OK;
+}
+
+mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params, const mmp_options* options,
+                       mmp_solver** out) {
+  if (!ctx || !out || !params) return fail(MMP_ERR_INVALID_ARGUMENT, "ctx, params and out are required");
   *out = nullptr;
+  if (n_fibers < 1 || n_fibers > 65535) return fail(MMP_ERR_INVALID_ARGUMENT, "n_fibers must be in [1, 65535]");
   mmp_options opt{};
   if (options) opt = *options;
   mmp_solver* s = new (std::nothrow) mmp_solver();
   if (!s) return fail(MMP_ERR_OUT_OF_MEMORY, "host allocation failed");
   s->ctx = ctx;
   s->opt = opt;
+  s->F = n_fibers;
+  s->fib.resize(n_fibers);
   std::string err;
-  mmp_status stt = prepare_problem(params, opt, s->P, err);
-  if (stt != MMP_OK) {
-    delete s;
-    return fail(stt, err);
+  for (uint32_t f = 0; f < n_fibers; f++) {
+    mmp_status stt = prepare_problem(&params[f], opt, s->fib[f], err);
+    if (stt == MMP_OK && f > 0 && !same_structure(s->fib[0], s->fib[f])) {
+      stt = MMP_ERR_INVALID_ARGUMENT;
+      err = "fibers of one batch must share n_modes, nt, method_order, raman and the pair structure of their "
+            "coupling lists";
+    }
+    if (stt != MMP_OK) {
+      delete s;
+      return fail(stt, n_fibers > 1 ? "fiber " + std::to_string(f) + ": " + err : err);
+    }
+    s->time_window.push_back(params[f].time_window);
+    s->fiber_length.push_back(params[f].fiber_length);
   }
-  s->time_window = params->time_window;
-  s->fiber_length = params->fiber_length;
+  s->z_host.assign(n_fibers, 0.);
+  s->steps.assign(n_fibers, 0u);
+  s->P = s->fib[0];  // structure of the batch (fiber 0's large arrays are uploaded from s->fib[0])
+  for (auto* v : {&s->P.imL_fourstep, &s->P.e1, &s->P.etab0, &s->P.psi0, &s->P.raman_response})
+    std::vector<double>().swap(*v);
   s->mixed = opt.precision == MMP_PRECISION_MIXED;
   s->wbytes = s->mixed ? sizeof(cf) : sizeof(cd);
   s->profile = (opt.flags & MMP_FLAG_PROFILE) != 0;
   s->graphs = !s->profile && (opt.flags & MMP_FLAG_NO_GRAPH) == 0;
   Problem& P = s->P;
+  const uint32_t F = n_fibers;
   s->g = Geometry{(int)P.N, (int)P.N1, (int)P.N2, (int)P.M};
   s->n_lines = (int)P.line_a.size();
+  s->use_fused = P.fused_chunks > 0 && !(opt.flags & MMP_FLAG_SEPARATE_COLUMNS);
+  if (F > 1 && !s->use_fused) {
+    delete s;
+    return fail(MMP_ERR_UNSUPPORTED,
+                "batched fibers need the column-stage kernels: Raman on, nt in {4096, 8192, 16384}, n_modes <= 12, "
+                "and no MMP_FLAG_SEPARATE_COLUMNS");
+  }
   auto cleanup = [&](mmp_status st2) {
     release(s);
     return st2;
   };
   if (set_device(s) != MMP_OK) return cleanup(MMP_ERR_DEVICE);
   const size_t MN = (size_t)P.M * P.N;
-  // device buffers
+  s->ntab = P.c_unique.size();
+  const size_t conv_n = (size_t)(s->n_lines ? s->n_lines : 1) * P.N;
+  bool any_stash = false;
+  for (int j = 0; j < P.rk.s; j++) any_stash = any_stash || P.stash_needed[j];
+  const bool dense = dense_gather_chunk((int)P.M) > 0;
+  s->batch.sf = MN;
+  s->batch.sl = conv_n;
+  s->batch.se = s->ntab * MN;
+  s->batch.ss = (size_t)P.rk.s * MN;
+  s->batch.sq = (size_t)P.P * P.M * P.M;
+  s->batch.sh = P.N;
+  // device buffers: per-fiber arrays are [F][...]; ybuf and U are also the FP64 scratch of
+  // getResult, set_state and energy (their first M x NT values)
   cd *rowtw, *coltw, *fourtw;
   mmp_status st = MMP_OK;
   auto A = [&](mmp_status x) {
     if (st == MMP_OK) st = x;
   };
-  A(dalloc(s, &s->psi_hat, MN));
-  A(dalloc(s, &s->acc, MN));
-  A(dalloc(s, &s->ybuf, MN));
-  A(dalloc(s, &s->U, MN));
-  A(dalloc(s, &s->imL, MN));
-  A(dalloc(s, &s->etab, MN * P.c_unique.size()));
-  A(dalloc(s, &s->e1, MN));
+  const size_t Fw = s->mixed ? 1 : F;  // FP64 ybuf/U copies: the work arrays in FP64 mode, scratch in mixed
+  A(dalloc(s, &s->psi_hat, F * MN));
+  A(dalloc(s, &s->acc, F * MN));
+  A(dalloc(s, &s->ybuf, Fw * MN));
+  A(dalloc(s, &s->U, Fw * MN));
+  A(dalloc(s, &s->imL, F * MN));
+  A(dalloc(s, &s->etab, F * s->batch.se));
+  A(dalloc(s, &s->etab0, F * s->batch.se));
+  A(dalloc(s, &s->e1, F * MN));
   A(dalloc(s, &rowtw, P.N2));
   A(dalloc(s, &coltw, P.N1));
   A(dalloc(s, &fourtw, P.N));
-  A(dalloc(s, &s->st, 1));
+  A(dalloc(s, &s->st, F));
+  A(dalloc(s, &s->d_fp, F));
+  s->batch.nheads = (size_t)P.rk.s + 1;
+  A(dalloc(s, &s->d_coef, F * s->batch.nheads * kHeadCoef));
   A(dalloc(s, &s->energy, P.M));
-  bool any_stash = false;
-  for (int j = 0; j < P.rk.s; j++) any_stash = any_stash || P.stash_needed[j];
-  if (any_stash) A(dalloc(s, &s->stash, (size_t)P.rk.s * MN));
+  if (any_stash) A(dalloc(s, &s->stash, F * s->batch.ss));
   if (P.raman) {
-    A(dalloc(s, &s->hspec, P.N));
+    A(dalloc(s, &s->hspec, F * (size_t)P.N));
     A(dalloc(s, &s->hR, P.N));
   }
   cf *rowtw32 = nullptr, *coltw32 = nullptr, *fourtw32 = nullptr;
-  const size_t conv_n = (size_t)(s->n_lines ? s->n_lines : 1) * P.N;
   if (s->mixed) {
     cf *yw, *uw, *cw = nullptr, *hw = nullptr;
-    A(dalloc(s, &yw, MN));
-    A(dalloc(s, &uw, MN));
+    A(dalloc(s, &yw, F * MN));
+    A(dalloc(s, &uw, F * MN));
     if (P.raman) {
-      A(dalloc(s, &cw, conv_n));
-      A(dalloc(s, &hw, P.N));
+      A(dalloc(s, &cw, F * conv_n));
+      A(dalloc(s, &hw, F * (size_t)P.N));
     }
     A(dalloc(s, &rowtw32, P.N2));
     A(dalloc(s, &coltw32, P.N1));
@@ -468,12 +605,21 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
     s->U_w = uw;
     s->conv_w = cw;
     s->hspec_w = hw;
+    if (dense) {
+      float* d;
+      A(dalloc(s, &d, F * s->batch.sq));
+      s->qdense_w = d;
+    }
   } else {
-    if (P.raman) A(dalloc(s, &s->conv, conv_n));
+    if (P.raman) A(dalloc(s, &s->conv, F * conv_n));
     s->ybuf_w = s->ybuf;
     s->U_w = s->U;
     s->conv_w = s->conv;
     s->hspec_w = s->hspec;
+    if (dense) {
+      A(dalloc(s, &s->qdense, F * s->batch.sq));
+      s->qdense_w = s->qdense;
+    }
   }
   uint32_t *d_coff, *d_k, *d_sa, *d_sb, *d_sl, *d_sk, *d_la, *d_lb, *d_ls;
   double* d_q;
@@ -489,7 +635,7 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
   A(dalloc(s, &d_lb, L1));
   A(dalloc(s, &d_ls, L1));
   if (st != MMP_OK) return cleanup(st);
-  // tables (host copies stay alive until the stream synchronisation below)
+  // batch-wide tables (host copies stay alive until the stream synchronisation below)
   std::vector<uint32_t> kslot(P.nnz);
   std::vector<double> tr = twiddle_table(P.N2), tc = twiddle_table(P.N1), t4 = twiddle_table(P.N);
   A(upload(s, rowtw, tr.data(), tr.size() * 8));
@@ -502,9 +648,7 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
     A(upload(s, coltw32, tc32.data(), tc32.size() * 4));
     A(upload(s, fourtw32, t432.data(), t432.size() * 4));
   }
-  A(upload(s, s->imL, P.imL_fourstep.data(), MN * 8));
   A(upload(s, d_coff, P.coff.data(), P.coff.size() * 4));
-  A(upload(s, s->e1, P.e1.data(), P.e1.size() * sizeof(double)));
   if (P.nnz) {
     for (uint32_t e = 0; e < P.nnz; e++) kslot[e] = P.k[e] | (P.slot[e] << 8);
     A(upload(s, d_k, kslot.data(), P.nnz * 4));
@@ -533,28 +677,11 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
   s->gather = GatherArgs{d_coff, d_k, s->q_w, d_sa, d_sb, d_sl, d_sk, (int)P.P, P.gather_chunk,
                          P.gather_chunks, P.gather_cap, P.raman, P.kv, P.kr};
   s->pairs = PairLineArgs{d_la, d_lb, d_ls};
-  std::vector<double> qd;
-  std::vector<float> qd32;
-  if (dense_gather_chunk((int)P.M) > 0) {
-    qd.assign((size_t)P.P * P.M * P.M, 0.);
-    for (uint32_t p = 0; p < P.M; p++)
-      for (uint32_t e = P.off[p]; e < P.off[p + 1]; e++) qd[((size_t)P.slot[e] * P.M + p) * P.M + P.k[e]] += P.q[e];
-    if (s->mixed) {
-      float* d;
-      qd32.assign(qd.begin(), qd.end());
-      A(dalloc(s, &d, qd32.size()));
-      if (st != MMP_OK) return cleanup(st);
-      A(upload(s, d, qd32.data(), qd32.size() * sizeof(float)));
-      s->qdense_w = d;
-    } else {
-      A(dalloc(s, &s->qdense, qd.size()));
-      if (st != MMP_OK) return cleanup(st);
-      A(upload(s, s->qdense, qd.data(), qd.size() * sizeof(double)));
-      s->qdense_w = s->qdense;
-    }
-  }
+  s->batch.fp = s->d_fp;
+  s->batch.coef = s->d_coef;
+  s->batch.st = s->st;
   std::vector<uint32_t> slot_meta(P.P);
-  if (P.fused_chunks > 0 && !(opt.flags & MMP_FLAG_SEPARATE_COLUMNS)) {
+  if (s->use_fused) {
     uint32_t *d_cl, *d_cs, *d_meta;
     A(dalloc(s, &d_cl, P.fused_line.size()));
     A(dalloc(s, &d_cs, P.fused_slot.size()));
@@ -566,22 +693,20 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
     A(upload(s, d_cs, P.fused_slot.data(), P.fused_slot.size() * 4));
     A(upload(s, d_meta, slot_meta.data(), P.P * 4));
     s->fused = FusedArgs{d_cl, d_cs, d_meta, P.fused_chunks, P.fused_max_lines, P.fused_max_slots};
-    s->use_fused = true;
   }
-  // stage heads
-  const double h = P.delta_z;
+  // stage heads: tableau coefficients here, each fiber's h, sgn and tables in the kernel
   // the close head also builds the next step's stage 0 unless the stage-0 Lawson table is
   // advanced by that same head (no tableau here has c_i = 0 beyond stage 0)
   s->chain = P.table_last_head[P.stage_table[0]] < P.rk.s;
   for (int i = 0; i <= P.rk.s; i++) {
     HeadArgs a{};
+    a.head = i;
     a.ysig = s->ybuf_w;
     a.psi_hat = s->psi_hat;
     a.acc = s->acc;
     a.stash = s->stash;
     a.yout = s->ybuf_w;
     a.e1 = s->e1;
-    a.sgn = P.sign_gamma;
     a.scale_cur = 1.0;
     a.has_prev = i > 0;
     a.prev_stage = i - 1;
@@ -591,12 +716,11 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
       const int u = P.stage_table[i - 1];
       a.e_prev = s->etab + (size_t)u * MN;
       a.update_prev = (P.table_last_head[u] == i);
-      a.bh_prev = (P.rk.b[i - 1] != 0.) ? P.rk.b[i - 1] * h : 0.;
+      a.b_prev = P.rk.b[i - 1];
       a.stash_prev = P.stash_needed[i - 1];
     }
     if (i == P.rk.s) {
-      a.st = s->st;
-      a.h = P.delta_z;
+      a.advance_z = 1;
       if (s->chain) {
         a.chain_next = 1;
         a.e_cur = s->etab + (size_t)P.stage_table[0] * MN;
@@ -607,31 +731,17 @@ mmp_status create_impl(mmp_ctx* ctx, const mmp_params* params, const mmp_options
       for (int j = 0; j < i; j++)
         if (P.rk.a[i][j] != 0.) {
           a.a_idx[a.n_a] = j;
-          a.a_h[a.n_a] = P.rk.a[i][j] * h;
+          a.a_coef[a.n_a] = P.rk.a[i][j];
           a.n_a++;
         }
     }
     s->heads.push_back(a);
   }
-  // initial state and Raman spectrum
-  A(upload(s, s->U, P.psi0.data(), MN * 16));
-  A(forward_state_from_U(s));
-  if (P.raman) {
-    A(upload(s, s->hR, P.raman_response.data(), P.N * 8));
-    hipError_t e = launch_col_forward_real(s->g, s->tw, s->hR, s->hspec, ctx->stream);
-    if (e != hipSuccess) A(hip_fail(e, "col_forward_real"));
-    e = launch_row_forward(s->g, s->tw, s->hspec, 1, ctx->stream);
-    if (e != hipSuccess) A(hip_fail(e, "row_forward"));
-    if (s->mixed) {
-      e = launch_narrow(s->hspec, static_cast<cf*>(s->hspec_w), P.N, ctx->stream);
-      if (e != hipSuccess) A(hip_fail(e, "narrow"));
-    }
-  }
-  A(reset_z(s));
   if (st == MMP_OK) {
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) A(hip_fail(e, "hipStreamSynchronize"));
   }
+  for (uint32_t f = 0; f < F && st == MMP_OK; f++) A(load_fiber(s, f, s->fib[f]));
   if (st != MMP_OK) return cleanup(st);
   *out = s;
   return MMP_OK;
@@ -656,8 +766,16 @@ mmp_status advance_impl(mmp_solver* s, uint32_t n) {
     }
   }
   for (uint32_t k = 0; k < left; k++) MMP_TRY(enqueue_step(s));
-  for (uint32_t k = 0; k < n; k++) s->z_host += s->P.delta_z;  // z += h (src/MultimodePropagation_CPU.cpp:32)
-  s->steps_taken += n;
+  for (uint32_t f = 0; f < s->F; f++) {
+    for (uint32_t k = 0; k < n; k++) s->z_host[f] += s->fib[f].delta_z;  // z += h (src/MultimodePropagation_CPU.cpp:32)
+    s->steps[f] += n;
+  }
+  return MMP_OK;
+}
+
+mmp_status check_fiber(mmp_solver* s, uint32_t f) {
+  if (!s) return fail(MMP_ERR_INVALID_ARGUMENT, "solver is NULL");
+  if (f >= s->F) return fail(MMP_ERR_INVALID_ARGUMENT, "fiber index out of range");
   return MMP_OK;
 }
 
@@ -734,7 +852,18 @@ mmp_status mmp_ctx_synchronize(mmp_ctx* ctx) {
 mmp_status mmp_solver_create(mmp_ctx* ctx, const mmp_params* params, const mmp_options* options,
                              mmp_solver** out) {
   try {
-    return create_impl(ctx, params, options, out);
+    return create_impl(ctx, 1, params, options, out);
+  } catch (const std::bad_alloc&) {
+    return fail(MMP_ERR_OUT_OF_MEMORY, "host allocation failed");
+  } catch (...) {
+    return fail(MMP_ERR_DEVICE, "unexpected exception");
+  }
+}
+
+mmp_status mmp_batch_create(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params, const mmp_options* options,
+                            mmp_solver** out) {
+  try {
+    return create_impl(ctx, n_fibers, params, options, out);
   } catch (const std::bad_alloc&) {
     return fail(MMP_ERR_OUT_OF_MEMORY, "host allocation failed");
   } catch (...) {
@@ -761,10 +890,13 @@ mmp_status mmp_advance(mmp_solver* s, uint32_t n) {
   return advance_impl(s, n);
 }
 
-mmp_status mmp_get_result(mmp_solver* s, double* out) {
-  if (!s || !out) return fail(MMP_ERR_INVALID_ARGUMENT, "solver and fields_out are required");
+mmp_status mmp_get_result(mmp_solver* s, double* out) { return mmp_fiber_result(s, 0, out); }
+
+mmp_status mmp_fiber_result(mmp_solver* s, uint32_t f, double* out) {
+  MMP_TRY(check_fiber(s, f));
+  if (!out) return fail(MMP_ERR_INVALID_ARGUMENT, "fields_out is required");
   MMP_TRY(set_device(s));
-  MMP_TRY(inverse_state_to_U(s, true));
+  MMP_TRY(inverse_state_to_U(s, f, true));
   MMP_TRY(read_fields(s, out));
   const size_t n = 2 * (size_t)s->P.M * s->P.N;
   for (size_t i = 0; i < n; i++)
@@ -779,7 +911,7 @@ mmp_status mmp_propagate_snapshots(mmp_solver* s, uint32_t n_steps, uint32_t n_s
   if (mode != MMP_SNAPSHOTS_REFERENCE_RESTART && mode != MMP_SNAPSHOTS_RESUMABLE)
     return fail(MMP_ERR_INVALID_ARGUMENT, "unknown snapshot mode");
   const uint32_t nz = n_steps / n_save;
-  const double dZ = s->fiber_length / n_steps;
+  const double dZ = s->fiber_length[0] / n_steps;
   const size_t stride = 2 * (size_t)s->P.M * s->P.N;
   double Z = 0.;
   for (uint32_t i = 0; i < n_save; i++) {
@@ -793,47 +925,82 @@ mmp_status mmp_propagate_snapshots(mmp_solver* s, uint32_t n_steps, uint32_t n_s
   return MMP_OK;
 }
 
-mmp_status mmp_get_state(mmp_solver* s, double* psi_out, double* z_out) {
-  if (!s) return fail(MMP_ERR_INVALID_ARGUMENT, "solver is NULL");
+mmp_status mmp_get_state(mmp_solver* s, double* psi_out, double* z_out) { return mmp_fiber_state(s, 0, psi_out, z_out); }
+
+mmp_status mmp_fiber_state(mmp_solver* s, uint32_t f, double* psi_out, double* z_out) {
+  MMP_TRY(check_fiber(s, f));
   MMP_TRY(set_device(s));
-  if (z_out) *z_out = s->z_host;
+  if (z_out) *z_out = s->z_host[f];
   if (psi_out) {
-    MMP_TRY(inverse_state_to_U(s, false));
+    MMP_TRY(inverse_state_to_U(s, f, false));
     MMP_TRY(read_fields(s, psi_out));
   }
   return MMP_OK;
 }
 
-mmp_status mmp_set_state(mmp_solver* s, const double* psi) {
-  if (!s || !psi) return fail(MMP_ERR_INVALID_ARGUMENT, "solver and psi are required");
+mmp_status mmp_set_state(mmp_solver* s, const double* psi) { return mmp_fiber_set_state(s, 0, psi); }
+
+mmp_status mmp_fiber_set_state(mmp_solver* s, uint32_t f, const double* psi) {
+  MMP_TRY(check_fiber(s, f));
+  if (!psi) return fail(MMP_ERR_INVALID_ARGUMENT, "psi is required");
   const size_t n = 2 * (size_t)s->P.M * s->P.N;
   for (size_t i = 0; i < n; i++)
     if (!std::isfinite(psi[i])) return fail(MMP_ERR_NONFINITE, "psi is not finite");
   MMP_TRY(set_device(s));
   MMP_TRY(upload(s, s->U, psi, n * 8));
-  MMP_TRY(forward_state_from_U(s));
-  MMP_TRY(reset_z(s));
+  MMP_TRY(forward_state_from_U(s, f));
+  MMP_TRY(reset_fiber(s, f));
   HIP_TRY(hipStreamSynchronize(s->ctx->stream), "hipStreamSynchronize");
   return MMP_OK;
 }
 
-mmp_status mmp_energy(mmp_solver* s, double* energy_out) {
-  if (!s || !energy_out) return fail(MMP_ERR_INVALID_ARGUMENT, "solver and energy_out are required");
+mmp_status mmp_fiber_load(mmp_solver* s, uint32_t f, const mmp_params* params) {
+  MMP_TRY(check_fiber(s, f));
+  if (!params) return fail(MMP_ERR_INVALID_ARGUMENT, "params is required");
+  try {
+    Problem Pf;
+    std::string err;
+    const mmp_status st = prepare_problem(params, s->opt, Pf, err);
+    if (st != MMP_OK) return fail(st, err);
+    if (!same_structure(s->P, Pf))
+      return fail(MMP_ERR_INVALID_ARGUMENT, "the fiber does not share the batch's n_modes, nt, method_order, raman "
+                                            "and pair structure");
+    MMP_TRY(set_device(s));
+    HIP_TRY(hipStreamSynchronize(s->ctx->stream), "hipStreamSynchronize");  // queued steps read the old fiber
+    MMP_TRY(load_fiber(s, f, Pf));
+    s->fib[f] = std::move(Pf);
+    s->time_window[f] = params->time_window;
+    s->fiber_length[f] = params->fiber_length;
+    return MMP_OK;
+  } catch (const std::bad_alloc&) {
+    return fail(MMP_ERR_OUT_OF_MEMORY, "host allocation failed");
+  }
+}
+
+mmp_status mmp_energy(mmp_solver* s, double* energy_out) { return mmp_fiber_energy(s, 0, energy_out); }
+
+mmp_status mmp_fiber_energy(mmp_solver* s, uint32_t f, double* energy_out) {
+  MMP_TRY(check_fiber(s, f));
+  if (!energy_out) return fail(MMP_ERR_INVALID_ARGUMENT, "energy_out is required");
   MMP_TRY(set_device(s));
-  MMP_TRY(inverse_state_to_U(s, true));
-  HIP_TRY(launch_energy(s->g, s->U, s->energy, s->time_window / s->P.N, s->ctx->stream), "energy");
+  MMP_TRY(inverse_state_to_U(s, f, true));
+  HIP_TRY(launch_energy(s->g, s->U, s->energy, s->time_window[f] / s->P.N, s->ctx->stream), "energy");
   HIP_TRY(hipMemcpyAsync(energy_out, s->energy, sizeof(double) * s->P.M, hipMemcpyDeviceToHost, s->ctx->stream),
           "hipMemcpyAsync D2H");
   HIP_TRY(hipStreamSynchronize(s->ctx->stream), "hipStreamSynchronize");
   return MMP_OK;
 }
 
-mmp_status mmp_solver_info(mmp_solver* s, mmp_info* o) {
-  if (!s || !o) return fail(MMP_ERR_INVALID_ARGUMENT, "solver and info_out are required");
-  fill_info(s->P, s->opt, o);
-  o->steps_taken = s->steps_taken;
-  o->z = s->z_host;
+mmp_status mmp_solver_info(mmp_solver* s, mmp_info* o) { return mmp_fiber_info(s, 0, o); }
+
+mmp_status mmp_fiber_info(mmp_solver* s, uint32_t f, mmp_info* o) {
+  MMP_TRY(check_fiber(s, f));
+  if (!o) return fail(MMP_ERR_INVALID_ARGUMENT, "info_out is required");
+  fill_info(s->fib[f], s->opt, o);
+  o->steps_taken = s->steps[f];
+  o->z = s->z_host[f];
   o->device_bytes = s->device_bytes;
+  o->n_fibers = s->F;
   return MMP_OK;
 }
 
