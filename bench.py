@@ -84,6 +84,11 @@ def main() -> int:
                     help="every column pass a kernel of its own (7 per stage instead of 4)")
     ap.add_argument("--mixed", action="store_true",
                     help="mixed precision (FP32 transforms and sums, FP64 state); the bench line is FP64 by default")
+    ap.add_argument("--workload", choices=["grin6", "batched"], default="grin6",
+                    help="grin6: the 6-mode GRIN fiber in FP64 (default); batched: 256 independent GRIN fibers "
+                         "(M in {3, 6, 10}, NT in {4096, 8192, 16384}) in batches per (M, NT)")
+    ap.add_argument("--fibers", type=int, default=256, help="batched workload: number of fibers")
+    ap.add_argument("--fp64-batch", action="store_true", help="batched workload in FP64 (default: mixed)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the bench, copy this many MiB device to device once with the L2 evicted first "
                          "(the reference dispatch for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -92,6 +97,8 @@ def main() -> int:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
                           "(independent fibers shard across GPUs with no exchange)"}))
         return 2
+    if args.workload == "batched":
+        return run_batched(args)
 
     from mm_propagation_amd import _capi, configs
     from mm_propagation_amd.solver import Context, MultimodePropagationGPU_RamanON
@@ -175,6 +182,118 @@ def main() -> int:
     ctx.close()
     if args.copy_calibration_mib > 0:
         copy_calibration(args.copy_calibration_mib)
+    return 0
+
+
+def run_batched(args) -> int:
+    """256 independent fibers (configs.batched_fiber(i), i < --fibers) grouped by (M, NT) into
+    batched solvers, one stream each, stepped together; a step is one RK4 step of every fiber.
+    Mixed precision (FP64 with --fp64-batch)."""
+    from mm_propagation_amd import _capi, configs
+    from mm_propagation_amd.batch import MultimodePropagationBatch
+    from mm_propagation_amd.solver import Context, MultimodePropagationGPU_RamanON
+
+    prec = _capi.MMP_PRECISION_FP64 if args.fp64_batch else _capi.MMP_PRECISION_MIXED
+    fibers = [configs.batched_fiber(i) for i in range(args.fibers)]
+    classes = {}
+    for i, sp in enumerate(fibers):
+        classes.setdefault((sp.n_modes, sp.nt), []).append(i)
+    batches = []
+    for key in sorted(classes):
+        ctx = Context(0)
+        b = MultimodePropagationBatch([fibers[i] for i in classes[key]], pair_decode=_capi.MMP_PAIRS_PHYSICAL,
+                                      precision=prec, ctx=ctx)
+        batches.append((key, classes[key], ctx, b))
+    units = sum(sp.n_modes * sp.nt for sp in fibers)  # M * NT summed over the fibers, per step
+    chunk = 16
+    steps = max(chunk, (args.steps // chunk) * chunk)
+
+    def run(n):
+        for _ in range(n // chunk):
+            for _, _, _, b in batches:
+                b.advance(chunk)
+        for _, _, ctx, _ in batches:
+            ctx.synchronize()
+
+    run(max(chunk, (args.warmup // chunk) * chunk))
+    for _, _, _, b in batches:
+        b.computeLawsonRK(0)  # every fiber back to z = 0 (fields keep their warmed-up values)
+    t0 = time.perf_counter()
+    run(steps)
+    elapsed = time.perf_counter() - t0
+    value = units * steps / elapsed
+
+    # the plain FP64 single-fiber solver on one fiber of each class, timed alone; the batch's
+    # fibers solved one after another at those rates
+    plain_time = 0.0
+    plain = {}
+    for key, idx, ctx, _ in batches:
+        sp = fibers[idx[0]]
+        s1 = MultimodePropagationGPU_RamanON(sp, pair_decode=_capi.MMP_PAIRS_PHYSICAL, ctx=ctx)
+        s1.advance(32)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        s1.advance(160)
+        ctx.synchronize()
+        per_step = (time.perf_counter() - t1) / 160
+        s1.close()
+        plain[f"M{key[0]}_NT{key[1]}"] = round(per_step * 1e6, 2)
+        plain_time += per_step * len(idx)
+    plain_value = units / plain_time
+
+    # accuracy: one fiber per mode count, batched result vs the FP64 oracle after 100 steps
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_binding as ob
+    acc = {}
+    if not args.no_cpu_baseline:
+        for key, idx, ctx, b in batches:
+            if f"M{key[0]}" in {k.split("_")[0] for k in acc} or key[1] != 4096:
+                continue
+            sp = fibers[idx[0]]
+            s1 = MultimodePropagationBatch([sp], pair_decode=_capi.MMP_PAIRS_PHYSICAL, precision=prec, ctx=ctx)
+            s1.computeLawsonRK(100)
+            got = s1.result(0)
+            s1.close()
+            o = ob.Oracle(sp, pair_decode=1, n_threads=min(16, os.cpu_count() or 1))
+            o.compute_lawson_rk(100)
+            err = configs.rel_l2_per_mode(got, o.get_result())
+            o.close()
+            acc[f"M{key[0]}_NT{key[1]}_fiber{idx[0]}"] = float(err.max())
+
+    # roofline of the dominant kernel in the largest class, with dispatch-recorded events
+    key, idx, ctx, _ = max(batches, key=lambda t: t[0][0] * t[0][1] * len(t[1]))
+    pb = MultimodePropagationBatch([fibers[i] for i in idx], pair_decode=_capi.MMP_PAIRS_PHYSICAL, precision=prec,
+                                   flags=_capi.MMP_FLAG_PROFILE, ctx=ctx)
+    pb.advance(4)
+    pb.kernel_times()
+    pb.advance(32)
+    kt = pb.kernel_times()
+    pb.close()
+    dom = max(kt, key=lambda k: k["total_ms"])
+    avg_s = dom["total_ms"] / 1e3 / max(dom["launches"], 1)
+    achieved = dom["bytes_per_launch"] / avg_s / 1e9
+    roofline = {"bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "class": f"M={key[0]} NT={key[1]} x {len(idx)} fibers", "avg_launch_us": round(avg_s * 1e6, 3),
+                "kernels": {k["name"]: round(k["total_ms"] * 1e3 / max(k["launches"], 1), 3) for k in kt}}
+    out = {
+        "metric": METRIC, "value": value, "unit": "M·NT·steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if args.fp64_batch else "f32+f64", "data": "synthetic",
+        "config": {"workload": f"batched{args.fibers}: fibers configs.batched_fiber(0..{args.fibers - 1}) "
+                               "(M in {3,6,10}, NT in {4096,8192,16384}, L U(0.2,1) m, T0 U(0.05,0.3) ps, "
+                               "P log-uniform [10 W, 10 kW], symmetric GRIN coupling, RK4, Raman on), "
+                               f"{len(batches)} batches by (M, NT)",
+                   "global_batch": args.fibers, "seq_len": "4096-16384", "parallelism": "single GPU"},
+        "plain_fp64_single_fiber": {"value": plain_value, "per_step_us": plain,
+                                    "ratio": value / plain_value},
+        "accuracy": {"rel_l2_vs_fp64_oracle_after_100_steps_max": acc},
+        "roofline": roofline,
+    }
+    print(json.dumps(out), flush=True)
+    for _, _, ctx, b in batches:
+        b.close()
+        ctx.close()
     return 0
 
 
