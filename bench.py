@@ -244,21 +244,38 @@ def run_batched(args) -> int:
     # accuracy: one fiber per mode count, batched result vs the FP64 oracle after 100 steps
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import oracle_binding as ob
+    # fibers whose field is no longer finite after the timed steps (the workload's 10 kW pulses
+    # make the nonlinear length comparable to h for the most powerful fibers)
+    nonfinite = 0
+    for _, idx, _, b in batches:
+        for f in range(len(idx)):
+            try:
+                b.result(f)
+            except _capi.MmpError as e:
+                if e.status != _capi.MMP_ERR_NONFINITE:
+                    raise
+                nonfinite += 1
     acc = {}
     if not args.no_cpu_baseline:
         for key, idx, ctx, b in batches:
-            if f"M{key[0]}" in {k.split("_")[0] for k in acc} or key[1] != 4096:
+            if key[1] != 4096:
                 continue
-            sp = fibers[idx[0]]
-            s1 = MultimodePropagationBatch([sp], pair_decode=_capi.MMP_PAIRS_PHYSICAL, precision=prec, ctx=ctx)
-            s1.computeLawsonRK(100)
-            got = s1.result(0)
-            s1.close()
-            o = ob.Oracle(sp, pair_decode=1, n_threads=min(16, os.cpu_count() or 1))
-            o.compute_lawson_rk(100)
-            err = configs.rel_l2_per_mode(got, o.get_result())
-            o.close()
-            acc[f"M{key[0]}_NT{key[1]}_fiber{idx[0]}"] = float(err.max())
+            for i in idx[:6]:  # the first fiber of the class whose 100-step field stays finite
+                sp = fibers[i]
+                s1 = MultimodePropagationBatch([sp], pair_decode=_capi.MMP_PAIRS_PHYSICAL, precision=prec, ctx=ctx)
+                s1.computeLawsonRK(100)
+                try:
+                    got = s1.result(0)
+                except _capi.MmpError:
+                    continue
+                finally:
+                    s1.close()
+                o = ob.Oracle(sp, pair_decode=1, n_threads=min(16, os.cpu_count() or 1))
+                o.compute_lawson_rk(100)
+                err = configs.rel_l2_per_mode(got, o.get_result())
+                o.close()
+                acc[f"M{key[0]}_NT{key[1]}_fiber{i}"] = float(err.max())
+                break
 
     # roofline of the dominant kernel in the largest class, with dispatch-recorded events
     key, idx, ctx, _ = max(batches, key=lambda t: t[0][0] * t[0][1] * len(t[1]))
@@ -287,7 +304,8 @@ def run_batched(args) -> int:
                    "global_batch": args.fibers, "seq_len": "4096-16384", "parallelism": "single GPU"},
         "plain_fp64_single_fiber": {"value": plain_value, "per_step_us": plain,
                                     "ratio": value / plain_value},
-        "accuracy": {"rel_l2_vs_fp64_oracle_after_100_steps_max": acc},
+        "accuracy": {"rel_l2_vs_fp64_oracle_after_100_steps_max": acc,
+                     "fibers_nonfinite_after_timed_steps": nonfinite},
         "roofline": roofline,
     }
     print(json.dumps(out), flush=True)
