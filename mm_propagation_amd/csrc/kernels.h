@@ -152,8 +152,10 @@ struct PairLineArgs {
 struct FusedArgs {
   const uint32_t* chunk_line;  // [n_chunks + 1]
   const uint32_t* chunk_slot;  // [n_chunks + 1]
-  const uint32_t* slot_meta;   // [P] a | b << 8 | kind << 16 | line << 18 of each slot
+  const uint32_t* slot_meta;   // [P] a | b << 8 | kind << 16 | parity << 18 | line << 19 of each slot
   int n_chunks, max_chunk_lines, max_chunk_slots;
+  int parity;                  // coupling blocks hold only the entries (p, k) with (p + k) % 2 == the
+                               // slot's parity: M x ceil(M/2) per slot, k = 2 kk + ((p + parity) % 2)
 };
 
 // Limits of one column-stage chunk: Raman lines, slots, and coupling coefficients

@@ -17,6 +17,8 @@
 
 #include <hip/hip_ext.h>
 
+#include <type_traits>
+
 namespace mmp {
 
 thread_local LaunchEvents g_launch_events;
@@ -704,7 +706,7 @@ __global__ void __launch_bounds__(256) k_colstage_fields_pairs(Geometry g, const
 
 // NW wavefronts (4, or 8 when M <= 6); each line transform runs inside one wavefront,
 // in place in its padded LDS row, and wavefronts without a line skip it.
-template <int M, int NW, class W>
+template <int M, int NW, class W, bool PAR>
 __global__ void __launch_bounds__(NW * 64) k_colstage_raman_gather(Geometry g, const cpx<W>* __restrict__ coltw,
                                                                    const cpx<W>* __restrict__ fourtw,
                                                                    const cpx<W>* __restrict__ U,
@@ -722,7 +724,8 @@ __global__ void __launch_bounds__(NW * 64) k_colstage_raman_gather(Geometry g, c
   constexpr int L = kColStageL, T = L / 8, RS = kColStageRS;
   constexpr int NT = NW * 64;
   constexpr int JOBS = NT / T;  // line slots per round
-  constexpr int MM = M * M;
+  constexpr int KH = (M + 1) / 2;
+  constexpr int MM = PAR ? M * KH : M * M;  // coupling coefficients per slot
   constexpr int UR = (M + 7) / 8 * 8;  // field rows, rounded up to whole wavefronts of lines
   constexpr int UIN = (M * L + NT - 1) / NT;
   constexpr int RCAP = kColStageRRows * RS;
@@ -818,7 +821,7 @@ __global__ void __launch_bounds__(NW * 64) k_colstage_raman_gather(Geometry g, c
     // imaginary part; everything but the sample is wave-uniform and read from LDS by broadcast
     for (int sl = s0 + wave; sl < s1; sl += NW) {
       const uint32_t meta = Msh[sl - s0];
-      const int pa = meta & 255u, pb = (meta >> 8) & 255u, kind = (meta >> 16) & 3u, line = (int)(meta >> 18);
+      const int pa = meta & 255u, pb = (meta >> 8) & 255u, kind = (meta >> 16) & 3u, line = (int)(meta >> 19);
       const W* qb = Qsh + (sl - s0) * MM;
       const cw r = Rsh[(line - l0) * RS + lane];
       const cw V = cmulc(Ush[pa * RS + lane], Ush[pb * RS + lane]);
@@ -833,12 +836,31 @@ __global__ void __launch_bounds__(NW * 64) k_colstage_raman_gather(Geometry g, c
       cw y[M];
 #pragma unroll
       for (int k = 0; k < M; k++) y[k] = cmul(X, u[k]);
+      if constexpr (PAR) {
+        // only k with (p + k) % 2 == the slot's parity: q[p][kk] belongs to k = 2 kk + ((p + par) % 2)
+        auto checker = [&](auto par_c) {
+          constexpr int par = decltype(par_c)::value;
 #pragma unroll
-      for (int p = 0; p < M; p++) {
-        cw tsum = acc[p];
+          for (int p = 0; p < M; p++) {
+            cw tsum = acc[p];
 #pragma unroll
-        for (int k = 0; k < M; k++) tsum = cadd(tsum, cscale(y[k], qb[p * M + k]));
-        acc[p] = tsum;
+            for (int kk = 0; kk < KH; kk++) {
+              const int k = 2 * kk + ((p + par) & 1);
+              if (k < M) tsum = cadd(tsum, cscale(y[k], qb[p * KH + kk]));
+            }
+            acc[p] = tsum;
+          }
+        };
+        if ((meta >> 18) & 1u) checker(std::integral_constant<int, 1>{});
+        else checker(std::integral_constant<int, 0>{});
+      } else {
+#pragma unroll
+        for (int p = 0; p < M; p++) {
+          cw tsum = acc[p];
+#pragma unroll
+          for (int k = 0; k < M; k++) tsum = cadd(tsum, cscale(y[k], qb[p * M + k]));
+          acc[p] = tsum;
+        }
       }
     }
   }
@@ -1137,14 +1159,20 @@ hipError_t launch_colstage_fields_pairs(const Geometry& g, const Twiddles& tw, c
   return hipGetLastError();
 }
 
+#define MMP_COLSTAGE_LAUNCH(MV, PARV)                                                                             \
+  if constexpr (MV <= 6)                                                                                          \
+    MMP_LAUNCH((k_colstage_raman_gather<MV, 8, W, PARV>), dim3(g.N2, F), dim3(512), 0, s, g, TwOf<W>::col(tw),   \
+               TwOf<W>::four(tw), U, conv, yout, ga, fa, Qd, scale, b);                                           \
+  else                                                                                                            \
+    MMP_LAUNCH((k_colstage_raman_gather<MV, 4, W, PARV>), dim3(g.N2, F), dim3(256), 0, s, g, TwOf<W>::col(tw),   \
+               TwOf<W>::four(tw), U, conv, yout, ga, fa, Qd, scale, b);
 #define MMP_COLSTAGE_CASE(MV)                                                                                    \
   case MV:                                                                                                        \
-    if constexpr (MV <= 6)                                                                                        \
-      MMP_LAUNCH((k_colstage_raman_gather<MV, 8, W>), dim3(g.N2, F), dim3(512), 0, s, g, TwOf<W>::col(tw),       \
-                         TwOf<W>::four(tw), U, conv, yout, ga, fa, Qd, scale, b);                                 \
-    else                                                                                                          \
-      MMP_LAUNCH((k_colstage_raman_gather<MV, 4, W>), dim3(g.N2, F), dim3(256), 0, s, g, TwOf<W>::col(tw),       \
-                         TwOf<W>::four(tw), U, conv, yout, ga, fa, Qd, scale, b);                                 \
+    if (fa.parity) {                                                                                              \
+      MMP_COLSTAGE_LAUNCH(MV, true)                                                                               \
+    } else {                                                                                                      \
+      MMP_COLSTAGE_LAUNCH(MV, false)                                                                              \
+    }                                                                                                             \
     break;
 
 template <class W>
@@ -1152,9 +1180,10 @@ hipError_t launch_colstage_raman_gather(const Geometry& g, const Twiddles& tw, c
                                         cpx<W>* yout, const GatherArgs& ga, const FusedArgs& fa, const W* Qd,
                                         double scale, const Batch& b, int F, hipStream_t s) {
   if (g.N1 != kColStageL || F < 1 || F > 65535) return hipErrorInvalidValue;
+  const int block = fa.parity ? g.M * ((g.M + 1) / 2) : g.M * g.M;
   if (fa.n_chunks < 1 || fa.max_chunk_lines < 1 || fa.max_chunk_lines > colstage_lines_per_chunk() ||
       fa.max_chunk_slots < 1 || fa.max_chunk_slots > colstage_slots_per_chunk() ||
-      fa.max_chunk_slots * g.M * g.M > colstage_coefficients_per_chunk())
+      fa.max_chunk_slots * block > colstage_coefficients_per_chunk())
     return hipErrorInvalidValue;
   switch (g.M) {
     MMP_COLSTAGE_CASE(1) MMP_COLSTAGE_CASE(2) MMP_COLSTAGE_CASE(3) MMP_COLSTAGE_CASE(4) MMP_COLSTAGE_CASE(5)

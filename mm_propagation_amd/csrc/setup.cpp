@@ -371,6 +371,25 @@ mmp_status prepare_problem(const mmp_params* in, const mmp_options& opt, Problem
       P.q[e] = in->coupling_values[src] / max_q;
     }
   }
+  // parity structure (zero coupling values do not count)
+  P.slot_parity.assign(P.P, 0);
+  {
+    std::vector<int> par(P.P, -1);
+    bool ok = true;
+    for (uint32_t p = 0; p < M && ok; p++)
+      for (uint32_t e = P.off[p]; e < P.off[p + 1]; e++) {
+        if (P.q[e] == 0.) continue;
+        const int v = (int)((p + P.k[e]) & 1u);
+        int& ps = par[P.slot[e]];
+        if (ps < 0) ps = v;
+        else if (ps != v) {
+          ok = false;
+          break;
+        }
+      }
+    P.parity = ok;
+    for (uint32_t j = 0; j < P.P; j++) P.slot_parity[j] = (uint8_t)(par[j] > 0 ? 1 : 0);
+  }
   gather_plan((int)M, (int)P.P, P.gather_cap, P.gather_chunk);
   if (!P.gather_cap || P.gather_chunk <= 0) {
     err = "the Kerr gather tile does not fit (n_modes too large)";
@@ -400,7 +419,8 @@ mmp_status prepare_problem(const mmp_params* in, const mmp_options& opt, Problem
   if (P.raman && P.N1 == 64 && (int)M <= colstage_max_modes() && dense_gather_chunk((int)M) > 0 && nl > 0 &&
       P.line_slot0.size() == (size_t)nl + 1 && P.line_slot0[0] == 0 && P.line_slot0[nl] == P.P) {
     const int max_lines = colstage_lines_per_chunk();
-    const int max_slots = std::min(colstage_slots_per_chunk(), colstage_coefficients_per_chunk() / (int)(M * M));
+    const int block = P.parity ? (int)(M * ((M + 1) / 2)) : (int)(M * M);  // coefficients per slot
+    const int max_slots = std::min(colstage_slots_per_chunk(), colstage_coefficients_per_chunk() / block);
     bool ok = max_slots >= 2;
     P.fused_line.push_back(0);
     P.fused_slot.push_back(0);

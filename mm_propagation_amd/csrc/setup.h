@@ -51,6 +51,11 @@ struct Problem {
   // Raman convolution lines
   std::vector<uint32_t> line_a, line_b, line_self;  // [n_lines]
   std::vector<uint32_t> line_slot0;                 // [n_lines + 1] first device slot of each line
+  // parity structure of the coupling: every nonzero (p, k, slot s) has (p + k) % 2 == slot_parity[s]
+  // (GRIN tensors: Q^(p)_klm != 0 only for k + l - m - p even), so each slot's M x M block is
+  // half zeros in a fixed checkerboard
+  bool parity = false;
+  std::vector<uint8_t> slot_parity;  // [P]
   // column-stage kernels: chunks of Raman lines (0 chunks: not eligible), their line and slot
   // bounds, and the largest chunk's line and slot counts
   int fused_chunks = 0, fused_max_lines = 0, fused_max_slots = 0;

@@ -400,7 +400,9 @@ size_t planned_device_bytes(const Problem& P, const mmp_options& opt, size_t F) 
   }
   if (P.fused_chunks > 0 && !(opt.flags & MMP_FLAG_SEPARATE_COLUMNS))
     b += 4 * (P.fused_line.size() + P.fused_slot.size() + c(P.P));
-  if (dense_gather_chunk((int)P.M) > 0) b += (mixed ? 4 : 8) * (size_t)P.P * P.M * P.M * F;
+  const bool fused = P.fused_chunks > 0 && !(opt.flags & MMP_FLAG_SEPARATE_COLUMNS);
+  const size_t block = (fused && P.parity) ? (size_t)P.M * ((P.M + 1) / 2) : (size_t)P.M * P.M;
+  if (dense_gather_chunk((int)P.M) > 0) b += (mixed ? 4 : 8) * (size_t)P.P * block * F;
   return b;
 }
 
@@ -435,15 +437,25 @@ bool same_structure(const Problem& a, const Problem& b) {
          a.slot_a == b.slot_a && a.slot_b == b.slot_b && a.slot_line == b.slot_line && a.slot_kind == b.slot_kind &&
          a.line_a == b.line_a && a.line_b == b.line_b && a.line_self == b.line_self &&
          a.line_slot0 == b.line_slot0 && a.fused_line == b.fused_line && a.fused_slot == b.fused_slot &&
-         a.fused_chunks == b.fused_chunks;
+         a.fused_chunks == b.fused_chunks && a.parity == b.parity && a.slot_parity == b.slot_parity;
 }
 
-// Dense coupling blocks Qd[s][p][k] = sum of Q / max|Q| over the nonzeros (k, slot s) of Q^(p).
-std::vector<double> dense_blocks(const Problem& P) {
+// Dense coupling blocks Qd[s][p][k] = sum of Q / max|Q| over the nonzeros (k, slot s) of Q^(p);
+// compact (parity-structured couplings): only k = 2 kk + ((p + parity_s) % 2), as [s][p][kk].
+std::vector<double> dense_blocks(const Problem& P, bool compact) {
   std::vector<double> qd((size_t)P.P * P.M * P.M, 0.);
   for (uint32_t p = 0; p < P.M; p++)
     for (uint32_t e = P.off[p]; e < P.off[p + 1]; e++) qd[((size_t)P.slot[e] * P.M + p) * P.M + P.k[e]] += P.q[e];
-  return qd;
+  if (!compact) return qd;
+  const uint32_t KH = (P.M + 1) / 2;
+  std::vector<double> qc((size_t)P.P * P.M * KH, 0.);
+  for (uint32_t s = 0; s < P.P; s++)
+    for (uint32_t p = 0; p < P.M; p++)
+      for (uint32_t kk = 0; kk < KH; kk++) {
+        const uint32_t k = 2 * kk + ((p + P.slot_parity[s]) & 1u);
+        if (k < P.M) qc[((size_t)s * P.M + p) * KH + kk] = qd[((size_t)s * P.M + p) * P.M + k];
+      }
+  return qc;
 }
 
 // Uploads fiber f's own data (operators, couplings, Raman spectrum, initial state, scalars),
@@ -459,7 +471,7 @@ mmp_status load_fiber(mmp_solver* s, uint32_t f, Problem& Pf) {
   std::vector<double> qd;
   std::vector<float> qd32;
   if (s->qdense_w) {
-    qd = dense_blocks(Pf);
+    qd = dense_blocks(Pf, s->fused.parity != 0);
     if (s->mixed) {
       qd32.assign(qd.begin(), qd.end());
       MMP_TRY(upload(s, static_cast<float*>(const_cast<void*>(s->qdense_w)) + f * b.sq, qd32.data(), qd32.size() * 4));
@@ -558,7 +570,9 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
   s->batch.sl = conv_n;
   s->batch.se = s->ntab * MN;
   s->batch.ss = (size_t)P.rk.s * MN;
-  s->batch.sq = (size_t)P.P * P.M * P.M;
+  const bool compact = s->use_fused && P.parity;
+  s->batch.sq = (size_t)P.P * P.M * (compact ? (P.M + 1) / 2 : P.M);
+  s->fused.parity = compact ? 1 : 0;
   s->batch.sh = P.N;
   // device buffers: per-fiber arrays are [F][...]; ybuf and U are also the FP64 scratch of
   // getResult, set_state and energy (their first M x NT values)
@@ -688,11 +702,12 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
     A(dalloc(s, &d_meta, P1));
     if (st != MMP_OK) return cleanup(st);
     for (uint32_t j = 0; j < P.P; j++)
-      slot_meta[j] = P.slot_a[j] | (P.slot_b[j] << 8) | (P.slot_kind[j] << 16) | (P.slot_line[j] << 18);
+      slot_meta[j] = P.slot_a[j] | (P.slot_b[j] << 8) | (P.slot_kind[j] << 16) | ((uint32_t)P.slot_parity[j] << 18) |
+                     (P.slot_line[j] << 19);
     A(upload(s, d_cl, P.fused_line.data(), P.fused_line.size() * 4));
     A(upload(s, d_cs, P.fused_slot.data(), P.fused_slot.size() * 4));
     A(upload(s, d_meta, slot_meta.data(), P.P * 4));
-    s->fused = FusedArgs{d_cl, d_cs, d_meta, P.fused_chunks, P.fused_max_lines, P.fused_max_slots};
+    s->fused = FusedArgs{d_cl, d_cs, d_meta, P.fused_chunks, P.fused_max_lines, P.fused_max_slots, compact ? 1 : 0};
   }
   // stage heads: tableau coefficients here, each fiber's h, sgn and tables in the kernel
   // the close head also builds the next step's stage 0 unless the stage-0 Lawson table is
