@@ -263,6 +263,11 @@ MMP_API mmp_status mmp_kernel_times(mmp_solver* solver, mmp_kernel_time* out,
 /* Block until all work queued on the context's stream is complete. */
 MMP_API mmp_status mmp_ctx_synchronize(mmp_ctx* ctx);
 
+/* Without blocking: MMP_OK when all work queued on the context's stream is complete,
+ * MMP_ERR_STATE while some is still running (no reference counterpart; for host loops that
+ * keep several contexts busy). */
+MMP_API mmp_status mmp_ctx_query(mmp_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif

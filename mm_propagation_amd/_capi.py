@@ -18,6 +18,7 @@ MMP_OK = 0
 MMP_ERR_INVALID_ARGUMENT = 1
 MMP_ERR_UNSUPPORTED = 2
 MMP_ERR_NONFINITE = 6
+MMP_ERR_STATE = 7
 STATUS_NAMES = {
     0: "MMP_OK",
     1: "MMP_ERR_INVALID_ARGUMENT",
@@ -107,6 +108,7 @@ EXPORTS = [
     ("mmp_advance", c_int, [c_void_p, c_uint32]),
     ("mmp_get_result", c_int, [c_void_p, POINTER(c_double)]),
     ("mmp_batch_create", c_int, [c_void_p, c_uint32, c_void_p, c_void_p, POINTER(c_void_p)]),
+    ("mmp_ctx_query", c_int, [c_void_p]),
     ("mmp_fiber_result", c_int, [c_void_p, c_uint32, POINTER(c_double)]),
     ("mmp_fiber_state", c_int, [c_void_p, c_uint32, POINTER(c_double), POINTER(c_double)]),
     ("mmp_fiber_set_state", c_int, [c_void_p, c_uint32, POINTER(c_double)]),

@@ -864,6 +864,15 @@ mmp_status mmp_ctx_synchronize(mmp_ctx* ctx) {
   return MMP_OK;
 }
 
+mmp_status mmp_ctx_query(mmp_ctx* ctx) {
+  if (!ctx) return fail(MMP_ERR_INVALID_ARGUMENT, "ctx is NULL");
+  HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  const hipError_t e = hipStreamQuery(ctx->stream);
+  if (e == hipSuccess) return MMP_OK;
+  if (e == hipErrorNotReady) return MMP_ERR_STATE;
+  return hip_fail(e, "hipStreamQuery");
+}
+
 mmp_status mmp_solver_create(mmp_ctx* ctx, const mmp_params* params, const mmp_options* options,
                              mmp_solver** out) {
   try {

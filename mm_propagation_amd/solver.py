@@ -30,6 +30,14 @@ class Context:
     def synchronize(self) -> None:
         _capi.check(self.lib.mmp_ctx_synchronize(self.h), "mmp_ctx_synchronize")
 
+    def idle(self) -> bool:
+        """True when all work queued on this context's stream has completed (non-blocking)."""
+        st = self.lib.mmp_ctx_query(self.h)
+        if st == _capi.MMP_ERR_STATE:
+            return False
+        _capi.check(st, "mmp_ctx_query")
+        return True
+
     def close(self) -> None:
         if self.h:
             self.lib.mmp_ctx_destroy(self.h)

@@ -1,0 +1,59 @@
+"""The serving loop: requests of mixed shapes admitted into batch slots and solo solvers while
+others run give, each of them, exactly the result of the same solve alone."""
+import numpy as np
+import pytest
+
+from gpu_util import note, require_device
+from mm_propagation_amd import _capi, configs
+from mm_propagation_amd.serving import Request, ServingEngine
+from mm_propagation_amd.solver import MultimodePropagationGPU_RamanON
+
+pytestmark = pytest.mark.gpu
+
+
+def mix():
+    """8 requests: M = 3 and 6 at NT = 4096 (batch slots, 2 per group, so some wait), and NT = 2048
+    (too short for the batched kernels: solo solvers); low powers so every field stays finite."""
+    shapes = [(3, 4096, 40), (6, 4096, 56), (3, 4096, 24), (6, 2048, 30), (3, 4096, 64), (6, 4096, 20),
+              (3, 2048, 45), (6, 4096, 33)]
+    reqs = []
+    for i, (M, nt, n) in enumerate(shapes):
+        sp = configs.grin_fiber(M=M, nt=nt, n_steps=20000, fiber_length=0.3 + 0.05 * i, seed=50 + i,
+                                t0=0.1 + 0.01 * i, symmetric=True)
+        reqs.append(Request(rid=i, params=sp, n_steps=n, arrival=0.002 * i))
+    return reqs
+
+
+@pytest.mark.parametrize("precision", [_capi.MMP_PRECISION_MIXED, _capi.MMP_PRECISION_FP64])
+def test_each_result_is_the_same_alone_and_in_the_mix(precision):
+    require_device()
+    reqs = mix()
+    note(f"case serving mix precision={precision}")
+    eng = ServingEngine(slots=2, max_chunk=16, precision=precision)
+    stats = eng.run(reqs)
+    eng.close()
+    assert stats["requests"] == 8 and stats["nonfinite"] == 0
+    assert any(r.placement.startswith("solo") for r in reqs) and any(r.placement.startswith("batch") for r in reqs)
+    for r in reqs:
+        s = MultimodePropagationGPU_RamanON(r.params, pair_decode=_capi.MMP_PAIRS_PHYSICAL, precision=precision)
+        s.computeLawsonRK(r.n_steps)
+        alone = s.getResult()
+        s.close()
+        assert np.array_equal(r.result, alone), (r.rid, r.placement)
+        assert r.latency is not None and r.latency >= 0
+
+
+def test_admission_waits_for_memory():
+    """With a budget that fits one batch group at a time, the second group's requests wait in
+    the queue and are admitted once memory frees; every request still completes."""
+    require_device()
+    reqs = [r for r in mix() if r.params.nt == 2048]  # two solo requests
+    for r in reqs:
+        r.arrival = 0.0
+    from mm_propagation_amd import plan
+    one = plan.plan(reqs[0].params, precision=_capi.MMP_PRECISION_MIXED, pair_decode=1)["device_bytes"]
+    eng = ServingEngine(hbm_budget=1.5 * one, slots=2, max_chunk=8)
+    stats = eng.run(reqs)
+    eng.close()
+    assert stats["requests"] == 2 and all(r.result is not None for r in reqs)
+    assert stats["peak_device_bytes"] <= 1.5 * one
