@@ -84,11 +84,17 @@ def main() -> int:
                     help="every column pass a kernel of its own (7 per stage instead of 4)")
     ap.add_argument("--mixed", action="store_true",
                     help="mixed precision (FP32 transforms and sums, FP64 state); the bench line is FP64 by default")
-    ap.add_argument("--workload", choices=["grin6", "batched"], default="grin6",
+    ap.add_argument("--workload", choices=["grin6", "batched", "serving"], default="grin6",
                     help="grin6: the 6-mode GRIN fiber in FP64 (default); batched: 256 independent GRIN fibers "
-                         "(M in {3, 6, 10}, NT in {4096, 8192, 16384}) in batches per (M, NT)")
+                         "(M in {3, 6, 10}, NT in {4096, 8192, 16384}) in batches per (M, NT); serving: Poisson "
+                         "arrivals of solves (M in {1, 3, 6, 10, 15}, NT 2^12..2^16, 500..20000 steps) through "
+                         "the continuous-batching engine")
     ap.add_argument("--fibers", type=int, default=256, help="batched workload: number of fibers")
     ap.add_argument("--fp64-batch", action="store_true", help="batched workload in FP64 (default: mixed)")
+    ap.add_argument("--requests", type=int, default=32, help="serving workload: number of requests")
+    ap.add_argument("--rate", type=float, default=2.0, help="serving workload: Poisson arrival rate, requests/s")
+    ap.add_argument("--serving-seed", type=int, default=0, help="serving workload: arrival and shape seed")
+    ap.add_argument("--slots", type=int, default=16, help="serving workload: fiber slots per batch group")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the bench, copy this many MiB device to device once with the L2 evicted first "
                          "(the reference dispatch for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -99,6 +105,8 @@ def main() -> int:
         return 2
     if args.workload == "batched":
         return run_batched(args)
+    if args.workload == "serving":
+        return run_serving(args)
 
     from mm_propagation_amd import _capi, configs
     from mm_propagation_amd.solver import Context, MultimodePropagationGPU_RamanON
@@ -312,6 +320,111 @@ def run_batched(args) -> int:
     for _, _, ctx, b in batches:
         b.close()
         ctx.close()
+    return 0
+
+
+def rk4_bytes_per_step(M: int, NT: int, P: int, raman: bool, c_w: int) -> float:
+    """Algorithmic HBM bytes of one RK4 step: every logical array read or written once per stage,
+    field state at 16 B per sample, work arrays at c_w B (16 FP64, 8 mixed):
+    NT * [6 * 16 * M + c_w * (19 M + 4 R M + 8 R P)]."""
+    R = 1 if raman else 0
+    return NT * (6 * 16 * M + c_w * (19 * M + 4 * R * M + 8 * R * P))
+
+
+def run_serving(args) -> int:
+    """Continuous batching of propagation solves: `--requests` requests arrive as a Poisson process
+    at `--rate` per second (serving_requests(seed)), with M in {1, 3, 6, 10, 15}, NT in
+    2^12..2^16, n_steps log-uniform in [500, 20000], pulses as in the batched workload (10 W to
+    10 kW, T0 U(0.05, 0.3) ps); mixed precision, physical pair decode. The engine admits them by
+    device footprint into batch slots (grouped by structure) or solo solvers, each on its own
+    stream. The timed region is the engine's run from the first arrival to the last result."""
+    from mm_propagation_amd import _capi, configs
+    from mm_propagation_amd.serving import ServingEngine, serving_requests
+    from mm_propagation_amd.solver import Context, MultimodePropagationGPU_RamanON
+
+    # warm-up: one small solve per engine path (module load, first graph instantiation)
+    warm = serving_requests(4, 1e6, seed=1000 + args.serving_seed, modes=(3,), nts=(4096, 2048), steps=(32, 64),
+                            power_range=(10.0, 100.0))
+    eng = ServingEngine(slots=args.slots)
+    eng.run(warm)
+    eng.close()
+
+    reqs = serving_requests(args.requests, args.rate, seed=args.serving_seed)
+    eng = ServingEngine(slots=args.slots)
+    stats = eng.run(reqs)
+    eng.close()
+    makespan = stats["makespan_s"]
+    units = sum(r.units for r in reqs)
+    value = units / makespan
+    steps = sum(r.n_steps for r in reqs)
+
+    # the same requests solved one after another by the plain FP64 single-fiber solver, from its
+    # per-step time for each shape (measured over 48 steps) times the request's steps
+    ctx = Context(0)
+    per_step = {}
+    plain_time = 0.0
+    for r in reqs:
+        key = (r.params.n_modes, r.params.nt)
+        if key not in per_step:
+            s1 = MultimodePropagationGPU_RamanON(r.params, pair_decode=_capi.MMP_PAIRS_PHYSICAL, ctx=ctx)
+            s1.advance(16)
+            ctx.synchronize()
+            t1 = time.perf_counter()
+            s1.advance(48)
+            ctx.synchronize()
+            per_step[key] = (time.perf_counter() - t1) / 48
+            s1.close()
+        plain_time += per_step[key] * r.n_steps
+
+    # accuracy: the three smallest finite requests against the plain FP64 GPU solver (itself held
+    # to 1e-10 of the FP64 oracle), with the total-energy drift between the two
+    acc = {}
+    finite = sorted((r for r in reqs if not r.nonfinite), key=lambda r: r.units)[:3]
+    for r in finite:
+        s1 = MultimodePropagationGPU_RamanON(r.params, pair_decode=_capi.MMP_PAIRS_PHYSICAL, ctx=ctx)
+        s1.computeLawsonRK(r.n_steps)
+        ref = s1.getResult()
+        s1.close()
+        err = configs.rel_l2_per_mode(r.result, ref)
+        e_ref = float(np.sum(np.abs(ref) ** 2))
+        e_got = float(np.sum(np.abs(r.result) ** 2))
+        acc[f"req{r.rid}_M{r.params.n_modes}_NT{r.params.nt}_steps{r.n_steps}"] = {
+            "rel_l2_max": float(err.max()), "energy_drift": (e_got - e_ref) / e_ref}
+    ctx.close()
+
+    # aggregate HBM rate: algorithmic bytes of every request's steps over the makespan
+    bytes_total = 0.0
+    for r in reqs:
+        info = eng.plan_for(r)
+        bytes_total += rk4_bytes_per_step(r.params.n_modes, r.params.nt, int(info["n_pairs"]), r.raman, 8) * r.n_steps
+    achieved = bytes_total / makespan / 1e9
+    placements = {}
+    for r in reqs:
+        kind = r.placement.split()[0]
+        placements[kind] = placements.get(kind, 0) + 1
+    out = {
+        "metric": METRIC, "value": value, "unit": "M·NT·steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": len(warm), "ms_per_step": makespan * 1e3 / steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"serving{args.requests}: Poisson arrivals at {args.rate}/s (seed {args.serving_seed}), "
+                               "M in {1,3,6,10,15}, NT in {4096..65536}, n_steps log-uniform [500, 20000], "
+                               "L U(0.2,1) m, T0 U(0.05,0.3) ps, P log-uniform [10 W, 10 kW], symmetric GRIN "
+                               f"coupling, RK4, Raman on, mixed precision, {args.slots} slots per batch group",
+                   "global_batch": args.requests, "seq_len": "4096-65536", "parallelism": "single GPU"},
+        "latency_s": {"p50": stats["p50_latency_s"], "p99": stats["p99_latency_s"],
+                      "max": max(r.latency for r in reqs)},
+        "makespan_s": makespan,
+        "requests_nonfinite": stats["nonfinite"],
+        "placements": placements, "batch_groups": stats["groups"],
+        "peak_device_bytes": stats["peak_device_bytes"],
+        "plain_fp64_sequential": {"value": units / plain_time, "seconds": plain_time,
+                                  "ratio": value / (units / plain_time)},
+        "accuracy_vs_plain_fp64": acc,
+        "roofline": {"bound": "hbm", "kernel": "whole run (all kernels)", "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": None},
+    }
+    print(json.dumps(out), flush=True)
     return 0
 
 

@@ -122,7 +122,7 @@ typedef struct mmp_info {
   double conversion_factor, sign_gamma, ld, max_beta2, max_q;
   size_t device_bytes;       /* device memory held by the solver (all fibers) */
   uint32_t n_fibers;         /* fibers in the solver (1 unless made by mmp_batch_create) */
-  uint32_t reserved;
+  uint32_t live;             /* 1 when the steps advance this fiber, 0 when it is parked */
 } mmp_info;
 
 typedef struct mmp_ctx mmp_ctx;
@@ -195,6 +195,12 @@ MMP_API mmp_status mmp_fiber_info(mmp_solver* solver, uint32_t fiber, mmp_info* 
  * request): its inputs are uploaded, it restarts at z = 0; the other fibers keep their state.
  * Waits for the work already queued on the context's stream. */
 MMP_API mmp_status mmp_fiber_load(mmp_solver* solver, uint32_t fiber, const mmp_params* params);
+/* Parks (live = 0) or resumes (live = 1) fiber f: the steps leave a parked fiber's field, z and
+ * Lawson tables untouched, and its workgroups exit at once, so a batch whose slots are partly
+ * empty costs about what its live fibers cost. mmp_fiber_load makes a fiber live. Waits for the
+ * work already queued on the context's stream. Fibers start live. MMP_ERR_UNSUPPORTED for a
+ * solver that does not run the column-stage kernels (batches always do). */
+MMP_API mmp_status mmp_fiber_set_live(mmp_solver* solver, uint32_t fiber, uint32_t live);
 
 /* Replaces computeLawsonRK(nz) (include/MultimodePropagation_GPU.cuh:143;
  * semantics of src/MultimodePropagation_CPU.cpp:24-34): restarts at z = 0

@@ -81,7 +81,7 @@ class MmpInfo(ctypes.Structure):
         ("z", c_double), ("conversion_factor", c_double), ("sign_gamma", c_double), ("ld", c_double),
         ("max_beta2", c_double), ("max_q", c_double),
         ("device_bytes", c_size_t),
-        ("n_fibers", c_uint32), ("reserved", c_uint32),
+        ("n_fibers", c_uint32), ("live", c_uint32),
     ]
 
 
@@ -115,6 +115,7 @@ EXPORTS = [
     ("mmp_fiber_energy", c_int, [c_void_p, c_uint32, POINTER(c_double)]),
     ("mmp_fiber_info", c_int, [c_void_p, c_uint32, c_void_p]),
     ("mmp_fiber_load", c_int, [c_void_p, c_uint32, c_void_p]),
+    ("mmp_fiber_set_live", c_int, [c_void_p, c_uint32, c_uint32]),
     ("mmp_propagate_snapshots", c_int, [c_void_p, c_uint32, c_uint32, c_uint32, POINTER(c_double),
                                         POINTER(c_double)]),
     ("mmp_get_state", c_int, [c_void_p, POINTER(c_double), POINTER(c_double)]),

@@ -89,6 +89,11 @@ class MultimodePropagationBatch:
         _capi.check(self.lib.mmp_fiber_load(self.h, int(f), ctypes.byref(cp.c)), "mmp_fiber_load")
         self._cps[f] = cp
 
+    def set_live(self, f: int, live: bool) -> None:
+        """Park (False) or resume (True) fiber f: a parked fiber is not advanced by the steps and
+        its workgroups exit at once."""
+        _capi.check(self.lib.mmp_fiber_set_live(self.h, int(f), 1 if live else 0), "mmp_fiber_set_live")
+
     def kernel_times(self) -> List[dict]:
         n = ctypes.c_uint32()
         _capi.check(self.lib.mmp_kernel_times(self.h, None, 0, ctypes.byref(n)), "mmp_kernel_times")

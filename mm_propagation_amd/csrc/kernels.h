@@ -71,6 +71,8 @@ struct FiberParams {
   double dt;   // delta t, the Raman convolution's quadrature weight
   double kv;   // slot mix: 1 - fR (Raman on) or 1 (off)
   double kr;   // slot mix: fR T0 (Raman on) or 0 (off)
+  uint32_t live;  // 0: the fiber is parked; the batched kernels leave its arrays and z untouched
+  uint32_t pad;
 };
 
 // Fiber f of a batch (blockIdx.y in the batched kernels) finds its arrays at base + f * stride.

@@ -90,6 +90,7 @@ __global__ void __launch_bounds__(64) k_row_head(Geometry g, const cpx<W>* __res
   // fiber blockIdx.y of the batch: its arrays and scalars
   const int fib = blockIdx.y;
   const FiberParams fp = b.fp[fib];
+  if (!fp.live) return;  // uniform over the workgroup
   const double* __restrict__ hc = b.coef + ((size_t)fib * b.nheads + a.head) * kHeadCoef;  // uniform
   cd* __restrict__ psi_hat = a.psi_hat + fib * b.sf;
   cd* __restrict__ accp = a.acc + fib * b.sf;
@@ -396,6 +397,7 @@ __global__ void __launch_bounds__(64) k_row_raman(Geometry g, const cpx<W>* __re
   using cw = cpx<W>;
   __shared__ cw smem[C::G * C::SLOTS];
   const int fib = blockIdx.y;
+  if (!b.fp[fib].live) return;  // uniform over the workgroup
   data += fib * b.sl;
   hspec += fib * b.sh;
   const double hscale = b.fp[fib].dt;
@@ -632,6 +634,7 @@ __global__ void __launch_bounds__(256) k_colstage_fields_pairs(Geometry g, const
   constexpr int L = F::L;
   __shared__ cw sh[F::FFT + kColStageMaxM * L];
   const int fib = blockIdx.y;
+  if (!b.fp[fib].live) return;  // uniform over the workgroup
   yin += fib * b.sf;
   U += fib * b.sf;
   conv += fib * b.sl;
@@ -717,6 +720,7 @@ __global__ void __launch_bounds__(NW * 64) k_colstage_raman_gather(Geometry g, c
   using cw = cpx<W>;
   const int fib = blockIdx.y;
   const FiberParams fp = b.fp[fib];
+  if (!fp.live) return;  // uniform over the workgroup
   U += fib * b.sf;
   yout += fib * b.sf;
   conv += fib * b.sl;

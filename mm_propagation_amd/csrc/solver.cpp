@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -107,6 +108,7 @@ struct mmp_solver {
   const void* q_w = nullptr;
   const void* qdense_w = nullptr;
   std::vector<uint32_t> steps;  // per fiber, since its last restart
+  std::vector<uint8_t> live;    // per fiber: advanced by the steps (1) or parked (0)
   bool graphs = true;
   hipGraphExec_t graph_exec = nullptr;
   hipGraph_t graph = nullptr;
@@ -428,6 +430,7 @@ void fill_info(const Problem& P, const mmp_options& opt, mmp_info* o) {
   o->max_q = P.max_q;
   o->device_bytes = planned_device_bytes(P, opt, 1);
   o->n_fibers = 1;
+  o->live = 1;
 }
 
 // Fibers of one batch must share everything the kernels take from fiber 0's problem.
@@ -479,7 +482,7 @@ mmp_status load_fiber(mmp_solver* s, uint32_t f, Problem& Pf) {
       MMP_TRY(upload(s, s->qdense + f * b.sq, qd.data(), qd.size() * 8));
     }
   }
-  const FiberParams fp{Pf.delta_z, Pf.sign_gamma, Pf.delta_t, Pf.kv, Pf.kr};
+  const FiberParams fp{Pf.delta_z, Pf.sign_gamma, Pf.delta_t, Pf.kv, Pf.kr, 1u, 0u};
   MMP_TRY(upload(s, s->d_fp + f, &fp, sizeof fp));
   // b_{i-1} h and a_ij h of every head, multiplied in FP64 on the host as the single solver does
   std::vector<double> coef(b.nheads * kHeadCoef, 0.);
@@ -503,6 +506,7 @@ mmp_status load_fiber(mmp_solver* s, uint32_t f, Problem& Pf) {
   HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");  // host arrays above are staging copies
   s->z_host[f] = 0.;
   s->steps[f] = 0;
+  s->live[f] = 1;
   for (auto* v : {&Pf.imL_fourstep, &Pf.e1, &Pf.etab0, &Pf.psi0, &Pf.raman_response}) std::vector<double>().swap(*v);
   return MMP_OK;
 }
@@ -537,6 +541,7 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
   }
   s->z_host.assign(n_fibers, 0.);
   s->steps.assign(n_fibers, 0u);
+  s->live.assign(n_fibers, 1);
   s->P = s->fib[0];  // structure of the batch (fiber 0's large arrays are uploaded from s->fib[0])
   for (auto* v : {&s->P.imL_fourstep, &s->P.e1, &s->P.etab0, &s->P.psi0, &s->P.raman_response})
     std::vector<double>().swap(*v);
@@ -782,6 +787,7 @@ mmp_status advance_impl(mmp_solver* s, uint32_t n) {
   }
   for (uint32_t k = 0; k < left; k++) MMP_TRY(enqueue_step(s));
   for (uint32_t f = 0; f < s->F; f++) {
+    if (!s->live[f]) continue;
     for (uint32_t k = 0; k < n; k++) s->z_host[f] += s->fib[f].delta_z;  // z += h (src/MultimodePropagation_CPU.cpp:32)
     s->steps[f] += n;
   }
@@ -1001,6 +1007,21 @@ mmp_status mmp_fiber_load(mmp_solver* s, uint32_t f, const mmp_params* params) {
   }
 }
 
+mmp_status mmp_fiber_set_live(mmp_solver* s, uint32_t f, uint32_t live) {
+  MMP_TRY(check_fiber(s, f));
+  if (live > 1) return fail(MMP_ERR_INVALID_ARGUMENT, "live must be 0 or 1");
+  if (s->live[f] == live) return MMP_OK;
+  // only the column-stage step is made of kernels that all honour the flag
+  if (!s->use_fused) return fail(MMP_ERR_UNSUPPORTED, "parking a fiber needs the column-stage kernels");
+  MMP_TRY(set_device(s));
+  const uint32_t v = live;
+  HIP_TRY(hipStreamSynchronize(s->ctx->stream), "hipStreamSynchronize");  // queued steps see the old flag
+  MMP_TRY(upload(s, reinterpret_cast<char*>(s->d_fp + f) + offsetof(FiberParams, live), &v, sizeof v));
+  HIP_TRY(hipStreamSynchronize(s->ctx->stream), "hipStreamSynchronize");
+  s->live[f] = (uint8_t)live;
+  return MMP_OK;
+}
+
 mmp_status mmp_energy(mmp_solver* s, double* energy_out) { return mmp_fiber_energy(s, 0, energy_out); }
 
 mmp_status mmp_fiber_energy(mmp_solver* s, uint32_t f, double* energy_out) {
@@ -1025,6 +1046,7 @@ mmp_status mmp_fiber_info(mmp_solver* s, uint32_t f, mmp_info* o) {
   o->z = s->z_host[f];
   o->device_bytes = s->device_bytes;
   o->n_fibers = s->F;
+  o->live = s->live[f];
   return MMP_OK;
 }
 

@@ -72,6 +72,8 @@ class _Group:
         self.busy = False
         self.chunk = 0
         self.bytes = int(self.batch.info()["device_bytes"])
+        for i in range(slots):
+            self.batch.set_live(i, False)  # empty slots cost nothing until a request is loaded
 
     def free_slot(self) -> Optional[int]:
         for i, r in enumerate(self.slots):
@@ -123,7 +125,7 @@ class ServingEngine:
         self._plans: Dict[int, Dict[str, float]] = {}
 
     # --- placement --------------------------------------------------------
-    def _plan(self, req: Request) -> Dict[str, float]:
+    def plan_for(self, req: Request) -> Dict[str, float]:
         if req.rid not in self._plans:  # host preparation once per request, not per admission attempt
             self._plans[req.rid] = plan.plan(req.params, raman=req.raman, pair_decode=self.pair_decode,
                                              precision=self.precision)
@@ -133,7 +135,7 @@ class ServingEngine:
         return req.raman and int(info["n1"]) == 64 and int(info["n_modes"]) <= 12
 
     def _admit(self, req: Request, now: float) -> bool:
-        info = self._plan(req)
+        info = self.plan_for(req)
         if self._batchable(req, info):
             key = _structure_key(req, info)
             g = self.groups.get(key)
@@ -193,6 +195,7 @@ class ServingEngine:
                     fields = None
                 self._finish(req, fields, now)
                 g.slots[i] = None
+                g.batch.set_live(i, False)
         g.chunk = 0
 
     def _launch_group(self, g: _Group):

@@ -87,6 +87,33 @@ def test_load_replaces_one_fiber_and_keeps_the_others():
     s.close()
 
 
+@pytest.mark.parametrize("precision", [_capi.MMP_PRECISION_FP64, _capi.MMP_PRECISION_MIXED])
+def test_parked_fiber_is_left_untouched_and_resumes(precision):
+    """Fiber 1 is parked for 32 steps (graph replays and single steps): its state, z and step
+    count stay as they were; resumed, it ends where 16 + 8 steps alone end. Fibers 0 and 2 are
+    the 56-step single solves."""
+    require_device()
+    sps = fibers(3, 4096, 3, base=20)
+    note(f"case batch park precision={precision}")
+    b = MultimodePropagationBatch(sps, pair_decode=1, precision=precision)
+    b.computeLawsonRK(16)
+    psi0, z0 = b.state(1)
+    b.set_live(1, False)
+    b.advance(32)
+    psi1, z1 = b.state(1)
+    assert b.info(1)["live"] == 0 and b.info(1)["steps_taken"] == 16 and z1 == z0
+    assert np.array_equal(psi0, psi1)
+    b.set_live(1, True)
+    b.advance(8)
+    r = [b.result(f) for f in range(3)]
+    steps = [b.info(f)["steps_taken"] for f in range(3)]
+    b.close()
+    assert steps == [56, 24, 56]
+    assert np.array_equal(r[0], single(sps[0], 56, precision))
+    assert np.array_equal(r[2], single(sps[2], 56, precision))
+    assert np.array_equal(r[1], single(sps[1], 24, precision))
+
+
 def test_batch_rejects_mixed_structures_and_ineligible_shapes():
     require_device()
     with pytest.raises(_capi.MmpError) as ei:
