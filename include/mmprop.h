@@ -69,7 +69,7 @@ enum {
   MMP_FLAG_NO_GRAPH = 1u << 1,    /* launch step kernels one by one instead of replaying a HIP graph */
   MMP_FLAG_SEPARATE_COLUMNS = 1u << 2 /* keep every column pass a kernel of its own (7 passes per stage)
                                          where the column-stage kernels would run (4 passes per stage:
-                                         Raman on, 64-sample columns, n_modes <= 12) */
+                                         Raman on, 64-sample columns, n_modes <= 16) */
 };
 
 /*
@@ -179,7 +179,7 @@ MMP_API void mmp_solver_destroy(mmp_solver* solver);
  * n_fibers independent fibers, params[f] each, that share n_modes, nt, method_order, raman and
  * the pair-slot structure of their coupling lists (their values, dispersion, lengths, pulses
  * and Raman responses may all differ). Every stage pass runs all fibers in one launch. Needs
- * the column-stage kernels (Raman on, nt in {4096, 8192, 16384}, n_modes <= 12). The calls
+ * the column-stage kernels (Raman on, nt in {4096, 8192, 16384}, n_modes <= 16). The calls
  * above act on every fiber (mmp_compute_lawson_rk, mmp_advance, mmp_propagate_snapshots's
  * stepping) or on fiber 0 (results, state, energy, info); the mmp_fiber_* calls address one
  * fiber. */

@@ -119,7 +119,7 @@ template <int L>
 struct LinePlan {
   static constexpr int n8 = (L >= 512) ? 3 : (L >= 64 ? 2 : 1);
   static constexpr int tail = L / (n8 == 3 ? 512 : (n8 == 2 ? 64 : 8));
-  static_assert(L >= 8 && L <= 256 && (L & (L - 1)) == 0, "line length must be a power of two in [8, 256]");
+  static_assert(L >= 8 && L <= 1024 && (L & (L - 1)) == 0, "line length must be a power of two in [8, 1024]");
   static constexpr int T = L / 8;
 };
 
@@ -141,7 +141,8 @@ struct LineSlots {
 
 // Stage twiddles a lane needs for one line transform, loaded before the data
 // arrives so their latency overlaps the data loads. Slots:
-//   [0..6]  radix-8 stage of span 8 (L >= 64):  W_L^((t%8) * m * L/64), m = 1..7
+//   [0..6]   radix-8 stage of span 8 (L >= 64):   W_L^((t%8) * m * L/64), m = 1..7
+//   [7..13]  radix-8 stage of span 64 (L >= 512): W_L^((t%64) * m * L/512), m = 1..7
 //   then the trailing stage, butterflies j = t + b*T:
 //     radix-4: W_L^(j*m), m = 1..3 (3 per butterfly, b = 0, 1)
 //     radix-2: W_L^j (b = 0..3)
@@ -149,7 +150,7 @@ template <int L, class T = double>
 struct LineTw {
   static constexpr int n8 = LinePlan<L>::n8;
   static constexpr int tail = LinePlan<L>::tail;
-  static constexpr int base = (n8 == 2) ? 7 : 0;
+  static constexpr int base = (n8 == 3) ? 14 : ((n8 == 2) ? 7 : 0);
   static constexpr int count = (L == 8) ? 1 : base + (tail == 4 ? 6 : (tail == 2 ? 4 : 0));
   cpx<T> w[count];
 };
@@ -164,10 +165,15 @@ __device__ __forceinline__ void load_line_tw(LineTw<L, T>& lt, const cpx<T>* __r
     (void)t;
     return;
   } else {
-    if constexpr (P::n8 == 2) {
+    if constexpr (P::n8 >= 2) {
       const int k = t % 8;
 #pragma unroll
       for (int m = 1; m < 8; m++) lt.w[m - 1] = tw[k * m * (L / 64)];
+    }
+    if constexpr (P::n8 == 3) {
+      const int k = t % 64;
+#pragma unroll
+      for (int m = 1; m < 8; m++) lt.w[6 + m] = tw[k * m * (L / 512)];
     }
     constexpr int b0 = LineTw<L, T>::base;
     if constexpr (P::tail == 4) {
@@ -226,20 +232,36 @@ __device__ __forceinline__ void line_fft(cpx<T>* v, cpx<T>* sm, const LineTw<L, 
     line_sync<WAVE>();
 #pragma unroll
     for (int i = 0; i < 8; i++) v[i] = sm[lds_pos<STRIDE, PAD>(t + i * LT)];
-    if constexpr (P::n8 == 2) {
+    if constexpr (P::n8 >= 2) {
       // radix-8 stage of span 8
 #pragma unroll
       for (int m = 1; m < 8; m++) v[m] = cmul(v[m], tw_dir<SIGN>(lt.w[m - 1]));
+      dft8<SIGN>(v);
+      if constexpr (P::n8 == 2 && P::tail == 1) {
+        line_sync<WAVE>();
+        return;  // outputs at positions t + q*LT
+      }
+      line_sync<WAVE>();
+      const int base = (t / 8) * 64 + t % 8;
+#pragma unroll
+      for (int q = 0; q < 8; q++) sm[lds_pos<STRIDE, PAD>(base + q * 8)] = v[q];
+      line_sync<WAVE>();
+#pragma unroll
+      for (int i = 0; i < 8; i++) v[i] = sm[lds_pos<STRIDE, PAD>(t + i * LT)];
+    }
+    if constexpr (P::n8 == 3) {
+      // radix-8 stage of span 64
+#pragma unroll
+      for (int m = 1; m < 8; m++) v[m] = cmul(v[m], tw_dir<SIGN>(lt.w[6 + m]));
       dft8<SIGN>(v);
       if constexpr (P::tail == 1) {
         line_sync<WAVE>();
         return;  // outputs at positions t + q*LT
       }
       line_sync<WAVE>();
-      const int k = t % 8;
-      const int base = (t / 8) * 64 + k;
+      const int base = (t / 64) * 512 + t % 64;
 #pragma unroll
-      for (int q = 0; q < 8; q++) sm[lds_pos<STRIDE, PAD>(base + q * 8)] = v[q];
+      for (int q = 0; q < 8; q++) sm[lds_pos<STRIDE, PAD>(base + q * 64)] = v[q];
       line_sync<WAVE>();
 #pragma unroll
       for (int i = 0; i < 8; i++) v[i] = sm[lds_pos<STRIDE, PAD>(t + i * LT)];

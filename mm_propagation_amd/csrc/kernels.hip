@@ -36,7 +36,7 @@ template <int L>
 struct RowCfg {
   static constexpr int T = L / 8;                  // lanes per row
   static constexpr int G = T >= 64 ? 1 : 64 / T;   // rows per workgroup
-  static constexpr int THREADS = G * T;            // 64
+  static constexpr int THREADS = G * T;            // 64 (128 for rows of 1024)
   static constexpr int SLOTS = LineSlots<L, 1>::value;
 };
 
@@ -76,7 +76,7 @@ __device__ __forceinline__ cd cmul_exact(cd a, cd b) {
 }
 
 template <int L, class W>
-__global__ void __launch_bounds__(64) k_row_head(Geometry g, const cpx<W>* __restrict__ rowtw, HeadArgs a, Batch b) {
+__global__ void __launch_bounds__(RowCfg<L>::THREADS) k_row_head(Geometry g, const cpx<W>* __restrict__ rowtw, HeadArgs a, Batch b) {
   using C = RowCfg<L>;
   using cw = cpx<W>;
   __shared__ cw smem[C::G * C::SLOTS];
@@ -334,7 +334,7 @@ __global__ void __launch_bounds__(256) k_col_pairs_forward(Geometry g, const cpx
 // row passes
 // ---------------------------------------------------------------------------
 template <int L>
-__global__ void __launch_bounds__(64) k_row_forward(Geometry g, const cd* __restrict__ rowtw, cd* __restrict__ data,
+__global__ void __launch_bounds__(RowCfg<L>::THREADS) k_row_forward(Geometry g, const cd* __restrict__ rowtw, cd* __restrict__ data,
                                                     int lines) {
   using C = RowCfg<L>;
   __shared__ cd smem[C::G * C::SLOTS];
@@ -358,7 +358,7 @@ __global__ void __launch_bounds__(64) k_row_forward(Geometry g, const cd* __rest
 
 // optional phase exp(i ImL z) (the reference's exp(L z)), then inverse row DFT
 template <int L>
-__global__ void __launch_bounds__(64) k_row_inverse_phase(Geometry g, const cd* __restrict__ rowtw,
+__global__ void __launch_bounds__(RowCfg<L>::THREADS) k_row_inverse_phase(Geometry g, const cd* __restrict__ rowtw,
                                                           const cd* __restrict__ in, cd* __restrict__ out,
                                                           const double* __restrict__ imL,
                                                           const StepState* __restrict__ st, int use_phase) {
@@ -390,7 +390,7 @@ __global__ void __launch_bounds__(64) k_row_inverse_phase(Geometry g, const cd* 
 
 // Raman convolution in the spectral rows: DFT, * H * hscale, inverse DFT (in place)
 template <int L, class W>
-__global__ void __launch_bounds__(64) k_row_raman(Geometry g, const cpx<W>* __restrict__ rowtw,
+__global__ void __launch_bounds__(RowCfg<L>::THREADS) k_row_raman(Geometry g, const cpx<W>* __restrict__ rowtw,
                                                   cpx<W>* __restrict__ data, int lines,
                                                   const cpx<W>* __restrict__ hspec, Batch b) {
   using C = RowCfg<L>;
@@ -602,7 +602,7 @@ __global__ void __launch_bounds__(512) k_gather(Geometry g, const cpx<W>* __rest
 // Lanes form 32 line slots of 8 lanes; a round transforms up to 32 lines.
 // ---------------------------------------------------------------------------
 constexpr int kColStageL = 64;       // column length the column-stage kernels take
-constexpr int kColStageMaxM = 12;    // modes kept in registers by the gather
+constexpr int kColStageMaxM = 16;    // modes kept in registers by the gather
 constexpr int kColStageXCap = 3072;  // complex LDS values for Raman lines / partial sums
 constexpr int kColStageQCap = 2048;  // coupling coefficients of one chunk's slots held in LDS
 constexpr int kColStageSCap = 128;   // slots of one chunk
@@ -732,8 +732,9 @@ __global__ void __launch_bounds__(NW * 64) k_colstage_raman_gather(Geometry g, c
   constexpr int MM = PAR ? M * KH : M * M;  // coupling coefficients per slot
   constexpr int UR = (M + 7) / 8 * 8;  // field rows, rounded up to whole wavefronts of lines
   constexpr int UIN = (M * L + NT - 1) / NT;
-  constexpr int RCAP = kColStageRRows * RS;
-  static_assert(M >= 1 && M <= kColStageMaxM && NW * M * L <= RCAP, "modes out of range");
+  // Raman line rows of a chunk, or the NW partial sums of M modes after the gather
+  constexpr int RCAP = kColStageRRows * RS > NW * M * L ? kColStageRRows * RS : NW * M * L;
+  static_assert(M >= 1 && M <= kColStageMaxM, "modes out of range");
   __shared__ cw Ush[UR * RS];   // field rows; sigma rows after the gather
   __shared__ cw Rsh[RCAP];      // Raman line rows of the chunk; partial sums after the gather
   __shared__ W Qsh[kColStageQCap];         // [slots of the chunk][M][M] coupling blocks
@@ -995,6 +996,18 @@ __global__ void __launch_bounds__(256) k_energy(Geometry g, const cd* __restrict
     case 64: return FN<64>(__VA_ARGS__);                   \
     case 128: return FN<128>(__VA_ARGS__);                 \
     case 256: return FN<256>(__VA_ARGS__);                 \
+    case 512: return FN<512>(__VA_ARGS__);                 \
+    case 1024: return FN<1024>(__VA_ARGS__);               \
+    default: return hipErrorInvalidValue;                  \
+  }
+
+// column lengths: N1 <= 64 for every supported nt (the split keeps rows within 1024)
+#define MMP_DISPATCH_COL_LW(Lv, FN, W, ...)                \
+  switch (Lv) {                                            \
+    case 8: return FN<8, W>(__VA_ARGS__);                  \
+    case 16: return FN<16, W>(__VA_ARGS__);                \
+    case 32: return FN<32, W>(__VA_ARGS__);                \
+    case 64: return FN<64, W>(__VA_ARGS__);                \
     default: return hipErrorInvalidValue;                  \
   }
 
@@ -1006,6 +1019,8 @@ __global__ void __launch_bounds__(256) k_energy(Geometry g, const cd* __restrict
     case 64: return FN<64, W>(__VA_ARGS__);                \
     case 128: return FN<128, W>(__VA_ARGS__);              \
     case 256: return FN<256, W>(__VA_ARGS__);              \
+    case 512: return FN<512, W>(__VA_ARGS__);              \
+    case 1024: return FN<1024, W>(__VA_ARGS__);            \
     default: return hipErrorInvalidValue;                  \
   }
 
@@ -1063,7 +1078,7 @@ static hipError_t col_inverse_impl(const Geometry& g, const Twiddles& tw, const 
 template <class W>
 hipError_t launch_col_inverse(const Geometry& g, const Twiddles& tw, const cpx<W>* in, cpx<W>* out, int lines,
                               double scale, double divisor, hipStream_t s) {
-  MMP_DISPATCH_LW(g.N1, col_inverse_impl, W, g, tw, in, out, lines, scale, divisor, s);
+  MMP_DISPATCH_COL_LW(g.N1, col_inverse_impl, W, g, tw, in, out, lines, scale, divisor, s);
 }
 
 template <int L, class W>
@@ -1074,7 +1089,7 @@ static hipError_t col_forward_impl(const Geometry& g, const Twiddles& tw, const 
 template <class W>
 hipError_t launch_col_forward(const Geometry& g, const Twiddles& tw, const cpx<W>* in, cpx<W>* out, int lines,
                               hipStream_t s) {
-  MMP_DISPATCH_LW(g.N1, col_forward_impl, W, g, tw, in, out, lines, s);
+  MMP_DISPATCH_COL_LW(g.N1, col_forward_impl, W, g, tw, in, out, lines, s);
 }
 
 template <int L, class W>
@@ -1084,7 +1099,7 @@ static hipError_t col_forward_real_impl(const Geometry& g, const Twiddles& tw, c
 }
 hipError_t launch_col_forward_real(const Geometry& g, const Twiddles& tw, const double* in, cd* out,
                                    hipStream_t s) {
-  MMP_DISPATCH_LW(g.N1, col_forward_real_impl, double, g, tw, in, out, s);
+  MMP_DISPATCH_COL_LW(g.N1, col_forward_real_impl, double, g, tw, in, out, s);
 }
 
 template <int L, class W>
@@ -1095,7 +1110,7 @@ static hipError_t col_pairs_forward_impl(const Geometry& g, const Twiddles& tw, 
 template <class W>
 hipError_t launch_col_pairs_forward(const Geometry& g, const Twiddles& tw, const cpx<W>* U, cpx<W>* out, int lines,
                                     const PairLineArgs& pl, hipStream_t s) {
-  MMP_DISPATCH_LW(g.N1, col_pairs_forward_impl, W, g, tw, U, out, lines, pl, s);
+  MMP_DISPATCH_COL_LW(g.N1, col_pairs_forward_impl, W, g, tw, U, out, lines, pl, s);
 }
 
 template <int L>
@@ -1192,7 +1207,8 @@ hipError_t launch_colstage_raman_gather(const Geometry& g, const Twiddles& tw, c
   switch (g.M) {
     MMP_COLSTAGE_CASE(1) MMP_COLSTAGE_CASE(2) MMP_COLSTAGE_CASE(3) MMP_COLSTAGE_CASE(4) MMP_COLSTAGE_CASE(5)
     MMP_COLSTAGE_CASE(6) MMP_COLSTAGE_CASE(7) MMP_COLSTAGE_CASE(8) MMP_COLSTAGE_CASE(9) MMP_COLSTAGE_CASE(10)
-    MMP_COLSTAGE_CASE(11) MMP_COLSTAGE_CASE(12)
+    MMP_COLSTAGE_CASE(11) MMP_COLSTAGE_CASE(12) MMP_COLSTAGE_CASE(13) MMP_COLSTAGE_CASE(14) MMP_COLSTAGE_CASE(15)
+    MMP_COLSTAGE_CASE(16)
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -94,10 +94,10 @@ bool split_length(uint32_t nt, uint32_t& n1, uint32_t& n2) {
   if (nt < 64 || nt > 65536 || (nt & (nt - 1)) != 0) return false;
   uint32_t lg = 0;
   while ((1u << lg) < nt) lg++;
-  // columns of 64 samples while rows stay within 256 (more columns = more parallel column
+  // columns of 64 samples while rows stay within 1024 (more columns = more parallel column
   // workgroups, and a 64-sample column of every field and Raman line fits in LDS)
   n1 = std::min(1u << (lg / 2), 64u);
-  if (nt / n1 > 256) n1 = nt / 256;
+  if (nt / n1 > 1024) n1 = nt / 1024;
   n2 = nt / n1;
   return true;
 }
@@ -407,7 +407,7 @@ mmp_status prepare_problem(const mmp_params* in, const mmp_options& opt, Problem
   }
   P.kv = P.raman ? 1 - P.fR : 1.;
   P.kr = P.raman ? P.fR * P.T0 : 0.;
-  // Column-stage kernels (Raman on, N1 = 64, dense gather, M <= 12): Raman lines are
+  // Column-stage kernels (Raman on, N1 = 64, M <= 16, dense coupling blocks): Raman lines are
   // processed in chunks that respect the LDS limits on lines, slots and coupling blocks;
   // chunk c covers lines [fused_line[c], fused_line[c+1]) and slots [fused_slot[c], fused_slot[c+1]).
   P.fused_chunks = 0;
@@ -416,7 +416,7 @@ mmp_status prepare_problem(const mmp_params* in, const mmp_options& opt, Problem
   P.fused_line.clear();
   P.fused_slot.clear();
   const int nl = (int)P.line_a.size();
-  if (P.raman && P.N1 == 64 && (int)M <= colstage_max_modes() && dense_gather_chunk((int)M) > 0 && nl > 0 &&
+  if (P.raman && P.N1 == 64 && (int)M <= colstage_max_modes() && nl > 0 &&
       P.line_slot0.size() == (size_t)nl + 1 && P.line_slot0[0] == 0 && P.line_slot0[nl] == P.P) {
     const int max_lines = colstage_lines_per_chunk();
     const int block = P.parity ? (int)(M * ((M + 1) / 2)) : (int)(M * M);  // coefficients per slot

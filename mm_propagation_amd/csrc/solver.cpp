@@ -97,7 +97,7 @@ struct mmp_solver {
   FusedArgs fused{};
   bool use_fused = false;
   bool chain = false;        // the step close also issues the next step's stage-0 head
-  double* qdense = nullptr;  // [P][M][M] dense coupling blocks for the register gather (M <= 12)
+  double* qdense = nullptr;  // [P][M][M] (or parity-compact) dense coupling blocks for the register gathers
   // work arrays of the step in the work precision (the FP64 buffers above in FP64 mode)
   bool mixed = false;
   size_t wbytes = 16;        // bytes per complex work value
@@ -404,7 +404,7 @@ size_t planned_device_bytes(const Problem& P, const mmp_options& opt, size_t F) 
     b += 4 * (P.fused_line.size() + P.fused_slot.size() + c(P.P));
   const bool fused = P.fused_chunks > 0 && !(opt.flags & MMP_FLAG_SEPARATE_COLUMNS);
   const size_t block = (fused && P.parity) ? (size_t)P.M * ((P.M + 1) / 2) : (size_t)P.M * P.M;
-  if (dense_gather_chunk((int)P.M) > 0) b += (mixed ? 4 : 8) * (size_t)P.P * block * F;
+  if (fused || dense_gather_chunk((int)P.M) > 0) b += (mixed ? 4 : 8) * (size_t)P.P * block * F;
   return b;
 }
 
@@ -570,7 +570,8 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
   const size_t conv_n = (size_t)(s->n_lines ? s->n_lines : 1) * P.N;
   bool any_stash = false;
   for (int j = 0; j < P.rk.s; j++) any_stash = any_stash || P.stash_needed[j];
-  const bool dense = dense_gather_chunk((int)P.M) > 0;
+  // dense blocks: the column-stage gather always takes them, the separate gather up to its own limit
+  const bool dense = s->use_fused || dense_gather_chunk((int)P.M) > 0;
   s->batch.sf = MN;
   s->batch.sl = conv_n;
   s->batch.se = s->ntab * MN;

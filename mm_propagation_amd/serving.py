@@ -3,7 +3,7 @@
 Requests (a SimulationParameters and a step count, arriving over time) are admitted by device
 memory footprint into
 
-* batch slots: requests the batched kernels take (Raman on, 64-sample columns, M <= 12) are
+* batch slots: requests the batched kernels take (Raman on, 64-sample columns, M <= 16) are
   grouped by structure (n_modes, nt, tableau, coupling pattern) into one batched solver per
   group with a fixed number of slots; a new request is loaded into a free slot of an idle
   group (mmp_fiber_load) and joins the group's next chunk of steps;
@@ -132,7 +132,7 @@ class ServingEngine:
         return self._plans[req.rid]
 
     def _batchable(self, req: Request, info) -> bool:
-        return req.raman and int(info["n1"]) == 64 and int(info["n_modes"]) <= 12
+        return req.raman and int(info["n1"]) == 64 and int(info["n_modes"]) <= 16
 
     def _admit(self, req: Request, now: float) -> bool:
         info = self.plan_for(req)

@@ -31,7 +31,7 @@ def single(sp, steps, precision=_capi.MMP_PRECISION_FP64):
 
 
 @pytest.mark.parametrize("precision", [_capi.MMP_PRECISION_FP64, _capi.MMP_PRECISION_MIXED])
-@pytest.mark.parametrize("M,nt", [(3, 4096), (6, 8192), (10, 16384)])
+@pytest.mark.parametrize("M,nt", [(3, 4096), (6, 8192), (10, 16384), (15, 4096)])
 def test_batch_equals_single_solves(M, nt, precision):
     require_device()
     sps = fibers(M, nt, 3)

@@ -8,6 +8,7 @@ import pytest
 import oracle_binding as ob
 from gpu_util import note, require_device
 from mm_propagation_amd import _capi, configs
+from mm_propagation_amd.problem import NonZeroValue
 from mm_propagation_amd.solver import (MultimodePropagationGPU_RamanOFF, MultimodePropagationGPU_RamanON)
 
 pytestmark = pytest.mark.gpu
@@ -87,13 +88,23 @@ def test_two_mode_gaussian_full_length():
     assert np.all(err <= TOL_FULL), err
 
 
-@pytest.mark.parametrize("M,nt", [(2, 4096), (6, 16384), (10, 8192), (12, 4096), (15, 8192)])
-def test_fused_column_passes_match_separate_passes(M, nt):
+def break_parity(sp):
+    """Adds Q^(0)_{1,0,0} (k + l - m - p odd) so that no slot keeps a single parity of p + k."""
+    sp.coupling_coefficients[0].addNonZeroValue(NonZeroValue((1, 0, 0), 0.25 * sp.coupling_coefficients[0]
+                                                            .getNonZeroValue(0).value))
+    return sp
+
+
+@pytest.mark.parametrize("M,nt,broken", [(2, 4096, False), (6, 16384, False), (10, 8192, False), (12, 4096, False),
+                                         (13, 4096, False), (14, 4096, True), (15, 8192, False), (16, 4096, False)])
+def test_fused_column_passes_match_separate_passes(M, nt, broken):
     """The column-stage kernels (4 passes per stage) against the separate passes (7 per stage),
-    and both against the oracle; 10 and 12 modes run the Raman lines in several LDS chunks;
-    15 modes are beyond the column-stage kernels and fall back to the separate passes."""
+    and both against the oracle; 10 modes and more run the Raman lines in several LDS chunks;
+    `broken` couplings take the full M x M coupling blocks instead of the parity-compact ones."""
     require_device()
     sp = configs.two_mode_gaussian(nt=nt) if M == 2 else configs.grin_fiber(M=M, nt=nt, symmetric=True)
+    if broken:
+        sp = break_parity(sp)
     dec = 0 if M == 2 else 1
     note(f"case fused vs separate M={M} NT={nt}")
     a = MultimodePropagationGPU_RamanON(sp, pair_decode=dec)
