@@ -105,15 +105,17 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_row_head(Geometry g, con
 
   // issue every load this lane needs before the first barrier
   cw v[8];
-  cd psi[8], accin[8], ep[8], ec[8], e1[8];
+  cd x[8], accb[8], ep[8], ec[8], e1[8];
   const cd zero = {0., 0.};
 #pragma unroll
   for (int i = 0; i < 8; i++) {
     const size_t idx = base + i * C::T;
     v[i] = (valid && a.has_prev) ? ysig[idx] : cw{0, 0};
     ep[i] = (valid && a.has_prev) ? e_prev[idx] : zero;
-    psi[i] = (valid && (a.prev_is_first || !a.final_stage)) ? psi_hat[idx] : zero;
-    accin[i] = (valid && a.has_prev && !a.prev_is_first) ? accp[idx] : zero;
+    const cd psi = (valid && (a.prev_is_first || !a.final_stage)) ? psi_hat[idx] : zero;
+    const cd accin = (valid && a.has_prev && !a.prev_is_first) ? accp[idx] : zero;
+    accb[i] = a.prev_is_first ? psi : accin;  // the step result so far
+    x[i] = psi;                               // stage i's argument (not the final stage)
     e1[i] = (valid && a.update_prev) ? e1p[idx] : zero;
     ec[i] = (valid && (!a.final_stage || a.chain_next)) ? e_cur[idx] : zero;
   }
@@ -121,46 +123,49 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_row_head(Geometry g, con
     b.st[fib].z += fp.h;  // z += h (repeated FP64 addition)
     b.st[fib].steps += 1;
   }
-  cd nh[8];
-  const double bh = hc[0];
-  if (a.has_prev) {
-    line_fft<L, -1, 1, 1>(v, sm, lt, t);
+  // stage i's argument psi + sum_j a_ij h N_j over j < i in increasing j: the stashed N_j now,
+  // N_{i-1} (the largest j, from this head's transform) last
+  const bool prev_term = a.n_a > 0 && a.a_idx[a.n_a - 1] == a.prev_stage;
+  if (!a.final_stage) {
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      const cd x = cmulc(widen(v[i]), ep[i]);      // sigma_hat * conj(E_{i-1})
-      nh[i] = {-x.im * fp.sgn, x.re * fp.sgn};     // * (i sgn)
-      cd acc = a.prev_is_first ? psi[i] : accin[i];
-      if (a.b_prev != 0.) acc = cadd(acc, cscale(nh[i], bh));
-      accin[i] = acc;
-    }
-    if (valid) {
+    for (int e = 0; e < kMaxStages; e++) {
+      if (e >= a.n_a - (prev_term ? 1 : 0)) break;
+      const int j = a.a_idx[e];
 #pragma unroll
       for (int i = 0; i < 8; i++) {
-        const size_t idx = base + i * C::T;
-        if (a.final_stage) psi_hat[idx] = accin[i];
-        else accp[idx] = accin[i];
-        if (a.stash_prev) stash[(size_t)a.prev_stage * MN + idx] = nh[i];
-        if (a.update_prev) e_prev[idx] = cmul_exact(ep[i], e1[i]);
+        const cd nj = valid ? stash[(size_t)j * MN + base + i * C::T] : zero;
+        x[i] = cadd(x[i], cscale(nj, hc[1 + e]));
       }
     }
   }
-  if (a.final_stage && !a.chain_next) return;
-  // stage i's product, or (chain_next) stage 0 of the next step from the new psi_hat
+  // the Lawson table of stage i-1 moves on by E1 (it is not read again in this step)
+  if (a.has_prev && a.update_prev && valid) {
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    const size_t idx = base + i * C::T;
-    cd x = a.final_stage ? accin[i] : psi[i];
-#pragma unroll
-    for (int e = 0; e < kMaxStages; e++) {
-      if (e >= a.n_a) break;
-      const int j = a.a_idx[e];
-      cd nj;
-      if (j == a.prev_stage) nj = nh[i];
-      else nj = valid ? stash[(size_t)j * MN + idx] : zero;
-      x = cadd(x, cscale(nj, hc[1 + e]));
-    }
-    v[i] = narrow<W>(cscale(cmul(ec[i], x), a.scale_cur));
+    for (int i = 0; i < 8; i++) e_prev[base + i * C::T] = cmul_exact(ep[i], e1[i]);
   }
+  if (a.has_prev) {
+    const double bh = hc[0];
+    const double cprev = prev_term ? hc[a.n_a] : 0.;
+    line_fft<L, -1, 1, 1>(v, sm, lt, t);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const size_t idx = base + i * C::T;
+      const cd y = cmulc(widen(v[i]), ep[i]);     // sigma_hat * conj(E_{i-1})
+      const cd nh = {-y.im * fp.sgn, y.re * fp.sgn};  // * (i sgn) = N_{i-1}
+      cd acc = accb[i];
+      if (a.b_prev != 0.) acc = cadd(acc, cscale(nh, bh));
+      if (valid) {
+        if (a.final_stage) psi_hat[idx] = acc;
+        else accp[idx] = acc;
+        if (a.stash_prev) stash[(size_t)a.prev_stage * MN + idx] = nh;
+      }
+      if (a.final_stage) x[i] = acc;  // chain_next: stage 0 of the next step from the new psi_hat
+      else if (prev_term) x[i] = cadd(x[i], cscale(nh, cprev));
+    }
+  }
+  if (a.final_stage && !a.chain_next) return;
+#pragma unroll
+  for (int i = 0; i < 8; i++) v[i] = narrow<W>(cscale(cmul(ec[i], x[i]), a.scale_cur));
   line_fft<L, 1, 1, 1>(v, sm, lt, t);
   if (valid) {
 #pragma unroll
