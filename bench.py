@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the Lawson-RK4 propagation step on one MI355X.
 
-Workload (one GPU): the 6-mode GRIN fiber in plain FP64 -- M = 6 modes,
+Default workload (one GPU): the 6-mode GRIN fiber in plain FP64 -- M = 6 modes,
 NT = 16384 samples, 20000 RK4 steps over 1 m, Raman on, fully symmetric GRIN
 coupling tensor (nnz = 648, 36 pair slots), physical pair decode. A "step" is
 one Lawson-RK4 step of the whole M x NT field (4 stages: 4 nonlinearity
@@ -13,6 +13,11 @@ Prints one JSON line (metric, value in M*NT*steps/s, roofline of the dominant
 kernel from HIP-event timings, CPU baseline of the FP64 C restatement on a
 bounded sample, and the per-mode rel-L2 of the device field against the
 oracle's over that same sample).
+
+Other workloads (--workload): gaussian2 (the 2-mode Gaussian, FP64, compared with the oracle
+over all 5000 steps), grin10_mixed (M = 10, NT = 65536, mixed precision: the oracle over a
+20-step prefix, the plain FP64 solver at z = L), batched (256 independent fibers in batches)
+and serving (Poisson arrivals through the continuous-batching engine).
 """
 from __future__ import annotations
 
@@ -45,19 +50,37 @@ def measured_traffic():
     return {}
 
 
-def workload():
-    from mm_propagation_amd import configs
-    return configs.grin_fiber(M=6, nt=16384, n_steps=20000, symmetric=True)
+def single_workloads():
+    """Single-fiber workloads: (parameters, precision, pair decode, oracle prefix steps, description)."""
+    from mm_propagation_amd import _capi, configs
+    return {
+        "grin6": (lambda: configs.grin_fiber(M=6, nt=16384, n_steps=20000, symmetric=True),
+                  _capi.MMP_PRECISION_FP64, _capi.MMP_PAIRS_PHYSICAL, 200,
+                  "grin6_fp64: M=6, NT=16384, T_w=20 ps, L=1 m, n_steps=20000, RK4, Raman on, "
+                  "symmetric GRIN coupling nnz=648, physical pair decode"),
+        "gaussian2": (lambda: configs.two_mode_gaussian(),
+                      _capi.MMP_PRECISION_FP64, _capi.MMP_PAIRS_REFERENCE_CPU, 5000,
+                      "gaussian2_fp64: M=2, NT=4096, T_w=10 ps, L=0.5 m, n_steps=5000, RK4, Raman on, "
+                      "3 nonzeros per mode, the reference CPU pair decode"),
+        # peak powers 1-100 W: at 10-1000 W this 10-mode fiber is not resolved by 20000 steps (the
+        # oracle's field, like the device's, is no longer finite by step 325 at NT = 16384)
+        "grin10_mixed": (lambda: configs.grin_fiber(M=10, nt=65536, n_steps=20000, symmetric=True,
+                                                    power_range=(1.0, 100.0)),
+                         _capi.MMP_PRECISION_MIXED, _capi.MMP_PAIRS_PHYSICAL, 20,
+                         "grin10_mixed: M=10, NT=65536, T_w=20 ps, L=1 m, n_steps=20000, RK4, Raman on, "
+                         "symmetric GRIN coupling nnz=5000, peak powers log-uniform in [1, 100] W, physical "
+                         "pair decode, FP32 transforms and sums with FP64 state, phases and accumulation"),
+    }
 
 
-def cpu_baseline(sp, steps: int):
+def cpu_baseline(sp, steps: int, decode: int, name: str):
     """Time the FP64 C restatement of the reference CPU path on a bounded sample: the first
     `steps` steps of the solve from z = 0 (computeLawsonRK(steps), operator set-up included).
     Returns the baseline record and the oracle's field at z = steps * h."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import oracle_binding as ob
     threads = min(16, os.cpu_count() or 1)
-    o = ob.Oracle(sp, pair_decode=1, n_threads=threads, parallel_pairs=True)
+    o = ob.Oracle(sp, pair_decode=decode, n_threads=threads, parallel_pairs=True)
     t0 = time.perf_counter()
     o.compute_lawson_rk(steps)
     dt = time.perf_counter() - t0
@@ -65,18 +88,19 @@ def cpu_baseline(sp, steps: int):
     o.close()
     rec = {"value": sp.n_modes * sp.nt * steps / dt, "unit": "M·NT·steps/s", "cores": threads,
            "kind": "port",
-           "sample": f"grin6 M=6 NT=16384: first {steps} of 20000 RK4 steps in {dt:.2f} s, OpenMP {threads} "
-                     "threads over modes and Raman pairs, radix-2 FFT"}
+           "sample": f"{name} M={sp.n_modes} NT={sp.nt}: first {steps} of {sp.n_steps} RK4 steps in {dt:.2f} s, "
+                     f"OpenMP {threads} threads over modes and Raman pairs, radix-2 FFT"}
     return rec, fields
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default: the workload's n_steps)")
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--profile-steps", type=int, default=400)
-    ap.add_argument("--cpu-steps", type=int, default=200)
+    ap.add_argument("--cpu-steps", type=int, default=None,
+                    help="oracle prefix for the CPU baseline and the accuracy check (default per workload)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels one by one instead of replaying HIP graphs (for kernel tracers)")
@@ -84,8 +108,11 @@ def main() -> int:
                     help="every column pass a kernel of its own (7 per stage instead of 4)")
     ap.add_argument("--mixed", action="store_true",
                     help="mixed precision (FP32 transforms and sums, FP64 state); the bench line is FP64 by default")
-    ap.add_argument("--workload", choices=["grin6", "batched", "serving"], default="grin6",
-                    help="grin6: the 6-mode GRIN fiber in FP64 (default); batched: 256 independent GRIN fibers "
+    ap.add_argument("--workload", choices=["grin6", "gaussian2", "grin10_mixed", "batched", "serving"],
+                    default="grin6",
+                    help="grin6: the 6-mode GRIN fiber in FP64 (default); gaussian2: the 2-mode Gaussian in FP64; "
+                         "grin10_mixed: the 10-mode GRIN fiber at NT=65536 in mixed precision; "
+                         "batched: 256 independent GRIN fibers "
                          "(M in {3, 6, 10}, NT in {4096, 8192, 16384}) in batches per (M, NT); serving: Poisson "
                          "arrivals of solves (M in {1, 3, 6, 10, 15}, NT 2^12..2^16, 500..20000 steps) through "
                          "the continuous-batching engine")
@@ -111,13 +138,17 @@ def main() -> int:
     from mm_propagation_amd import _capi, configs
     from mm_propagation_amd.solver import Context, MultimodePropagationGPU_RamanON
 
-    sp = workload()
+    make, prec, decode, cpu_steps, desc = single_workloads()[args.workload]
+    sp = make()
+    if args.mixed:
+        prec = _capi.MMP_PRECISION_MIXED
+    steps = args.steps if args.steps is not None else sp.n_steps
+    cpu_steps = args.cpu_steps if args.cpu_steps is not None else cpu_steps
+    mixed = prec == _capi.MMP_PRECISION_MIXED
     M, NT = sp.n_modes, sp.nt
     ctx = Context(0)
     flags = (_capi.MMP_FLAG_NO_GRAPH if args.no_graph else 0) | (_capi.MMP_FLAG_SEPARATE_COLUMNS if args.separate_columns else 0)
-    prec = _capi.MMP_PRECISION_MIXED if args.mixed else _capi.MMP_PRECISION_FP64
-    s = MultimodePropagationGPU_RamanON(sp, pair_decode=_capi.MMP_PAIRS_PHYSICAL, flags=flags, ctx=ctx,
-                                        precision=prec)
+    s = MultimodePropagationGPU_RamanON(sp, pair_decode=decode, flags=flags, ctx=ctx, precision=prec)
     psi0, _ = s.get_state()
     if args.warmup:
         s.advance(args.warmup)
@@ -125,19 +156,30 @@ def main() -> int:
     s.set_state(psi0)  # back to z = 0 with the initial field
     ctx.synchronize()
     t0 = time.perf_counter()
-    s.advance(args.steps)
+    s.advance(steps)
     ctx.synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0
-    value = M * NT * args.steps / elapsed
+    value = M * NT * steps / elapsed
     fields = s.getResult()
     energy = s.energy()
     e0 = configs.energy_per_mode(sp.initial_fields, sp.time_window)
-    accuracy = {"energy_drift_rel": float(abs(energy.sum() - e0.sum()) / e0.sum())}
+    accuracy = {"energy_drift_rel": float(abs(energy.sum() - e0.sum()) / e0.sum()),
+                "energy_note": "total energy at z = steps * h against the initial field's (the Raman term moves "
+                               "energy between frequencies, so it is not an invariant of this physics)"}
     s.close()
+    if mixed and steps == sp.n_steps:
+        # the plain FP64 solver over the same steps: the mixed field and energy at z = L against it
+        f64 = MultimodePropagationGPU_RamanON(sp, pair_decode=decode, ctx=ctx)
+        f64.computeLawsonRK(steps)
+        ref64 = f64.getResult()
+        en64 = f64.energy()
+        f64.close()
+        accuracy["rel_l2_vs_fp64_gpu_at_L_max"] = float(configs.rel_l2_per_mode(fields, ref64).max())
+        accuracy["energy_drift_vs_fp64_gpu_at_L"] = float((energy.sum() - en64.sum()) / en64.sum())
 
     # per-kernel HIP-event timings on the solver's stream over a profiled run of the same workload
-    ps = MultimodePropagationGPU_RamanON(sp, pair_decode=_capi.MMP_PAIRS_PHYSICAL,
+    ps = MultimodePropagationGPU_RamanON(sp, pair_decode=decode,
                                          flags=_capi.MMP_FLAG_PROFILE | (flags & _capi.MMP_FLAG_SEPARATE_COLUMNS), ctx=ctx,
                                          precision=prec)
     ps.advance(8)
@@ -149,7 +191,8 @@ def main() -> int:
     dom = max(kt, key=lambda k: k["total_ms"])
     avg_s = dom["total_ms"] / 1e3 / max(dom["launches"], 1)
     achieved = dom["bytes_per_launch"] / avg_s / 1e9
-    traffic = measured_traffic().get(dom["name"]) if not (args.mixed or args.separate_columns) else None
+    traffic = (measured_traffic().get(dom["name"])
+               if args.workload == "grin6" and not (mixed or args.separate_columns) else None)
     roofline = {"bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "traffic_source": "profiles/hbm_traffic_grin6_fp64.md (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, "
@@ -163,26 +206,29 @@ def main() -> int:
                             for k in kt}}
 
     cpu = None
-    if not args.no_cpu_baseline and args.cpu_steps > 0:
-        cpu, ref = cpu_baseline(sp, args.cpu_steps)
+    if not args.no_cpu_baseline and cpu_steps > 0:
+        cpu, ref = cpu_baseline(sp, cpu_steps, decode, args.workload)
         # the same solve prefix on the device, compared field for field with the oracle
-        vs = MultimodePropagationGPU_RamanON(sp, pair_decode=_capi.MMP_PAIRS_PHYSICAL, flags=flags, ctx=ctx,
-                                             precision=prec)
-        vs.computeLawsonRK(args.cpu_steps)
-        err = configs.rel_l2_per_mode(vs.getResult(), ref)
+        vs = MultimodePropagationGPU_RamanON(sp, pair_decode=decode, flags=flags, ctx=ctx, precision=prec)
+        vs.computeLawsonRK(cpu_steps)
+        got = vs.getResult()
+        en = vs.energy()
         vs.close()
-        accuracy["rel_l2_vs_oracle_at_steps"] = args.cpu_steps
+        err = configs.rel_l2_per_mode(got, ref)
+        e_ref = configs.energy_per_mode(ref, sp.time_window)
+        accuracy["rel_l2_vs_oracle_at_steps"] = cpu_steps
         accuracy["rel_l2_vs_oracle_max"] = float(err.max())
         accuracy["rel_l2_vs_oracle_per_mode"] = [float(x) for x in err]
-        accuracy["note"] = ("fields compared after the first cpu_steps steps: this GRIN workload is chaotic, "
-                            "two FP64 solvers that differ only in FFT rounding separate to 1e-4 by step 500")
+        accuracy["energy_drift_vs_oracle_at_steps"] = float((en.sum() - e_ref.sum()) / e_ref.sum())
+        if args.workload != "gaussian2":
+            accuracy["note"] = ("fields compared after the first cpu_steps steps: this GRIN workload is chaotic, "
+                                "two FP64 solvers that differ only in FFT rounding separate to 1e-4 by step 500")
 
     out = {
-        "metric": METRIC, "value": value, "unit": "M·NT·steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if args.mixed else "f64", "data": "synthetic",
-        "config": {"workload": "grin6_fp64: M=6, NT=16384, T_w=20 ps, L=1 m, n_steps=20000, RK4, Raman on, "
-                               "symmetric GRIN coupling nnz=648, physical pair decode",
+        "metric": METRIC, "value": value, "unit": "M·NT·steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if mixed else "f64", "data": "synthetic",
+        "config": {"workload": desc + (" (run in mixed precision)" if args.mixed else ""),
                    "global_batch": 1, "seq_len": NT, "parallelism": "single GPU"},
         "accuracy": accuracy, "roofline": roofline, "cpu_baseline": cpu,
     }
@@ -214,7 +260,7 @@ def run_batched(args) -> int:
         batches.append((key, classes[key], ctx, b))
     units = sum(sp.n_modes * sp.nt for sp in fibers)  # M * NT summed over the fibers, per step
     chunk = 16
-    steps = max(chunk, (args.steps // chunk) * chunk)
+    steps = max(chunk, ((args.steps if args.steps is not None else 2000) // chunk) * chunk)
 
     def run(n):
         for _ in range(n // chunk):
@@ -268,7 +314,10 @@ def run_batched(args) -> int:
         for key, idx, ctx, b in batches:
             if key[1] != 4096:
                 continue
-            for i in idx[:6]:  # the first fiber of the class whose 100-step field stays finite
+            # the class's fibers from the lowest peak power up: the first whose 100-step field stays
+            # finite (the 10 kW pulses are not resolved by these step sizes, in FP64 either)
+            by_power = sorted(idx, key=lambda i: float(np.max(np.abs(fibers[i].initial_fields) ** 2)))
+            for i in by_power[:6]:
                 sp = fibers[i]
                 s1 = MultimodePropagationBatch([sp], pair_decode=_capi.MMP_PAIRS_PHYSICAL, precision=prec, ctx=ctx)
                 s1.computeLawsonRK(100)
