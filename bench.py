@@ -104,6 +104,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels one by one instead of replaying HIP graphs (for kernel tracers)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="MMP_FLAG_AUTOTUNE: each solver times its gather variants at creation and keeps the fastest")
     ap.add_argument("--separate-columns", action="store_true",
                     help="every column pass a kernel of its own (7 per stage instead of 4)")
     ap.add_argument("--mixed", action="store_true",
@@ -147,8 +149,11 @@ def main() -> int:
     mixed = prec == _capi.MMP_PRECISION_MIXED
     M, NT = sp.n_modes, sp.nt
     ctx = Context(0)
-    flags = (_capi.MMP_FLAG_NO_GRAPH if args.no_graph else 0) | (_capi.MMP_FLAG_SEPARATE_COLUMNS if args.separate_columns else 0)
+    flags = ((_capi.MMP_FLAG_NO_GRAPH if args.no_graph else 0) |
+             (_capi.MMP_FLAG_SEPARATE_COLUMNS if args.separate_columns else 0) |
+             (_capi.MMP_FLAG_AUTOTUNE if args.autotune else 0))
     s = MultimodePropagationGPU_RamanON(sp, pair_decode=decode, flags=flags, ctx=ctx, precision=prec)
+    tuned = {k: s.info()[k] for k in ("gather_waves", "chunk_lines", "tuned")}
     psi0, _ = s.get_state()
     if args.warmup:
         s.advance(args.warmup)
@@ -180,8 +185,9 @@ def main() -> int:
 
     # per-kernel HIP-event timings on the solver's stream over a profiled run of the same workload
     ps = MultimodePropagationGPU_RamanON(sp, pair_decode=decode,
-                                         flags=_capi.MMP_FLAG_PROFILE | (flags & _capi.MMP_FLAG_SEPARATE_COLUMNS), ctx=ctx,
-                                         precision=prec)
+                                         flags=_capi.MMP_FLAG_PROFILE |
+                                         (flags & (_capi.MMP_FLAG_SEPARATE_COLUMNS | _capi.MMP_FLAG_AUTOTUNE)),
+                                         ctx=ctx, precision=prec)
     ps.advance(8)
     ps.kernel_times()
     ps.advance(args.profile_steps)
@@ -230,7 +236,7 @@ def main() -> int:
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if mixed else "f64", "data": "synthetic",
         "config": {"workload": desc + (" (run in mixed precision)" if args.mixed else ""),
                    "global_batch": 1, "seq_len": NT, "parallelism": "single GPU"},
-        "accuracy": accuracy, "roofline": roofline, "cpu_baseline": cpu,
+        "accuracy": accuracy, "roofline": roofline, "cpu_baseline": cpu, "gather_plan": tuned,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
@@ -253,11 +259,15 @@ def run_batched(args) -> int:
     for i, sp in enumerate(fibers):
         classes.setdefault((sp.n_modes, sp.nt), []).append(i)
     batches = []
+    bflags = _capi.MMP_FLAG_AUTOTUNE if args.autotune else 0
+    t_tune = time.perf_counter()
     for key in sorted(classes):
         ctx = Context(0)
         b = MultimodePropagationBatch([fibers[i] for i in classes[key]], pair_decode=_capi.MMP_PAIRS_PHYSICAL,
-                                      precision=prec, ctx=ctx)
+                                      precision=prec, ctx=ctx, flags=bflags)
         batches.append((key, classes[key], ctx, b))
+    t_tune = time.perf_counter() - t_tune
+    plans = {f"M{k[0]}_NT{k[1]}": [b.info(0)["gather_waves"], b.info(0)["chunk_lines"]] for k, _, _, b in batches}
     units = sum(sp.n_modes * sp.nt for sp in fibers)  # M * NT summed over the fibers, per step
     chunk = 16
     steps = max(chunk, ((args.steps if args.steps is not None else 2000) // chunk) * chunk)
@@ -337,7 +347,7 @@ def run_batched(args) -> int:
     # roofline of the dominant kernel in the largest class, with dispatch-recorded events
     key, idx, ctx, _ = max(batches, key=lambda t: t[0][0] * t[0][1] * len(t[1]))
     pb = MultimodePropagationBatch([fibers[i] for i in idx], pair_decode=_capi.MMP_PAIRS_PHYSICAL, precision=prec,
-                                   flags=_capi.MMP_FLAG_PROFILE, ctx=ctx)
+                                   flags=_capi.MMP_FLAG_PROFILE | bflags, ctx=ctx)
     pb.advance(4)
     pb.kernel_times()
     pb.advance(32)
@@ -364,6 +374,8 @@ def run_batched(args) -> int:
         "accuracy": {"rel_l2_vs_fp64_oracle_after_100_steps_max": acc,
                      "fibers_nonfinite_after_timed_steps": nonfinite},
         "roofline": roofline,
+        "gather_plans": {"waves_and_lines_per_class": plans, "autotune": bool(args.autotune),
+                         "creation_s": round(t_tune, 2)},
     }
     print(json.dumps(out), flush=True)
     for _, _, ctx, b in batches:

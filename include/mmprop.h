@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MMPROP_ABI_VERSION 2
+#define MMPROP_ABI_VERSION 3
 
 /* Only the entry points below are exported; the library is built with hidden visibility. */
 #if defined(__GNUC__) || defined(__clang__)
@@ -67,9 +67,14 @@ enum {
   MMP_FLAG_NONE = 0,
   MMP_FLAG_PROFILE = 1u << 0,     /* record per-kernel HIP event times (see mmp_kernel_times) */
   MMP_FLAG_NO_GRAPH = 1u << 1,    /* launch step kernels one by one instead of replaying a HIP graph */
-  MMP_FLAG_SEPARATE_COLUMNS = 1u << 2 /* keep every column pass a kernel of its own (7 passes per stage)
-                                         where the column-stage kernels would run (4 passes per stage:
-                                         Raman on, 64-sample columns, n_modes <= 16) */
+  MMP_FLAG_SEPARATE_COLUMNS = 1u << 2, /* keep every column pass a kernel of its own (7 passes per stage)
+                                          where the column-stage kernels would run (4 passes per stage:
+                                          Raman on, 64-sample columns, n_modes <= 16) */
+  MMP_FLAG_AUTOTUNE = 1u << 3 /* at creation, time the column-stage gather variants (wavefronts per
+                                 workgroup, Raman lines per LDS chunk) on this solver's own data and
+                                 keep the fastest; the choice is cached per structure for the process.
+                                 The solver is left at z = 0 with its initial field. Results stay within
+                                 rounding of the untuned solver (the sum order over slots changes). */
 };
 
 /*
@@ -123,6 +128,10 @@ typedef struct mmp_info {
   size_t device_bytes;       /* device memory held by the solver (all fibers) */
   uint32_t n_fibers;         /* fibers in the solver (1 unless made by mmp_batch_create) */
   uint32_t live;             /* 1 when the steps advance this fiber, 0 when it is parked */
+  uint32_t gather_waves;     /* column-stage gather: wavefronts per workgroup (0: separate passes) */
+  uint32_t chunk_lines;      /* column-stage gather: Raman lines per LDS chunk at most */
+  uint32_t tuned;            /* 1 when MMP_FLAG_AUTOTUNE chose the two above */
+  uint32_t reserved;
 } mmp_info;
 
 typedef struct mmp_ctx mmp_ctx;

@@ -39,6 +39,7 @@ MMP_PAIRS_PHYSICAL = 1
 MMP_FLAG_PROFILE = 1 << 0
 MMP_FLAG_NO_GRAPH = 1 << 1
 MMP_FLAG_SEPARATE_COLUMNS = 1 << 2
+MMP_FLAG_AUTOTUNE = 1 << 3
 
 
 class MmpParams(ctypes.Structure):
@@ -82,6 +83,7 @@ class MmpInfo(ctypes.Structure):
         ("max_beta2", c_double), ("max_q", c_double),
         ("device_bytes", c_size_t),
         ("n_fibers", c_uint32), ("live", c_uint32),
+        ("gather_waves", c_uint32), ("chunk_lines", c_uint32), ("tuned", c_uint32), ("reserved", c_uint32),
     ]
 
 

@@ -158,11 +158,14 @@ struct FusedArgs {
   int n_chunks, max_chunk_lines, max_chunk_slots;
   int parity;                  // coupling blocks hold only the entries (p, k) with (p + k) % 2 == the
                                // slot's parity: M x ceil(M/2) per slot, k = 2 kk + ((p + parity) % 2)
+  int waves;                   // wavefronts of the gather workgroup: 4, or 8 (colstage_wide_ok)
 };
 
 // Limits of one column-stage chunk: Raman lines, slots, and coupling coefficients
 // (slots x M x M) held in LDS; and the largest mode count.
 int colstage_max_modes();
+// Whether the 8-wavefront gather workgroup exists for M modes in this precision.
+bool colstage_wide_ok(int M, bool mixed);
 int colstage_lines_per_chunk();
 int colstage_slots_per_chunk();
 int colstage_coefficients_per_chunk();

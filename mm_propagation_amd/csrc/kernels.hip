@@ -624,6 +624,12 @@ struct ColStageCfg {
 };
 
 int colstage_max_modes() { return kColStageMaxM; }
+
+// 8-wavefront gather workgroups: up to 12 modes in FP32, 6 in FP64 (beyond, the registers of
+// a 512-lane workgroup would spill)
+template <int M, class W>
+constexpr bool kColStageWide = std::is_same<W, float>::value ? M <= 12 : M <= 6;
+bool colstage_wide_ok(int M, bool mixed) { return mixed ? M <= 12 : M <= 6; }
 int colstage_lines_per_chunk() { return kColStageLines; }
 int colstage_coefficients_per_chunk() { return kColStageQCap; }
 int colstage_slots_per_chunk() { return kColStageSCap; }
@@ -1184,12 +1190,14 @@ hipError_t launch_colstage_fields_pairs(const Geometry& g, const Twiddles& tw, c
 }
 
 #define MMP_COLSTAGE_LAUNCH(MV, PARV)                                                                             \
-  if constexpr (MV <= 6)                                                                                          \
-    MMP_LAUNCH((k_colstage_raman_gather<MV, 8, W, PARV>), dim3(g.N2, F), dim3(512), 0, s, g, TwOf<W>::col(tw),   \
-               TwOf<W>::four(tw), U, conv, yout, ga, fa, Qd, scale, b);                                           \
-  else                                                                                                            \
-    MMP_LAUNCH((k_colstage_raman_gather<MV, 4, W, PARV>), dim3(g.N2, F), dim3(256), 0, s, g, TwOf<W>::col(tw),   \
-               TwOf<W>::four(tw), U, conv, yout, ga, fa, Qd, scale, b);
+  if constexpr (kColStageWide<MV, W>)                                                                             \
+    if (fa.waves == 8) {                                                                                          \
+      MMP_LAUNCH((k_colstage_raman_gather<MV, 8, W, PARV>), dim3(g.N2, F), dim3(512), 0, s, g, TwOf<W>::col(tw), \
+                 TwOf<W>::four(tw), U, conv, yout, ga, fa, Qd, scale, b);                                         \
+      break;                                                                                                      \
+    }                                                                                                             \
+  MMP_LAUNCH((k_colstage_raman_gather<MV, 4, W, PARV>), dim3(g.N2, F), dim3(256), 0, s, g, TwOf<W>::col(tw),     \
+             TwOf<W>::four(tw), U, conv, yout, ga, fa, Qd, scale, b);
 #define MMP_COLSTAGE_CASE(MV)                                                                                    \
   case MV:                                                                                                        \
     if (fa.parity) {                                                                                              \

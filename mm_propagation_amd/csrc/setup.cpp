@@ -407,18 +407,29 @@ mmp_status prepare_problem(const mmp_params* in, const mmp_options& opt, Problem
   }
   P.kv = P.raman ? 1 - P.fR : 1.;
   P.kr = P.raman ? P.fR * P.T0 : 0.;
-  // Column-stage kernels (Raman on, N1 = 64, M <= 16, dense coupling blocks): Raman lines are
-  // processed in chunks that respect the LDS limits on lines, slots and coupling blocks;
-  // chunk c covers lines [fused_line[c], fused_line[c+1]) and slots [fused_slot[c], fused_slot[c+1]).
+  plan_colstage_chunks(P, colstage_lines_per_chunk());
+  // a stage result N_j needs a stash slot when a stage i > j + 1 reads it
+  P.stash_needed.assign(P.rk.s, 0);
+  for (int i = 0; i < P.rk.s; i++)
+    for (int j = 0; j + 1 < i; j++)
+      if (P.rk.a[i][j] != 0.) P.stash_needed[j] = 1;
+  return MMP_OK;
+}
+
+// Column-stage kernels (Raman on, N1 = 64, M <= 16, dense coupling blocks): Raman lines are
+// processed in chunks that respect the LDS limits on lines, slots and coupling blocks;
+// chunk c covers lines [fused_line[c], fused_line[c+1]) and slots [fused_slot[c], fused_slot[c+1]).
+bool plan_colstage_chunks(Problem& P, int max_lines) {
+  const uint32_t M = P.M;
   P.fused_chunks = 0;
   P.fused_max_lines = 0;
   P.fused_max_slots = 0;
   P.fused_line.clear();
   P.fused_slot.clear();
   const int nl = (int)P.line_a.size();
+  if (max_lines < 1 || max_lines > colstage_lines_per_chunk()) return false;
   if (P.raman && P.N1 == 64 && (int)M <= colstage_max_modes() && nl > 0 &&
       P.line_slot0.size() == (size_t)nl + 1 && P.line_slot0[0] == 0 && P.line_slot0[nl] == P.P) {
-    const int max_lines = colstage_lines_per_chunk();
     const int block = P.parity ? (int)(M * ((M + 1) / 2)) : (int)(M * M);  // coefficients per slot
     const int max_slots = std::min(colstage_slots_per_chunk(), colstage_coefficients_per_chunk() / block);
     bool ok = max_slots >= 2;
@@ -442,18 +453,13 @@ mmp_status prepare_problem(const mmp_params* in, const mmp_options& opt, Problem
     }
     if (ok) {
       P.fused_chunks = (int)P.fused_line.size() - 1;
-    } else {
-      P.fused_line.clear();
-      P.fused_slot.clear();
-      P.fused_max_lines = P.fused_max_slots = 0;
+      return true;
     }
+    P.fused_line.clear();
+    P.fused_slot.clear();
+    P.fused_max_lines = P.fused_max_slots = 0;
   }
-  // a stage result N_j needs a stash slot when a stage i > j + 1 reads it
-  P.stash_needed.assign(P.rk.s, 0);
-  for (int i = 0; i < P.rk.s; i++)
-    for (int j = 0; j + 1 < i; j++)
-      if (P.rk.a[i][j] != 0.) P.stash_needed[j] = 1;
-  return MMP_OK;
+  return false;
 }
 
 }  // namespace mmp

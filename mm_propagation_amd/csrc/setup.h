@@ -74,9 +74,13 @@ struct Problem {
 // Validates and prepares. On failure returns a status and fills err.
 mmp_status prepare_problem(const mmp_params* in, const mmp_options& opt, Problem& out, std::string& err);
 
-// Four-step split NT = N1 * N2: N1 = min(2^floor(log2(NT)/2), 64), raised to NT/256 when the
-// rows would exceed 256 samples; false if NT is unsupported.
+// Four-step split NT = N1 * N2: N1 = min(2^floor(log2(NT)/2), 64), raised to NT/1024 when the
+// rows would exceed 1024 samples; false if NT is unsupported.
 bool split_length(uint32_t nt, uint32_t& n1, uint32_t& n2);
+
+// Chunks of Raman lines (at most max_lines each, within the column-stage LDS limits) for the
+// column-stage kernels: fills P.fused_*; false (and no chunks) when those kernels do not apply.
+bool plan_colstage_chunks(Problem& P, int max_lines);
 
 // exp(-2*pi*i*j/n), j < n, evaluated in long double and rounded once.
 std::vector<double> twiddle_table(uint32_t n);

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -96,6 +97,8 @@ struct mmp_solver {
   PairLineArgs pairs{};
   FusedArgs fused{};
   bool use_fused = false;
+  int chunk_lines = 0;       // column-stage gather: Raman lines per LDS chunk at most
+  bool tuned = false;        // MMP_FLAG_AUTOTUNE chose fused.waves and chunk_lines
   bool chain = false;        // the step close also issues the next step's stage-0 head
   double* qdense = nullptr;  // [P][M][M] (or parity-compact) dense coupling blocks for the register gathers
   // work arrays of the step in the work precision (the FP64 buffers above in FP64 mode)
@@ -431,6 +434,9 @@ void fill_info(const Problem& P, const mmp_options& opt, mmp_info* o) {
   o->device_bytes = planned_device_bytes(P, opt, 1);
   o->n_fibers = 1;
   o->live = 1;
+  const bool fused = P.fused_chunks > 0 && !(opt.flags & MMP_FLAG_SEPARATE_COLUMNS);
+  o->gather_waves = fused ? (P.M <= 6 ? 8u : 4u) : 0u;  // the defaults, before any tuning
+  o->chunk_lines = fused ? (uint32_t)colstage_lines_per_chunk() : 0u;
 }
 
 // Fibers of one batch must share everything the kernels take from fiber 0's problem.
@@ -439,8 +445,7 @@ bool same_structure(const Problem& a, const Problem& b) {
          a.stage_table == b.stage_table && a.stash_needed == b.stash_needed && a.c_unique == b.c_unique &&
          a.slot_a == b.slot_a && a.slot_b == b.slot_b && a.slot_line == b.slot_line && a.slot_kind == b.slot_kind &&
          a.line_a == b.line_a && a.line_b == b.line_b && a.line_self == b.line_self &&
-         a.line_slot0 == b.line_slot0 && a.fused_line == b.fused_line && a.fused_slot == b.fused_slot &&
-         a.fused_chunks == b.fused_chunks && a.parity == b.parity && a.slot_parity == b.slot_parity;
+         a.line_slot0 == b.line_slot0 && a.parity == b.parity && a.slot_parity == b.slot_parity;
 }
 
 // Dense coupling blocks Qd[s][p][k] = sum of Q / max|Q| over the nonzeros (k, slot s) of Q^(p);
@@ -510,6 +515,8 @@ mmp_status load_fiber(mmp_solver* s, uint32_t f, Problem& Pf) {
   for (auto* v : {&Pf.imL_fourstep, &Pf.e1, &Pf.etab0, &Pf.psi0, &Pf.raman_response}) std::vector<double>().swap(*v);
   return MMP_OK;
 }
+
+mmp_status autotune(mmp_solver* s);
 
 mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params, const mmp_options* options,
                        mmp_solver** out) {
@@ -703,8 +710,9 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
   std::vector<uint32_t> slot_meta(P.P);
   if (s->use_fused) {
     uint32_t *d_cl, *d_cs, *d_meta;
-    A(dalloc(s, &d_cl, P.fused_line.size()));
-    A(dalloc(s, &d_cs, P.fused_slot.size()));
+    const size_t max_chunks = P.line_a.size() + 1;  // room for the finest chunking the tuner may pick
+    A(dalloc(s, &d_cl, max_chunks));
+    A(dalloc(s, &d_cs, max_chunks));
     A(dalloc(s, &d_meta, P1));
     if (st != MMP_OK) return cleanup(st);
     for (uint32_t j = 0; j < P.P; j++)
@@ -713,7 +721,9 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
     A(upload(s, d_cl, P.fused_line.data(), P.fused_line.size() * 4));
     A(upload(s, d_cs, P.fused_slot.data(), P.fused_slot.size() * 4));
     A(upload(s, d_meta, slot_meta.data(), P.P * 4));
-    s->fused = FusedArgs{d_cl, d_cs, d_meta, P.fused_chunks, P.fused_max_lines, P.fused_max_slots, compact ? 1 : 0};
+    s->fused = FusedArgs{d_cl,   d_cs, d_meta, P.fused_chunks, P.fused_max_lines, P.fused_max_slots, compact ? 1 : 0,
+                         P.M <= 6 ? 8 : 4};
+    s->chunk_lines = colstage_lines_per_chunk();
   }
   // stage heads: tableau coefficients here, each fiber's h, sgn and tables in the kernel
   // the close head also builds the next step's stage 0 unless the stage-0 Lawson table is
@@ -763,8 +773,122 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
     if (e != hipSuccess) A(hip_fail(e, "hipStreamSynchronize"));
   }
   for (uint32_t f = 0; f < F && st == MMP_OK; f++) A(load_fiber(s, f, s->fib[f]));
+  if (st == MMP_OK && (opt.flags & MMP_FLAG_AUTOTUNE) && s->use_fused) A(autotune(s));
   if (st != MMP_OK) return cleanup(st);
   *out = s;
+  return MMP_OK;
+}
+
+void drop_graph(mmp_solver* s) {
+  if (s->graph_exec) hipGraphExecDestroy(s->graph_exec);
+  if (s->graph) hipGraphDestroy(s->graph);
+  s->graph_exec = nullptr;
+  s->graph = nullptr;
+}
+
+// Column-stage gather variant: `waves` wavefronts per workgroup, at most `lines` Raman lines per
+// LDS chunk. The next step rebuilds the graph.
+mmp_status apply_colstage_plan(mmp_solver* s, int waves, int lines) {
+  if (!s->use_fused) return fail(MMP_ERR_UNSUPPORTED, "no column-stage kernels in this solver");
+  if (waves != 4 && !(waves == 8 && colstage_wide_ok((int)s->P.M, s->mixed)))
+    return fail(MMP_ERR_INVALID_ARGUMENT, "gather wavefronts must be 4, or 8 where that variant exists");
+  Problem& P = s->P;
+  if (!plan_colstage_chunks(P, lines)) return fail(MMP_ERR_INVALID_ARGUMENT, "no chunk plan for these limits");
+  MMP_TRY(upload(s, const_cast<uint32_t*>(s->fused.chunk_line), P.fused_line.data(), P.fused_line.size() * 4));
+  MMP_TRY(upload(s, const_cast<uint32_t*>(s->fused.chunk_slot), P.fused_slot.data(), P.fused_slot.size() * 4));
+  HIP_TRY(hipStreamSynchronize(s->ctx->stream), "hipStreamSynchronize");  // host staging copies
+  s->fused.n_chunks = P.fused_chunks;
+  s->fused.max_chunk_lines = P.fused_max_lines;
+  s->fused.max_chunk_slots = P.fused_max_slots;
+  s->fused.waves = waves;
+  s->chunk_lines = lines;
+  drop_graph(s);
+  return MMP_OK;
+}
+
+mmp_status advance_impl(mmp_solver* s, uint32_t n);
+
+// Autotuner: every (wavefronts, lines per chunk) variant of the column-stage gather that gives a
+// distinct plan runs 2 graphs of steps on this solver's own fibers after one warm-up graph;
+// the fastest is kept, and remembered per structure (modes, samples, stages, slots, lines,
+// parity, precision, fibers) for later solvers of the process. The fibers' spectra are saved
+// first and restored after, and z, steps and Lawson tables are reset, so the solver ends as
+// created.
+struct TuneKey {
+  uint32_t v[8];
+  bool operator<(const TuneKey& o) const { return std::lexicographical_compare(v, v + 8, o.v, o.v + 8); }
+};
+std::mutex g_tune_mutex;
+std::map<TuneKey, std::pair<int, int>> g_tune_cache;
+
+mmp_status autotune(mmp_solver* s) {
+  const Problem& P = s->P;
+  const TuneKey key{{P.M, P.N, (uint32_t)P.rk.s, P.P, (uint32_t)s->n_lines, (uint32_t)P.parity, s->mixed ? 1u : 0u,
+                     s->F}};
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      MMP_TRY(apply_colstage_plan(s, it->second.first, it->second.second));
+      s->tuned = true;
+      return MMP_OK;
+    }
+  }
+  const size_t state_bytes = (size_t)s->F * s->batch.sf * sizeof(cd);
+  void* saved = nullptr;
+  HIP_TRY(hipMalloc(&saved, state_bytes), "hipMalloc");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  mmp_status st = MMP_OK;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) st = fail(MMP_ERR_DEVICE, "hipEventCreate");
+  if (st == MMP_OK && hipMemcpyAsync(saved, s->psi_hat, state_bytes, hipMemcpyDeviceToDevice, s->ctx->stream) != hipSuccess)
+    st = fail(MMP_ERR_DEVICE, "hipMemcpyAsync D2D");
+  std::vector<std::pair<int, int>> cands;
+  std::vector<std::vector<uint32_t>> seen;
+  for (int waves : {4, 8}) {
+    if (waves == 8 && !colstage_wide_ok((int)P.M, s->mixed)) continue;
+    seen.clear();
+    for (int lines : {colstage_lines_per_chunk(), 24, 16}) {
+      Problem probe = P;
+      if (!plan_colstage_chunks(probe, lines)) continue;
+      if (std::find(seen.begin(), seen.end(), probe.fused_line) != seen.end()) continue;  // same plan
+      seen.push_back(probe.fused_line);
+      cands.push_back({waves, lines});
+    }
+  }
+  std::pair<int, int> best{s->fused.waves, s->chunk_lines};
+  float best_ms = 0.f;
+  for (size_t c = 0; c < cands.size() && st == MMP_OK; c++) {
+    st = apply_colstage_plan(s, cands[c].first, cands[c].second);
+    if (st == MMP_OK) st = advance_impl(s, kGraphSteps);  // warm-up, graph build
+    if (st == MMP_OK && hipEventRecord(e0, s->ctx->stream) != hipSuccess) st = fail(MMP_ERR_DEVICE, "hipEventRecord");
+    if (st == MMP_OK) st = advance_impl(s, 2 * kGraphSteps);
+    if (st == MMP_OK && hipEventRecord(e1, s->ctx->stream) != hipSuccess) st = fail(MMP_ERR_DEVICE, "hipEventRecord");
+    float ms = 0.f;
+    if (st == MMP_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+      st = fail(MMP_ERR_DEVICE, "hipEventElapsedTime");
+    if (st == MMP_OK && (c == 0 || ms < best_ms)) {
+      best_ms = ms;
+      best = cands[c];
+    }
+  }
+  if (st == MMP_OK) st = apply_colstage_plan(s, best.first, best.second);
+  if (st == MMP_OK && hipMemcpyAsync(s->psi_hat, saved, state_bytes, hipMemcpyDeviceToDevice, s->ctx->stream) != hipSuccess)
+    st = fail(MMP_ERR_DEVICE, "hipMemcpyAsync D2D");
+  if (st == MMP_OK) st = reset_z(s);  // synchronises the stream before `saved` is freed
+  if (st == MMP_OK) st = drain_profile(s);
+  if (st == MMP_OK) {
+    std::fill(s->kernel_ms.begin(), s->kernel_ms.end(), 0.);  // the tuning steps are not the caller's
+    std::fill(s->kernel_count.begin(), s->kernel_count.end(), 0);
+    std::fill(s->kernel_bytes.begin(), s->kernel_bytes.end(), 0.);
+  }
+  hipStreamSynchronize(s->ctx->stream);
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  hipFree(saved);
+  if (st != MMP_OK) return st;
+  s->tuned = true;
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  g_tune_cache[key] = best;
   return MMP_OK;
 }
 
@@ -1048,6 +1172,9 @@ mmp_status mmp_fiber_info(mmp_solver* s, uint32_t f, mmp_info* o) {
   o->device_bytes = s->device_bytes;
   o->n_fibers = s->F;
   o->live = s->live[f];
+  o->gather_waves = s->use_fused ? (uint32_t)s->fused.waves : 0u;
+  o->chunk_lines = s->use_fused ? (uint32_t)s->chunk_lines : 0u;
+  o->tuned = s->tuned ? 1u : 0u;
   return MMP_OK;
 }
 
