@@ -283,6 +283,68 @@ MMP_API mmp_status mmp_ctx_synchronize(mmp_ctx* ctx);
  * keep several contexts busy). */
 MMP_API mmp_status mmp_ctx_query(mmp_ctx* ctx);
 
+/* ------------------------------------------------------------------------------------------
+ * Request server (no reference counterpart): continuous batching of propagation solves of
+ * mixed n_modes, nt and step counts on one device. A request is admitted when its device
+ * footprint (mmp_plan) fits the budget: into a free slot of the batch group of its structure
+ * (requests the batched kernels take; mmp_fiber_load at the group's next chunk boundary) or a
+ * solver of its own. Every group and solo solver has its own stream, so their work overlaps;
+ * a group advances by the fewest steps any of its live requests still needs (at most
+ * max_chunk), finished requests are read back and their slots parked. Each result is
+ * bit-identical to the same solve alone with the same options. One host thread drives a
+ * server; mmp_serve_poll never blocks.
+ * ---------------------------------------------------------------------------------------- */
+typedef struct mmp_server mmp_server;
+
+typedef struct mmp_serve_options {
+  int device;
+  uint32_t slots;        /* fiber slots per batch group, >= 1 */
+  uint32_t max_chunk;    /* steps a group or solver takes between scheduling points, >= 1 */
+  uint32_t precision;    /* MMP_PRECISION_* for every solver */
+  uint32_t pair_decode;  /* MMP_PAIRS_* */
+  uint32_t flags;        /* MMP_FLAG_* for every solver (MMP_FLAG_AUTOTUNE tunes each structure once) */
+  double hbm_budget;     /* bytes of device memory the server's solvers may hold */
+} mmp_serve_options;
+
+enum { MMP_REQ_QUEUED = 0, MMP_REQ_RUNNING = 1, MMP_REQ_DONE = 2, MMP_REQ_NONFINITE = 3 };
+
+typedef struct mmp_serve_record {
+  uint32_t state;      /* MMP_REQ_* */
+  uint32_t batched;    /* 1: a slot of a batch group, 0: a solver of its own */
+  uint32_t group;      /* the group's index when batched */
+  uint32_t slot;       /* the slot when batched */
+  double t_submit;     /* seconds since mmp_serve_create */
+  double t_admit;
+  double t_finish;
+  uint64_t units;      /* n_modes * nt * n_steps */
+  size_t device_bytes; /* planned footprint of one fiber of this request */
+} mmp_serve_record;
+
+typedef struct mmp_serve_stats {
+  uint64_t submitted, queued, running, done, nonfinite;
+  uint32_t groups, solos;       /* live now */
+  uint32_t groups_created, solos_created;
+  size_t bytes_in_use, peak_bytes;
+} mmp_serve_stats;
+
+MMP_API mmp_status mmp_serve_create(const mmp_serve_options* options, mmp_server** out);
+MMP_API void mmp_serve_destroy(mmp_server* server);
+/* Copies the request (every array of params); it is queued at the current time. The request's
+ * n_steps argument is the number of steps to take (params->n_steps sets h = L / n_steps). */
+MMP_API mmp_status mmp_serve_submit(mmp_server* server, const mmp_params* params, uint32_t n_steps,
+                                    uint64_t* id_out);
+/* One scheduling pass: harvests groups and solvers whose chunk has finished, admits queued
+ * requests that fit, launches the next chunks. active_out (optional) = queued + running. */
+MMP_API mmp_status mmp_serve_poll(mmp_server* server, uint32_t* active_out);
+/* Polls until no request is queued or running. MMP_ERR_STATE if a queued request can never be
+ * admitted (its footprint exceeds the budget) while nothing runs. */
+MMP_API mmp_status mmp_serve_drain(mmp_server* server);
+MMP_API mmp_status mmp_serve_record_get(mmp_server* server, uint64_t id, mmp_serve_record* out);
+/* The physical field of a finished request ([M][NT] complex); MMP_ERR_STATE while it is queued
+ * or running, MMP_ERR_NONFINITE when its field is not finite. */
+MMP_API mmp_status mmp_serve_result(mmp_server* server, uint64_t id, double* fields_out);
+MMP_API mmp_status mmp_serve_stats_get(mmp_server* server, mmp_serve_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

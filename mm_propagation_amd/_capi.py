@@ -87,6 +87,35 @@ class MmpInfo(ctypes.Structure):
     ]
 
 
+MMP_REQ_QUEUED = 0
+MMP_REQ_RUNNING = 1
+MMP_REQ_DONE = 2
+MMP_REQ_NONFINITE = 3
+
+
+class MmpServeOptions(ctypes.Structure):
+    _fields_ = [
+        ("device", c_int), ("slots", c_uint32), ("max_chunk", c_uint32), ("precision", c_uint32),
+        ("pair_decode", c_uint32), ("flags", c_uint32), ("hbm_budget", c_double),
+    ]
+
+
+class MmpServeRecord(ctypes.Structure):
+    _fields_ = [
+        ("state", c_uint32), ("batched", c_uint32), ("group", c_uint32), ("slot", c_uint32),
+        ("t_submit", c_double), ("t_admit", c_double), ("t_finish", c_double),
+        ("units", c_uint64), ("device_bytes", c_size_t),
+    ]
+
+
+class MmpServeStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", c_uint64), ("queued", c_uint64), ("running", c_uint64), ("done", c_uint64),
+        ("nonfinite", c_uint64), ("groups", c_uint32), ("solos", c_uint32), ("groups_created", c_uint32),
+        ("solos_created", c_uint32), ("bytes_in_use", c_size_t), ("peak_bytes", c_size_t),
+    ]
+
+
 class MmpKernelTime(ctypes.Structure):
     _fields_ = [
         ("name", c_char * 48),
@@ -126,6 +155,14 @@ EXPORTS = [
     ("mmp_solver_info", c_int, [c_void_p, POINTER(MmpInfo)]),
     ("mmp_kernel_times", c_int, [c_void_p, POINTER(MmpKernelTime), c_uint32, POINTER(c_uint32)]),
     ("mmp_ctx_synchronize", c_int, [c_void_p]),
+    ("mmp_serve_create", c_int, [POINTER(MmpServeOptions), POINTER(c_void_p)]),
+    ("mmp_serve_destroy", None, [c_void_p]),
+    ("mmp_serve_submit", c_int, [c_void_p, POINTER(MmpParams), c_uint32, POINTER(c_uint64)]),
+    ("mmp_serve_poll", c_int, [c_void_p, POINTER(c_uint32)]),
+    ("mmp_serve_drain", c_int, [c_void_p]),
+    ("mmp_serve_record_get", c_int, [c_void_p, c_uint64, POINTER(MmpServeRecord)]),
+    ("mmp_serve_result", c_int, [c_void_p, c_uint64, POINTER(c_double)]),
+    ("mmp_serve_stats_get", c_int, [c_void_p, POINTER(MmpServeStats)]),
     ("mmp_plan", c_int, [POINTER(MmpParams), POINTER(MmpOptions), POINTER(MmpInfo)]),
     ("mmp_tableau", c_int, [c_uint32, POINTER(c_uint32), POINTER(c_double), POINTER(c_double), POINTER(c_double)]),
     ("mmp_plan_arrays", c_int, [POINTER(MmpParams), POINTER(MmpOptions), POINTER(c_double), POINTER(c_double),

@@ -22,7 +22,7 @@ BUILD = os.path.join(PKG, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["kernels.hip", "solver.cpp", "setup.cpp"]
+HIP_SOURCES = ["kernels.hip", "solver.cpp", "setup.cpp", "serving.cpp"]
 DROPIN_SOURCES = ["dropin/ComplexArray.cpp", "dropin/ComplexArraysContainer.cpp", "dropin/Sparse3DArray.cpp",
                   "dropin/RungeKutta.cpp", "dropin/MultimodePropagation.cpp", "dropin/MultimodePropagation_GPU.cpp",
                   "dropin/FieldOutput.cpp"]

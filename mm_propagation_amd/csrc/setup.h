@@ -82,6 +82,9 @@ bool split_length(uint32_t nt, uint32_t& n1, uint32_t& n2);
 // column-stage kernels: fills P.fused_*; false (and no chunks) when those kernels do not apply.
 bool plan_colstage_chunks(Problem& P, int max_lines);
 
+// Sets the calling thread's mmp_last_error() text and returns st.
+mmp_status record_error(mmp_status st, const std::string& msg);
+
 // exp(-2*pi*i*j/n), j < n, evaluated in long double and rounded once.
 std::vector<double> twiddle_table(uint32_t n);
 

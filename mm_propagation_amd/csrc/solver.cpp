@@ -35,6 +35,13 @@ mmp_status hip_fail(hipError_t e, const char* what) {
   return MMP_ERR_DEVICE;
 }
 
+}  // namespace
+
+// Errors raised by other translation units of the library (the request server).
+mmp_status mmp::record_error(mmp_status st, const std::string& msg) { return fail(st, msg); }
+
+namespace {
+
 #define HIP_TRY(expr, what)                        \
   do {                                             \
     hipError_t e_ = (expr);                        \

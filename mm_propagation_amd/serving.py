@@ -1,35 +1,26 @@
-"""Continuous-batching serving loop for propagation solves (no reference counterpart).
+"""Continuous-batching serving of propagation solves: the Python face of the library's request
+server (mmp_serve_* in include/mmprop.h; no reference counterpart).
 
-Requests (a SimulationParameters and a step count, arriving over time) are admitted by device
-memory footprint into
-
-* batch slots: requests the batched kernels take (Raman on, 64-sample columns, M <= 16) are
-  grouped by structure (n_modes, nt, tableau, coupling pattern) into one batched solver per
-  group with a fixed number of slots; a new request is loaded into a free slot of an idle
-  group (mmp_fiber_load) and joins the group's next chunk of steps;
-* solo solvers: any other request gets a single-fiber solver of its own.
-
-Every group and solo solver owns a context (one HIP stream), so their chunks overlap on the
-device. A group advances by the smallest number of steps any of its active requests still
-needs (capped), so no request takes a step too many; finished requests are read back and
-their slots freed at chunk boundaries. Requests keep their fields and spectra on the device
-for their whole solve. A request's result equals the result of the same solve alone: the
-batched kernels do the same arithmetic per fiber and a solve split into chunks equals an
-unsplit one.
+Requests (a SimulationParameters and a step count, arriving over time) are submitted to the
+native server when they arrive; the server admits them by device footprint into batch slots
+(requests the batched kernels take, grouped by structure: n_modes, nt, tableau, coupling
+pattern) or solvers of their own, keeps every group and solver on its own stream, refills
+slots at chunk boundaries and parks empty ones. `ServingEngine.run` replays a list of
+requests at their arrival times and reports p50/p99 latency and aggregate M*NT*steps/s.
+A request's result equals the result of the same solve alone.
 """
 from __future__ import annotations
 
+import ctypes
 import math
 import time
-from dataclasses import dataclass, field
-from typing import Dict, List, Optional, Sequence, Tuple
+from dataclasses import dataclass
+from typing import Dict, List, Optional, Sequence
 
 import numpy as np
 
 from . import _capi, plan
-from .batch import MultimodePropagationBatch
-from .problem import SimulationParameters
-from .solver import Context, MultimodePropagationGPU_RamanOFF, MultimodePropagationGPU_RamanON
+from .problem import CParams, SimulationParameters
 
 
 @dataclass
@@ -54,236 +45,109 @@ class Request:
         return None if self.finished is None else self.finished - self.arrival
 
 
-def _structure_key(req: Request, info: Dict[str, float]) -> Tuple:
-    return (int(info["n_modes"]), int(info["nt"]), int(info["stages"]), int(info["n_pairs"]), int(info["nnz"]),
-            int(info["n_conv_lines"]), req.raman)
-
-
-class _Group:
-    """A batched solver with fixed slots for requests of one structure."""
-
-    def __init__(self, key, req: Request, slots: int, precision: int, pair_decode: int):
-        self.key = key
-        self.ctx = Context(0)
-        self.batch = MultimodePropagationBatch([req.params] * slots, raman=req.raman, pair_decode=pair_decode,
-                                               precision=precision, ctx=self.ctx)
-        self.slots: List[Optional[Request]] = [None] * slots
-        self.remaining = [0] * slots
-        self.busy = False
-        self.chunk = 0
-        self.bytes = int(self.batch.info()["device_bytes"])
-        for i in range(slots):
-            self.batch.set_live(i, False)  # empty slots cost nothing until a request is loaded
-
-    def free_slot(self) -> Optional[int]:
-        for i, r in enumerate(self.slots):
-            if r is None:
-                return i
-        return None
-
-    def active(self) -> List[int]:
-        return [i for i, r in enumerate(self.slots) if r is not None]
-
-    def close(self):
-        self.batch.close()
-        self.ctx.close()
-
-
-class _Solo:
-    """A single-fiber solver for a request the batched kernels do not take."""
-
-    def __init__(self, req: Request, precision: int, pair_decode: int):
-        self.ctx = Context(0)
-        cls = MultimodePropagationGPU_RamanON if req.raman else MultimodePropagationGPU_RamanOFF
-        self.solver = cls(req.params, pair_decode=pair_decode, precision=precision, ctx=self.ctx)
-        self.req = req
-        self.remaining = req.n_steps
-        self.started = False
-        self.busy = False
-        self.chunk = 0
-        self.bytes = int(self.solver.info()["device_bytes"])
-
-    def close(self):
-        self.solver.close()
-        self.ctx.close()
-
-
 class ServingEngine:
+    """One native request server (one device)."""
+
     def __init__(self, hbm_budget: float = 64e9, slots: int = 16, max_chunk: int = 256,
                  precision: int = _capi.MMP_PRECISION_MIXED, pair_decode: int = _capi.MMP_PAIRS_PHYSICAL,
-                 poll_s: float = 2e-4):
-        self.hbm_budget = hbm_budget
-        self.slots = slots
-        self.max_chunk = max_chunk
+                 flags: int = 0, poll_s: float = 2e-4, device: int = 0):
+        self.lib = _capi.load_library()
         self.precision = precision
         self.pair_decode = pair_decode
         self.poll_s = poll_s
-        self.groups: Dict[Tuple, _Group] = {}
-        self.solos: List[_Solo] = []
-        self.bytes_in_use = 0
-        self.peak_bytes = 0
+        opt = _capi.MmpServeOptions(device=device, slots=slots, max_chunk=max_chunk, precision=precision,
+                                    pair_decode=pair_decode, flags=flags, hbm_budget=float(hbm_budget))
+        h = ctypes.c_void_p()
+        _capi.check(self.lib.mmp_serve_create(ctypes.byref(opt), ctypes.byref(h)), "mmp_serve_create")
+        self.h = h
         self._plans: Dict[int, Dict[str, float]] = {}
 
-    # --- placement --------------------------------------------------------
     def plan_for(self, req: Request) -> Dict[str, float]:
-        if req.rid not in self._plans:  # host preparation once per request, not per admission attempt
+        """mmp_plan of the request (host only): footprint and structure."""
+        if req.rid not in self._plans:
             self._plans[req.rid] = plan.plan(req.params, raman=req.raman, pair_decode=self.pair_decode,
                                              precision=self.precision)
         return self._plans[req.rid]
 
-    def _batchable(self, req: Request, info) -> bool:
-        return req.raman and int(info["n1"]) == 64 and int(info["n_modes"]) <= 16
+    def submit(self, req: Request) -> int:
+        cp = CParams(req.params, req.raman)
+        rid = ctypes.c_uint64()
+        _capi.check(self.lib.mmp_serve_submit(self.h, ctypes.byref(cp.c), int(req.n_steps), ctypes.byref(rid)),
+                    "mmp_serve_submit")
+        return rid.value
 
-    def _admit(self, req: Request, now: float) -> bool:
-        info = self.plan_for(req)
-        if self._batchable(req, info):
-            key = _structure_key(req, info)
-            g = self.groups.get(key)
-            if g is None:
-                need = info["device_bytes"] * self.slots
-                if self.bytes_in_use + need > self.hbm_budget:
-                    return False
-                g = _Group(key, req, self.slots, self.precision, self.pair_decode)
-                self.groups[key] = g
-                self.bytes_in_use += g.bytes
-                self.peak_bytes = max(self.peak_bytes, self.bytes_in_use)
-            if g.busy:
-                return False  # slots are filled at chunk boundaries
-            i = g.free_slot()
-            if i is None:
-                return False
-            try:
-                g.batch.load(i, req.params)
-            except _capi.MmpError as e:
-                if e.status != _capi.MMP_ERR_INVALID_ARGUMENT:
-                    raise
-                return self._admit_solo(req, info, now)  # same sizes, different coupling pattern
-            g.slots[i] = req
-            g.remaining[i] = req.n_steps
-            req.admitted = now
-            req.placement = f"batch M={key[0]} NT={key[1]} slot {i}"
-            return True
-        return self._admit_solo(req, info, now)
+    def poll(self) -> int:
+        active = ctypes.c_uint32()
+        _capi.check(self.lib.mmp_serve_poll(self.h, ctypes.byref(active)), "mmp_serve_poll")
+        return active.value
 
-    def _admit_solo(self, req: Request, info, now: float) -> bool:
-        if self.bytes_in_use + info["device_bytes"] > self.hbm_budget:
-            return False
-        s = _Solo(req, self.precision, self.pair_decode)
-        self.solos.append(s)
-        self.bytes_in_use += s.bytes
-        self.peak_bytes = max(self.peak_bytes, self.bytes_in_use)
-        req.admitted = now
-        req.placement = f"solo M={int(info['n_modes'])} NT={int(info['nt'])}"
-        return True
+    def record(self, sid: int) -> _capi.MmpServeRecord:
+        rec = _capi.MmpServeRecord()
+        _capi.check(self.lib.mmp_serve_record_get(self.h, sid, ctypes.byref(rec)), "mmp_serve_record_get")
+        return rec
 
-    # --- stepping ---------------------------------------------------------
-    def _finish(self, req: Request, fields: Optional[np.ndarray], now: float):
-        req.result = fields
-        req.nonfinite = fields is None
-        req.finished = now
+    def result(self, sid: int, req: Request) -> Optional[np.ndarray]:
+        out = np.empty((int(req.params.n_modes), int(req.params.nt)), dtype=np.complex128)
+        st = self.lib.mmp_serve_result(self.h, sid, out.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        if st == _capi.MMP_ERR_NONFINITE:
+            return None
+        _capi.check(st, "mmp_serve_result")
+        return out
 
-    def _harvest_group(self, g: _Group, now: float):
-        for i in g.active():
-            g.remaining[i] -= g.chunk
-            if g.remaining[i] == 0:
-                req = g.slots[i]
-                try:
-                    fields = g.batch.result(i)
-                except _capi.MmpError as e:
-                    if e.status != _capi.MMP_ERR_NONFINITE:
-                        raise
-                    fields = None
-                self._finish(req, fields, now)
-                g.slots[i] = None
-                g.batch.set_live(i, False)
-        g.chunk = 0
+    def stats(self) -> Dict[str, int]:
+        st = _capi.MmpServeStats()
+        _capi.check(self.lib.mmp_serve_stats_get(self.h, ctypes.byref(st)), "mmp_serve_stats_get")
+        return {name: getattr(st, name) for name, _ in _capi.MmpServeStats._fields_}
 
-    def _launch_group(self, g: _Group):
-        act = g.active()
-        if not act:
-            return
-        g.chunk = min(min(g.remaining[i] for i in act), self.max_chunk)
-        g.batch.advance(g.chunk)
-        g.busy = True
-
-    def _step_solo(self, s: _Solo, now: float) -> bool:
-        """Returns True when the solo request finished (the solver is then released)."""
-        if s.busy:
-            s.remaining -= s.chunk
-            s.busy = False
-        if s.remaining == 0:
-            try:
-                fields = s.solver.getResult()
-            except _capi.MmpError as e:
-                if e.status != _capi.MMP_ERR_NONFINITE:
-                    raise
-                fields = None
-            self._finish(s.req, fields, now)
-            return True
-        s.chunk = min(s.remaining, self.max_chunk)
-        if not s.started:
-            s.solver.computeLawsonRK(s.chunk)  # the request's solve from z = 0
-            s.started = True
-        else:
-            s.solver.advance(s.chunk)
-        s.busy = True
-        return False
-
-    # --- the loop ---------------------------------------------------------
     def run(self, requests: Sequence[Request], clock=time.perf_counter) -> Dict[str, float]:
+        """Submits each request at its arrival time (seconds from now), polls the server until
+        every request has finished, and fills in each request's result, times and placement."""
         pending = sorted(requests, key=lambda r: r.arrival)
-        queue: List[Request] = []
-        done = 0
+        sids: List[int] = []
+        submit_wall: List[float] = []
         t0 = clock()
         k = 0
-        while done < len(pending):
+        while True:
             now = clock() - t0
             while k < len(pending) and pending[k].arrival <= now:
-                queue.append(pending[k])
+                sids.append(self.submit(pending[k]))
+                submit_wall.append(clock() - t0)
                 k += 1
-            # chunk boundaries: harvest idle groups and solos, then admit, then launch
-            for g in self.groups.values():
-                if g.busy and g.ctx.idle():
-                    g.busy = False
-                    before = len(g.active())
-                    self._harvest_group(g, clock() - t0)
-                    done += before - len(g.active())
-            for s in list(self.solos):
-                if not s.busy or s.ctx.idle():
-                    if self._step_solo(s, clock() - t0):
-                        done += 1
-                        self.solos.remove(s)
-                        self.bytes_in_use -= s.bytes
-                        s.close()
-            for req in list(queue):
-                if self._admit(req, clock() - t0):
-                    queue.remove(req)
-            for s in self.solos:
-                if not s.busy and s.remaining > 0:
-                    self._step_solo(s, clock() - t0)
-            for g in self.groups.values():
-                if not g.busy:
-                    self._launch_group(g)
-            busy = any(g.busy for g in self.groups.values()) or any(s.busy for s in self.solos)
-            if not busy and not queue and k < len(pending):
+            active = self.poll()
+            if k == len(pending) and active == 0:
+                break
+            if active == 0:
                 time.sleep(max(0.0, pending[k].arrival - (clock() - t0)))
             else:
                 time.sleep(self.poll_s)
         makespan = clock() - t0
+        for req, sid, tw in zip(pending, sids, submit_wall):
+            rec = self.record(sid)
+            # server times are relative to its own clock; the request arrived at req.arrival
+            req.admitted = tw + (rec.t_admit - rec.t_submit)
+            req.finished = tw + (rec.t_finish - rec.t_submit)
+            req.nonfinite = rec.state == _capi.MMP_REQ_NONFINITE
+            req.result = None if req.nonfinite else self.result(sid, req)
+            kind = f"batch group {rec.group} slot {rec.slot}" if rec.batched else "solo"
+            req.placement = f"{kind} M={int(req.params.n_modes)} NT={int(req.params.nt)}"
         lat = np.array([r.latency for r in pending], dtype=np.float64)
         units = sum(r.units for r in pending)
+        st = self.stats()
         return {"requests": len(pending), "makespan_s": makespan, "units_per_s": units / makespan,
                 "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
-                "nonfinite": sum(1 for r in pending if r.nonfinite), "groups": len(self.groups),
-                "peak_device_bytes": self.peak_bytes}
+                "nonfinite": sum(1 for r in pending if r.nonfinite), "groups": st["groups_created"],
+                "solos": st["solos_created"], "peak_device_bytes": st["peak_bytes"]}
 
     def close(self):
-        for g in self.groups.values():
-            g.close()
-        self.groups.clear()
-        for s in self.solos:
-            s.close()
-        self.solos.clear()
+        if self.h:
+            self.lib.mmp_serve_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def serving_requests(count: int, rate: float, seed: int = 0, modes=(1, 3, 6, 10, 15),
