@@ -168,7 +168,7 @@ int colstage_max_modes();
 bool colstage_wide_ok(int M, bool mixed);
 int colstage_lines_per_chunk();
 int colstage_slots_per_chunk();
-int colstage_coefficients_per_chunk();
+int colstage_coefficients_per_chunk(bool mixed);
 
 // Launchers (return hipSuccess or the launch error). Those templated on W run in the
 // work precision W (double, or float for the mixed-precision step); the others are FP64.

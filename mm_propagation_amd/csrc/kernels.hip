@@ -609,7 +609,10 @@ __global__ void __launch_bounds__(512) k_gather(Geometry g, const cpx<W>* __rest
 constexpr int kColStageL = 64;       // column length the column-stage kernels take
 constexpr int kColStageMaxM = 16;    // modes kept in registers by the gather
 constexpr int kColStageXCap = 3072;  // complex LDS values for Raman lines / partial sums
-constexpr int kColStageQCap = 2048;  // coupling coefficients of one chunk's slots held in LDS
+// coupling coefficients of one chunk's slots held in LDS (64 KB FP64 would crowd out the lines)
+template <class W>
+constexpr int kColStageQCapOf = std::is_same<W, float>::value ? 8192 : 4096;
+constexpr int kColStageQPre = 2048;  // of those, prefetched into registers before the chunk's barrier
 constexpr int kColStageSCap = 128;   // slots of one chunk
 constexpr int kColStageLines = 40;   // Raman lines of one chunk in the gather kernel
 constexpr int kColStageRS = LineSlots<kColStageL, 1>::value;  // LDS row stride of a line (72)
@@ -631,7 +634,7 @@ template <int M, class W>
 constexpr bool kColStageWide = std::is_same<W, float>::value ? M <= 12 : M <= 6;
 bool colstage_wide_ok(int M, bool mixed) { return mixed ? M <= 12 : M <= 6; }
 int colstage_lines_per_chunk() { return kColStageLines; }
-int colstage_coefficients_per_chunk() { return kColStageQCap; }
+int colstage_coefficients_per_chunk(bool mixed) { return mixed ? kColStageQCapOf<float> : kColStageQCapOf<double>; }
 int colstage_slots_per_chunk() { return kColStageSCap; }
 
 template <class W>
@@ -748,7 +751,7 @@ __global__ void __launch_bounds__(NW * 64) k_colstage_raman_gather(Geometry g, c
   static_assert(M >= 1 && M <= kColStageMaxM, "modes out of range");
   __shared__ cw Ush[UR * RS];   // field rows; sigma rows after the gather
   __shared__ cw Rsh[RCAP];      // Raman line rows of the chunk; partial sums after the gather
-  __shared__ W Qsh[kColStageQCap];         // [slots of the chunk][M][M] coupling blocks
+  __shared__ W Qsh[kColStageQCapOf<W>];    // [slots of the chunk][M][M] coupling blocks
   __shared__ uint32_t Msh[kColStageSCap];  // slot metadata of the chunk
   const int tid = threadIdx.x;
   const int job = tid / T, t = tid % T;
@@ -770,7 +773,7 @@ __global__ void __launch_bounds__(NW * 64) k_colstage_raman_gather(Geometry g, c
       uin[r] = e < M * L ? U[(size_t)(e / L) * g.N + (size_t)(e % L) * g.N2 + n2] : cw{0, 0};
     }
   }
-  constexpr int QIN = kColStageQCap / NT;
+  constexpr int QIN = kColStageQPre / NT;
   W qin[QIN];
   uint32_t min_ = 0;
 #pragma unroll
@@ -809,6 +812,7 @@ __global__ void __launch_bounds__(NW * 64) k_colstage_raman_gather(Geometry g, c
     }
 #pragma unroll
     for (int r = 0; r < QIN; r++) Qsh[tid + r * NT] = qin[r];
+    for (int e = QIN * NT + tid; e < (s1 - s0) * MM; e += NT) Qsh[e] = Qd[(size_t)s0 * MM + e];
     if (tid < s1 - s0) Msh[tid] = min_;
     for (int r0 = l0; r0 < l1; r0 += JOBS) {
       const int q = r0 + job;
@@ -1215,7 +1219,7 @@ hipError_t launch_colstage_raman_gather(const Geometry& g, const Twiddles& tw, c
   const int block = fa.parity ? g.M * ((g.M + 1) / 2) : g.M * g.M;
   if (fa.n_chunks < 1 || fa.max_chunk_lines < 1 || fa.max_chunk_lines > colstage_lines_per_chunk() ||
       fa.max_chunk_slots < 1 || fa.max_chunk_slots > colstage_slots_per_chunk() ||
-      fa.max_chunk_slots * block > colstage_coefficients_per_chunk())
+      fa.max_chunk_slots * block > colstage_coefficients_per_chunk(std::is_same<W, float>::value))
     return hipErrorInvalidValue;
   switch (g.M) {
     MMP_COLSTAGE_CASE(1) MMP_COLSTAGE_CASE(2) MMP_COLSTAGE_CASE(3) MMP_COLSTAGE_CASE(4) MMP_COLSTAGE_CASE(5)

@@ -141,6 +141,7 @@ mmp_status prepare_problem(const mmp_params* in, const mmp_options& opt, Problem
   }
   if (in->n_steps == 0) { err = "n_steps must be >= 1"; return MMP_ERR_INVALID_ARGUMENT; }
   if (in->raman && !in->raman_response) { err = "raman_response is required with Raman on"; return MMP_ERR_INVALID_ARGUMENT; }
+  P.mixed = opt.precision == MMP_PRECISION_MIXED;
   if (opt.precision != MMP_PRECISION_FP64 && opt.precision != MMP_PRECISION_MIXED) {
     err = "unknown precision";
     return MMP_ERR_UNSUPPORTED;
@@ -431,7 +432,7 @@ bool plan_colstage_chunks(Problem& P, int max_lines) {
   if (P.raman && P.N1 == 64 && (int)M <= colstage_max_modes() && nl > 0 &&
       P.line_slot0.size() == (size_t)nl + 1 && P.line_slot0[0] == 0 && P.line_slot0[nl] == P.P) {
     const int block = P.parity ? (int)(M * ((M + 1) / 2)) : (int)(M * M);  // coefficients per slot
-    const int max_slots = std::min(colstage_slots_per_chunk(), colstage_coefficients_per_chunk() / block);
+    const int max_slots = std::min(colstage_slots_per_chunk(), colstage_coefficients_per_chunk(P.mixed) / block);
     bool ok = max_slots >= 2;
     P.fused_line.push_back(0);
     P.fused_slot.push_back(0);

@@ -58,6 +58,7 @@ struct Problem {
   std::vector<uint8_t> slot_parity;  // [P]
   // column-stage kernels: chunks of Raman lines (0 chunks: not eligible), their line and slot
   // bounds, and the largest chunk's line and slot counts
+  bool mixed = false;  // FP32 work arrays (the column-stage LDS limits depend on it)
   int fused_chunks = 0, fused_max_lines = 0, fused_max_slots = 0;
   std::vector<uint32_t> fused_line;                 // [fused_chunks + 1]
   std::vector<uint32_t> fused_slot;                 // [fused_chunks + 1]

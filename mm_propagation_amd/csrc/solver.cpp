@@ -411,7 +411,7 @@ size_t planned_device_bytes(const Problem& P, const mmp_options& opt, size_t F) 
     if (P.raman) b += 8 * (size_t)P.N * F;
   }
   if (P.fused_chunks > 0 && !(opt.flags & MMP_FLAG_SEPARATE_COLUMNS))
-    b += 4 * (P.fused_line.size() + P.fused_slot.size() + c(P.P));
+    b += 4 * (2 * (P.line_a.size() + 1) + c(P.P));  // chunk bounds sized for the finest plan, slot metadata
   const bool fused = P.fused_chunks > 0 && !(opt.flags & MMP_FLAG_SEPARATE_COLUMNS);
   const size_t block = (fused && P.parity) ? (size_t)P.M * ((P.M + 1) / 2) : (size_t)P.M * P.M;
   if (fused || dense_gather_chunk((int)P.M) > 0) b += (mixed ? 4 : 8) * (size_t)P.P * block * F;
