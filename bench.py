@@ -259,7 +259,7 @@ def run_batched(args) -> int:
     for i, sp in enumerate(fibers):
         classes.setdefault((sp.n_modes, sp.nt), []).append(i)
     batches = []
-    bflags = _capi.MMP_FLAG_AUTOTUNE if args.autotune else 0
+    bflags = (_capi.MMP_FLAG_AUTOTUNE if args.autotune else 0) | (_capi.MMP_FLAG_NO_GRAPH if args.no_graph else 0)
     t_tune = time.perf_counter()
     for key in sorted(classes):
         ctx = Context(0)
