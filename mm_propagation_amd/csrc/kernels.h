@@ -166,9 +166,9 @@ struct FusedArgs {
 int colstage_max_modes();
 // Whether the 8-wavefront gather workgroup exists for M modes in this precision.
 bool colstage_wide_ok(int M, bool mixed);
-int colstage_lines_per_chunk();
+int colstage_lines_per_chunk(int M);
 int colstage_slots_per_chunk();
-int colstage_coefficients_per_chunk(bool mixed);
+int colstage_coefficients_per_chunk(int M, bool mixed);
 
 // Launchers (return hipSuccess or the launch error). Those templated on W run in the
 // work precision W (double, or float for the mixed-precision step); the others are FP64.

@@ -608,15 +608,24 @@ __global__ void __launch_bounds__(512) k_gather(Geometry g, const cpx<W>* __rest
 // ---------------------------------------------------------------------------
 constexpr int kColStageL = 64;       // column length the column-stage kernels take
 constexpr int kColStageMaxM = 16;    // modes kept in registers by the gather
-constexpr int kColStageXCap = 3072;  // complex LDS values for Raman lines / partial sums
-// coupling coefficients of one chunk's slots held in LDS (64 KB FP64 would crowd out the lines)
-template <class W>
-constexpr int kColStageQCapOf = std::is_same<W, float>::value ? 8192 : 4096;
-constexpr int kColStageQPre = 2048;  // of those, prefetched into registers before the chunk's barrier
+// LDS limits of one chunk, by mode count: a few modes have few lines and small coupling
+// blocks, and a small LDS footprint lets more workgroups share a CU (batched fibers)
+constexpr int colstage_lines_cap(int M) { return M <= 4 ? 16 : (M <= 6 ? 24 : 40); }  // Raman lines
+// coupling coefficients held in LDS (64 KB FP64 would crowd out the lines)
+constexpr int colstage_qcap(int M, bool mixed) { return M <= 4 ? 1024 : (M <= 6 ? 2048 : (mixed ? 8192 : 4096)); }
+template <int M, class W>
+constexpr int kColStageQCapOf = colstage_qcap(M, std::is_same<W, float>::value);
+// of those, prefetched into registers before the chunk's barrier (the rest go straight to LDS):
+// what a small tensor needs in all (up to 4 modes: 512, up to 6: 1024, parity-compact), 4096
+// where the registers allow it (FP32, 7 to 12 modes), else 2048
+template <int M, class W>
+constexpr int kColStageQPreOf =
+    M <= 4 ? 512 : (M <= 6 ? 1024 : ((std::is_same<W, float>::value && M <= 12) ? 4096 : 2048));
 constexpr int kColStageSCap = 128;   // slots of one chunk
-constexpr int kColStageLines = 40;   // Raman lines of one chunk in the gather kernel
 constexpr int kColStageRS = LineSlots<kColStageL, 1>::value;  // LDS row stride of a line (72)
-constexpr int kColStageRRows = 48;   // LDS rows for the lines of a chunk (lines rounded up to 8)
+// LDS rows for the lines of a chunk: every wavefront of a round transforms 8 line slots, so
+// the rows cover the chunk's lines rounded up to 8
+constexpr int colstage_rows(int M) { return (colstage_lines_cap(M) + 7) / 8 * 8; }
 
 struct ColStageCfg {
   static constexpr int L = kColStageL;
@@ -633,8 +642,8 @@ int colstage_max_modes() { return kColStageMaxM; }
 template <int M, class W>
 constexpr bool kColStageWide = std::is_same<W, float>::value ? M <= 12 : M <= 6;
 bool colstage_wide_ok(int M, bool mixed) { return mixed ? M <= 12 : M <= 6; }
-int colstage_lines_per_chunk() { return kColStageLines; }
-int colstage_coefficients_per_chunk(bool mixed) { return mixed ? kColStageQCapOf<float> : kColStageQCapOf<double>; }
+int colstage_lines_per_chunk(int M) { return colstage_lines_cap(M); }
+int colstage_coefficients_per_chunk(int M, bool mixed) { return colstage_qcap(M, mixed); }
 int colstage_slots_per_chunk() { return kColStageSCap; }
 
 template <class W>
@@ -747,11 +756,11 @@ __global__ void __launch_bounds__(NW * 64) k_colstage_raman_gather(Geometry g, c
   constexpr int UR = (M + 7) / 8 * 8;  // field rows, rounded up to whole wavefronts of lines
   constexpr int UIN = (M * L + NT - 1) / NT;
   // Raman line rows of a chunk, or the NW partial sums of M modes after the gather
-  constexpr int RCAP = kColStageRRows * RS > NW * M * L ? kColStageRRows * RS : NW * M * L;
+  constexpr int RCAP = colstage_rows(M) * RS > NW * M * L ? colstage_rows(M) * RS : NW * M * L;
   static_assert(M >= 1 && M <= kColStageMaxM, "modes out of range");
   __shared__ cw Ush[UR * RS];   // field rows; sigma rows after the gather
   __shared__ cw Rsh[RCAP];      // Raman line rows of the chunk; partial sums after the gather
-  __shared__ W Qsh[kColStageQCapOf<W>];    // [slots of the chunk][M][M] coupling blocks
+  __shared__ W Qsh[kColStageQCapOf<M, W>];  // [slots of the chunk][M][M] coupling blocks
   __shared__ uint32_t Msh[kColStageSCap];  // slot metadata of the chunk
   const int tid = threadIdx.x;
   const int job = tid / T, t = tid % T;
@@ -773,7 +782,7 @@ __global__ void __launch_bounds__(NW * 64) k_colstage_raman_gather(Geometry g, c
       uin[r] = e < M * L ? U[(size_t)(e / L) * g.N + (size_t)(e % L) * g.N2 + n2] : cw{0, 0};
     }
   }
-  constexpr int QIN = kColStageQPre / NT;
+  constexpr int QIN = kColStageQPreOf<M, W> / NT;
   W qin[QIN];
   uint32_t min_ = 0;
 #pragma unroll
@@ -1217,9 +1226,9 @@ hipError_t launch_colstage_raman_gather(const Geometry& g, const Twiddles& tw, c
                                         double scale, const Batch& b, int F, hipStream_t s) {
   if (g.N1 != kColStageL || F < 1 || F > 65535) return hipErrorInvalidValue;
   const int block = fa.parity ? g.M * ((g.M + 1) / 2) : g.M * g.M;
-  if (fa.n_chunks < 1 || fa.max_chunk_lines < 1 || fa.max_chunk_lines > colstage_lines_per_chunk() ||
+  if (fa.n_chunks < 1 || fa.max_chunk_lines < 1 || fa.max_chunk_lines > colstage_lines_per_chunk(g.M) ||
       fa.max_chunk_slots < 1 || fa.max_chunk_slots > colstage_slots_per_chunk() ||
-      fa.max_chunk_slots * block > colstage_coefficients_per_chunk(std::is_same<W, float>::value))
+      fa.max_chunk_slots * block > colstage_coefficients_per_chunk(g.M, std::is_same<W, float>::value))
     return hipErrorInvalidValue;
   switch (g.M) {
     MMP_COLSTAGE_CASE(1) MMP_COLSTAGE_CASE(2) MMP_COLSTAGE_CASE(3) MMP_COLSTAGE_CASE(4) MMP_COLSTAGE_CASE(5)

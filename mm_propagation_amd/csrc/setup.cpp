@@ -408,7 +408,7 @@ mmp_status prepare_problem(const mmp_params* in, const mmp_options& opt, Problem
   }
   P.kv = P.raman ? 1 - P.fR : 1.;
   P.kr = P.raman ? P.fR * P.T0 : 0.;
-  plan_colstage_chunks(P, colstage_lines_per_chunk());
+  plan_colstage_chunks(P, colstage_lines_per_chunk((int)M));
   // a stage result N_j needs a stash slot when a stage i > j + 1 reads it
   P.stash_needed.assign(P.rk.s, 0);
   for (int i = 0; i < P.rk.s; i++)
@@ -428,11 +428,12 @@ bool plan_colstage_chunks(Problem& P, int max_lines) {
   P.fused_line.clear();
   P.fused_slot.clear();
   const int nl = (int)P.line_a.size();
-  if (max_lines < 1 || max_lines > colstage_lines_per_chunk()) return false;
+  if (max_lines < 1 || max_lines > colstage_lines_per_chunk((int)M)) return false;
   if (P.raman && P.N1 == 64 && (int)M <= colstage_max_modes() && nl > 0 &&
       P.line_slot0.size() == (size_t)nl + 1 && P.line_slot0[0] == 0 && P.line_slot0[nl] == P.P) {
     const int block = P.parity ? (int)(M * ((M + 1) / 2)) : (int)(M * M);  // coefficients per slot
-    const int max_slots = std::min(colstage_slots_per_chunk(), colstage_coefficients_per_chunk(P.mixed) / block);
+    const int max_slots =
+        std::min(colstage_slots_per_chunk(), colstage_coefficients_per_chunk((int)M, P.mixed) / block);
     bool ok = max_slots >= 2;
     P.fused_line.push_back(0);
     P.fused_slot.push_back(0);

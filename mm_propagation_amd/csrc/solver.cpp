@@ -443,7 +443,7 @@ void fill_info(const Problem& P, const mmp_options& opt, mmp_info* o) {
   o->live = 1;
   const bool fused = P.fused_chunks > 0 && !(opt.flags & MMP_FLAG_SEPARATE_COLUMNS);
   o->gather_waves = fused ? (P.M <= 6 ? 8u : 4u) : 0u;  // the defaults, before any tuning
-  o->chunk_lines = fused ? (uint32_t)colstage_lines_per_chunk() : 0u;
+  o->chunk_lines = fused ? (uint32_t)colstage_lines_per_chunk((int)P.M) : 0u;
 }
 
 // Fibers of one batch must share everything the kernels take from fiber 0's problem.
@@ -730,7 +730,7 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
     A(upload(s, d_meta, slot_meta.data(), P.P * 4));
     s->fused = FusedArgs{d_cl,   d_cs, d_meta, P.fused_chunks, P.fused_max_lines, P.fused_max_slots, compact ? 1 : 0,
                          P.M <= 6 ? 8 : 4};
-    s->chunk_lines = colstage_lines_per_chunk();
+    s->chunk_lines = colstage_lines_per_chunk((int)P.M);
   }
   // stage heads: tableau coefficients here, each fiber's h, sgn and tables in the kernel
   // the close head also builds the next step's stage 0 unless the stage-0 Lawson table is
@@ -854,7 +854,8 @@ mmp_status autotune(mmp_solver* s) {
   for (int waves : {4, 8}) {
     if (waves == 8 && !colstage_wide_ok((int)P.M, s->mixed)) continue;
     seen.clear();
-    for (int lines : {colstage_lines_per_chunk(), 24, 16}) {
+    for (int lines : {colstage_lines_per_chunk((int)P.M), 24, 16}) {
+      if (lines > colstage_lines_per_chunk((int)P.M)) continue;
       Problem probe = P;
       if (!plan_colstage_chunks(probe, lines)) continue;
       if (std::find(seen.begin(), seen.end(), probe.fused_line) != seen.end()) continue;  // same plan
