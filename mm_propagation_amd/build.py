@@ -25,7 +25,7 @@ ARCH = "gfx950"
 HIP_SOURCES = ["kernels.hip", "solver.cpp", "setup.cpp", "serving.cpp"]
 DROPIN_SOURCES = ["dropin/ComplexArray.cpp", "dropin/ComplexArraysContainer.cpp", "dropin/Sparse3DArray.cpp",
                   "dropin/RungeKutta.cpp", "dropin/MultimodePropagation.cpp", "dropin/MultimodePropagation_GPU.cpp",
-                  "dropin/FieldOutput.cpp"]
+                  "dropin/FieldOutput.cpp", "dropin/MultimodePropagationBatch.cpp"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result", "-fvisibility=hidden",
@@ -76,7 +76,7 @@ def build_dropin_library() -> str:
     return out
 
 
-CPP_TESTS = ["dropin_propagate", "dropin_selftest"]
+CPP_TESTS = ["dropin_propagate", "dropin_selftest", "dropin_batch"]
 
 
 def build_cpp_tests() -> list:

@@ -49,3 +49,11 @@ def test_dropin_program_matches_oracle(tmp_path, raman):
     ref = orc.get_result()
     err = configs.rel_l2_per_mode(got, ref)
     assert np.all(err <= 1e-10), err
+
+
+@pytest.mark.gpu
+def test_dropin_batch_and_server_equal_the_solves_alone():
+    """tests/cpp/dropin_batch.cpp: MultimodePropagationBatchGPU and PropagationServer (three
+    2-mode fibers, a server with two slots) against MultimodePropagationGPU_RamanON, bit for bit."""
+    r = subprocess.run([_binary("dropin_batch")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
