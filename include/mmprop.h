@@ -188,7 +188,7 @@ MMP_API void mmp_solver_destroy(mmp_solver* solver);
  * n_fibers independent fibers, params[f] each, that share n_modes, nt, method_order, raman and
  * the pair-slot structure of their coupling lists (their values, dispersion, lengths, pulses
  * and Raman responses may all differ). Every stage pass runs all fibers in one launch. Needs
- * the column-stage kernels (Raman on, nt in {4096, 8192, 16384}, n_modes <= 16). The calls
+ * the column-stage kernels (Raman on, nt in [4096, 65536], n_modes <= 16). The calls
  * above act on every fiber (mmp_compute_lawson_rk, mmp_advance, mmp_propagate_snapshots's
  * stepping) or on fiber 0 (results, state, energy, info); the mmp_fiber_* calls address one
  * fiber. */

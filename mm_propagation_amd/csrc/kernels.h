@@ -148,7 +148,7 @@ struct PairLineArgs {
   const uint32_t* line_self;  // [n_lines] 1 = packed self pairs, 0 = cross pair a, b
 };
 
-// Column-stage kernels (one workgroup per column, Raman on, N1 = 64, M <= 12): Raman lines
+// Column-stage kernels (one workgroup per column, Raman on, N1 = 64, M <= 16): Raman lines
 // are processed in chunks; chunk c covers lines [chunk_line[c], chunk_line[c+1]) and their
 // slots [chunk_slot[c], chunk_slot[c+1]) (slots are numbered by line).
 struct FusedArgs {

@@ -596,7 +596,7 @@ __global__ void __launch_bounds__(512) k_gather(Geometry g, const cpx<W>* __rest
 }
 
 // ---------------------------------------------------------------------------
-// Column-stage kernels (Raman on, N1 = 64, M <= 12). One workgroup of 256 lanes
+// Column-stage kernels (Raman on, N1 = 64, M <= 16). One workgroup of 256 (or 512) lanes
 // owns one column n2 and does every column-side step of a stage there, with the
 // column of each field and Raman line held in LDS between the steps:
 //   C1  inverse column DFT of the M fields -> U (HBM and LDS) -> pair products ->

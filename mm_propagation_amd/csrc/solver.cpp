@@ -571,7 +571,7 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
   if (F > 1 && !s->use_fused) {
     delete s;
     return fail(MMP_ERR_UNSUPPORTED,
-                "batched fibers need the column-stage kernels: Raman on, nt in {4096, 8192, 16384}, n_modes <= 12, "
+                "batched fibers need the column-stage kernels: Raman on, nt in [4096, 65536], n_modes <= 16, "
                 "and no MMP_FLAG_SEPARATE_COLUMNS");
   }
   auto cleanup = [&](mmp_status st2) {
