@@ -847,51 +847,66 @@ __global__ void __launch_bounds__(NW * 64) k_colstage_raman_gather(Geometry g, c
       for (int k = 0; k < M; k++) u[k] = Ush[k * RS + lane];
     }
     // slot s mixes kv U_a conj(U_b) with its Raman line read as is, conjugated, or by real or
-    // imaginary part; everything but the sample is wave-uniform and read from LDS by broadcast
-    for (int sl = s0 + wave; sl < s1; sl += NW) {
-      const uint32_t meta = Msh[sl - s0];
-      const int pa = meta & 255u, pb = (meta >> 8) & 255u, kind = (meta >> 16) & 3u, line = (int)(meta >> 19);
-      const W* qb = Qsh + (sl - s0) * MM;
-      const cw r = Rsh[(line - l0) * RS + lane];
-      const cw V = cmulc(Ush[pa * RS + lane], Ush[pb * RS + lane]);
-      cw vr;
-      switch (kind) {
-        case kDirect: vr = r; break;
-        case kConj: vr = cconj(r); break;
-        case kRe: vr = {r.re, 0}; break;
-        default: vr = {r.im, 0}; break;
-      }
-      const cw X = cadd(cscale(V, kv), cscale(vr, kr));
-      cw y[M];
+    // imaginary part; everything but the sample is wave-uniform and read from LDS by broadcast.
+    // With parity-structured blocks each wavefront passes over its slots once per parity, so
+    // the parity is a compile-time constant of the loop body (no per-slot selects).
+    auto slot_pass = [&](auto par_c) {
+      constexpr int par = decltype(par_c)::value;
+      // each slot's metadata is read from LDS one slot ahead of its use
+      uint32_t meta_next = s0 + wave < s1 ? Msh[wave] : 0u;
+      for (int sl = s0 + wave; sl < s1; sl += NW) {
+        const uint32_t meta = __builtin_amdgcn_readfirstlane(meta_next);  // wave-uniform
+        meta_next = sl + NW < s1 ? Msh[sl + NW - s0] : 0u;
+        if (PAR && (int)((meta >> 18) & 1u) != par) continue;
+        const int pa = meta & 255u, pb = (meta >> 8) & 255u, kind = (meta >> 16) & 3u, line = (int)(meta >> 19);
+        const W* qb = Qsh + (sl - s0) * MM;
+        const cw r = Rsh[(line - l0) * RS + lane];
+        const cw V = cmulc(Ush[pa * RS + lane], Ush[pb * RS + lane]);
+        cw vr;
+        switch (kind) {
+          case kDirect: vr = r; break;
+          case kConj: vr = cconj(r); break;
+          case kRe: vr = {r.re, 0}; break;
+          default: vr = {r.im, 0}; break;
+        }
+        const cw X = cadd(cscale(V, kv), cscale(vr, kr));
+        cw y[M];
 #pragma unroll
-      for (int k = 0; k < M; k++) y[k] = cmul(X, u[k]);
-      if constexpr (PAR) {
-        // only k with (p + k) % 2 == the slot's parity: q[p][kk] belongs to k = 2 kk + ((p + par) % 2)
-        auto checker = [&](auto par_c) {
-          constexpr int par = decltype(par_c)::value;
+        for (int k = 0; k < M; k++) y[k] = cmul(X, u[k]);
+        // the block's rows in groups of about 224 bytes: each group's LDS reads are issued
+        // together (the scheduling barrier keeps them ahead of their uses), so one LDS latency
+        // is exposed per group rather than one per pair of coefficients
+        constexpr int QROW = PAR ? KH : M;  // coefficients per row p
+        constexpr int PG = (int)(224 / sizeof(W)) / QROW > 0 ? (int)(224 / sizeof(W)) / QROW : 1;
 #pragma unroll
-          for (int p = 0; p < M; p++) {
+        for (int p0 = 0; p0 < M; p0 += PG) {
+          W qv[PG * QROW];
+#pragma unroll
+          for (int e = 0; e < PG * QROW; e++) qv[e] = p0 * QROW + e < M * QROW ? qb[p0 * QROW + e] : (W)0;
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int pp = 0; pp < PG; pp++) {
+            const int p = p0 + pp;
+            if (p >= M) break;
             cw tsum = acc[p];
+            if constexpr (PAR) {
+              // only k with (p + k) % 2 == the slot's parity: q[p][kk] belongs to k = 2 kk + ((p + par) % 2)
 #pragma unroll
-            for (int kk = 0; kk < KH; kk++) {
-              const int k = 2 * kk + ((p + par) & 1);
-              if (k < M) tsum = cadd(tsum, cscale(y[k], qb[p * KH + kk]));
+              for (int kk = 0; kk < KH; kk++) {
+                const int k = 2 * kk + ((p + par) & 1);
+                if (k < M) tsum = cadd(tsum, cscale(y[k], qv[pp * QROW + kk]));
+              }
+            } else {
+#pragma unroll
+              for (int k = 0; k < M; k++) tsum = cadd(tsum, cscale(y[k], qv[pp * QROW + k]));
             }
             acc[p] = tsum;
           }
-        };
-        if ((meta >> 18) & 1u) checker(std::integral_constant<int, 1>{});
-        else checker(std::integral_constant<int, 0>{});
-      } else {
-#pragma unroll
-        for (int p = 0; p < M; p++) {
-          cw tsum = acc[p];
-#pragma unroll
-          for (int k = 0; k < M; k++) tsum = cadd(tsum, cscale(y[k], qb[p * M + k]));
-          acc[p] = tsum;
         }
       }
-    }
+    };
+    slot_pass(std::integral_constant<int, 0>{});
+    if constexpr (PAR) slot_pass(std::integral_constant<int, 1>{});
   }
   __syncthreads();  // every lane done with Ush and Rsh
 #pragma unroll
