@@ -873,11 +873,13 @@ __global__ void __launch_bounds__(NW * 64) k_colstage_raman_gather(Geometry g, c
         cw y[M];
 #pragma unroll
         for (int k = 0; k < M; k++) y[k] = cmul(X, u[k]);
-        // the block's rows in groups of about 224 bytes: each group's LDS reads are issued
-        // together (the scheduling barrier keeps them ahead of their uses), so one LDS latency
-        // is exposed per group rather than one per pair of coefficients
+        // the block's rows in groups of about 224 bytes (128 for FP32 beyond 10 modes, whose
+        // registers are fuller): each group's LDS reads are issued together (the scheduling
+        // barrier keeps them ahead of their uses), so one LDS latency is exposed per group
+        // rather than one per pair of coefficients
         constexpr int QROW = PAR ? KH : M;  // coefficients per row p
-        constexpr int PG = (int)(224 / sizeof(W)) / QROW > 0 ? (int)(224 / sizeof(W)) / QROW : 1;
+        constexpr int QGB = (std::is_same<W, float>::value && M > 10) ? 128 : 224;
+        constexpr int PG = (int)(QGB / sizeof(W)) / QROW > 0 ? (int)(QGB / sizeof(W)) / QROW : 1;
 #pragma unroll
         for (int p0 = 0; p0 < M; p0 += PG) {
           W qv[PG * QROW];
