@@ -667,12 +667,19 @@ __global__ void __launch_bounds__(256) k_colstage_fields_pairs(Geometry g, const
   const int n2 = xcd_block(blockIdx.x, gridDim.x);
   cw* fsm = sh + job * F::SLOTS;
   cw w4[8], v[8];
-  // the first round's field samples are requested before the twiddles
+  // the first round's field samples are requested before the twiddles, and the first round's
+  // line descriptors with them (each later round's during the round before)
   {
     const bool valid = job < g.M;
     const size_t base = (size_t)(valid ? job : 0) * g.N + n2;
 #pragma unroll
     for (int i = 0; i < 8; i++) v[i] = valid ? yin[base + (size_t)(t + i * F::T) * g.N2] : cw{0, 0};
+  }
+  uint32_t la_n = 0, lb_n = 0, self_n = 0;
+  if (job < n_lines) {
+    la_n = pl.line_a[job];
+    lb_n = pl.line_b[job];
+    self_n = pl.line_self[job];
   }
   LineTw<L, W> lt;
   load_line_tw<L>(lt, coltw, t);
@@ -706,10 +713,14 @@ __global__ void __launch_bounds__(256) k_colstage_fields_pairs(Geometry g, const
   for (int r0 = 0; r0 < n_lines; r0 += F::NL) {
     const int q = r0 + job;
     const bool valid = q < n_lines;
-    const int qq = valid ? q : 0;
-    if (r0 + wave_job0 >= n_lines) continue;  // no line of this wavefront in the round
+    if (r0 + wave_job0 >= n_lines) continue;  // no line of this wavefront in this round or later
     // cross line: U_a conj(U_b); self line: |U_a|^2 + i |U_b|^2 (b absent: 0)
-    const uint32_t la = pl.line_a[qq], lb = pl.line_b[qq], self = pl.line_self[qq];
+    const uint32_t la = la_n, lb = lb_n, self = self_n;  // lanes past the last line read line 0's
+    if (q + F::NL < n_lines) {
+      la_n = pl.line_a[q + F::NL];
+      lb_n = pl.line_b[q + F::NL];
+      self_n = pl.line_self[q + F::NL];
+    }
     const bool has_b = lb != 0xffffffffu;
     const int ia = (int)la * L, ib = (int)(has_b ? lb : la) * L;
     const W bmask = has_b ? (W)1 : (W)0;
