@@ -57,3 +57,32 @@ def test_admission_waits_for_memory():
     eng.close()
     assert stats["requests"] == 2 and all(r.result is not None for r in reqs)
     assert stats["peak_device_bytes"] <= 1.5 * one
+
+
+def test_idle_groups_are_reclaimed_for_other_structures():
+    """A budget that holds one 2-slot batch group: requests of a second structure wait until
+    the first group has no live request and is closed to make room; every result still equals
+    its solve alone."""
+    require_device()
+    a = [Request(rid=i, params=configs.grin_fiber(M=3, nt=4096, n_steps=20000, fiber_length=0.3 + 0.1 * i,
+                                                   seed=70 + i, symmetric=True), n_steps=24 + 8 * i)
+         for i in range(2)]
+    b = [Request(rid=2 + i, params=configs.grin_fiber(M=6, nt=4096, n_steps=20000, fiber_length=0.3 + 0.1 * i,
+                                                       seed=80 + i, symmetric=True), n_steps=16 + 8 * i)
+         for i in range(2)]
+    from mm_propagation_amd import plan
+    one_group = max(plan.plan(r.params, precision=_capi.MMP_PRECISION_FP64, pair_decode=1)["device_bytes"]
+                    for r in a + b) * 2
+    note("case serving reclaim")
+    eng = ServingEngine(hbm_budget=1.2 * one_group, slots=2, max_chunk=8, precision=_capi.MMP_PRECISION_FP64)
+    stats = eng.run(a + b)
+    st = eng.stats()
+    eng.close()
+    assert stats["requests"] == 4 and stats["nonfinite"] == 0
+    assert st["groups_created"] == 2 and st["groups"] == 1  # the first group was closed for the second
+    assert stats["peak_device_bytes"] <= 1.2 * one_group
+    for r in a + b:
+        s = MultimodePropagationGPU_RamanON(r.params, pair_decode=_capi.MMP_PAIRS_PHYSICAL)
+        s.computeLawsonRK(r.n_steps)
+        assert np.array_equal(r.result, s.getResult()), r.rid
+        s.close()
