@@ -259,7 +259,14 @@ mmp_status admit(mmp_server* s, Request& r, bool& admitted) {
     if (st == MMP_ERR_OUT_OF_MEMORY) return MMP_OK;
     return st;
   }
-  for (uint32_t i = 1; i < F; i++) SRV_TRY(mmp_fiber_set_live(g->batch, i, 0));
+  for (uint32_t i = 1; i < F; i++) {
+    const mmp_status ps = mmp_fiber_set_live(g->batch, i, 0);
+    if (ps != MMP_OK) {  // the group is not registered yet: release it here
+      mmp_solver_destroy(g->batch);
+      mmp_ctx_destroy(g->ctx);
+      return ps;
+    }
+  }
   mmp_info info{};
   mmp_solver_info(g->batch, &info);
   g->bytes = info.device_bytes;
