@@ -41,7 +41,8 @@ def test_abi_version_and_status_strings(capi_lib):
 def test_struct_layouts_match_the_c_compiler(tmp_path):
     """Every ctypes structure has the size and field offsets gcc gives the C declaration."""
     structs = {"mmp_params": _capi.MmpParams, "mmp_options": _capi.MmpOptions, "mmp_info": _capi.MmpInfo,
-               "mmp_kernel_time": _capi.MmpKernelTime}
+               "mmp_kernel_time": _capi.MmpKernelTime, "mmp_serve_options": _capi.MmpServeOptions,
+               "mmp_serve_record": _capi.MmpServeRecord, "mmp_serve_stats": _capi.MmpServeStats}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mmprop.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
