@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MMPROP_ABI_VERSION 3
+#define MMPROP_ABI_VERSION 4
 
 /* Only the entry points below are exported; the library is built with hidden visibility. */
 #if defined(__GNUC__) || defined(__clang__)
@@ -131,6 +131,8 @@ typedef struct mmp_info {
   uint32_t gather_waves;     /* column-stage gather: wavefronts per workgroup (0: separate passes) */
   uint32_t chunk_lines;      /* column-stage gather: Raman lines per LDS chunk at most */
   uint32_t tuned;            /* 1 when MMP_FLAG_AUTOTUNE chose the two above */
+  uint32_t pair_columns;     /* column-stage pair pass: neighbouring columns per workgroup (0: separate passes) */
+  uint32_t pair_groups;      /* column-stage pair pass: workgroups sharing one column block's Raman lines */
   uint32_t reserved;
 } mmp_info;
 

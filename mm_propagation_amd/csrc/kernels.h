@@ -169,6 +169,11 @@ bool colstage_wide_ok(int M, bool mixed);
 int colstage_lines_per_chunk(int M);
 int colstage_slots_per_chunk();
 int colstage_coefficients_per_chunk(int M, bool mixed);
+constexpr size_t kColStagePairsLds = 65536;
+size_t colstage_pairs_lds_bytes(int M, int columns, bool mixed);
+// C1 layout for F fibers: four columns per workgroup where the LDS allows (else two, else
+// one), then line groups (powers of two, at least 4 lines each) until there are 512 workgroups.
+void colstage_pairs_plan(int M, int N2, int n_lines, int F, bool mixed, int& columns, int& groups);
 
 // Launchers (return hipSuccess or the launch error). Those templated on W run in the
 // work precision W (double, or float for the mixed-precision step); the others are FP64.
@@ -197,10 +202,13 @@ hipError_t launch_row_raman(const Geometry& g, const Twiddles& tw, cpx<W>* data,
 template <class W>
 hipError_t launch_gather(const Geometry& g, const cpx<W>* U, const cpx<W>* VR, cpx<W>* sigma, const GatherArgs& a,
                          hipStream_t s);
+// C1 takes `columns` (1, 2 or 4) neighbouring columns per workgroup and deals each column
+// block's Raman lines to `groups` workgroups; the LDS it then needs is
+// colstage_pairs_lds_bytes, at most kColStagePairsLds. Results do not depend on either.
 template <class W>
 hipError_t launch_colstage_fields_pairs(const Geometry& g, const Twiddles& tw, const cpx<W>* yin, cpx<W>* U,
                                         cpx<W>* conv, int n_lines, const PairLineArgs& pl, double scale,
-                                        const Batch& b, int F, hipStream_t s);
+                                        const Batch& b, int F, int columns, int groups, hipStream_t s);
 template <class W>
 hipError_t launch_colstage_raman_gather(const Geometry& g, const Twiddles& tw, const cpx<W>* U, const cpx<W>* conv,
                                         cpx<W>* yout, const GatherArgs& ga, const FusedArgs& fa, const W* Qd,

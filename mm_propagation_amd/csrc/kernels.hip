@@ -646,52 +646,67 @@ int colstage_lines_per_chunk(int M) { return colstage_lines_cap(M); }
 int colstage_coefficients_per_chunk(int M, bool mixed) { return colstage_qcap(M, mixed); }
 int colstage_slots_per_chunk() { return kColStageSCap; }
 
-template <class W>
+// C1 over CW neighbouring columns per workgroup, the column block's Raman lines dealt to
+// `groups` workgroups (each transforms every field of the block, group 0 stores them to U).
+// Lanes are ordered column-fastest, tid = (row_job * 8 + t) * CW + c, so every global access
+// of a wavefront reads or writes runs of CW consecutive samples of one row: with one column
+// per workgroup each lane's sample sits in a cache line of its own and the pass is bound by
+// the L2 request rate, not by HBM. The arithmetic of every line is the same for any CW and
+// groups, so the results are too.
+template <class W, int CW>
 __global__ void __launch_bounds__(256) k_colstage_fields_pairs(Geometry g, const cpx<W>* __restrict__ coltw,
                                                                const cpx<W>* __restrict__ fourtw,
                                                                const cpx<W>* __restrict__ yin, cpx<W>* __restrict__ U,
                                                                cpx<W>* __restrict__ conv, int n_lines,
-                                                               PairLineArgs pl, double scale, Batch b) {
+                                                               PairLineArgs pl, double scale, Batch b, int groups) {
   using F = ColStageCfg;
   using cw = cpx<W>;
   constexpr int L = F::L;
-  __shared__ cw sh[F::FFT + kColStageMaxM * L];
+  constexpr int JL = F::NL / CW;  // field or line rows per round: 32 jobs = JL rows x CW columns
+  static_assert(CW == 1 || CW == 2 || CW == 4, "columns per workgroup");
+  extern __shared__ __align__(16) unsigned char c1_lds[];
+  cw* sh = reinterpret_cast<cw*>(c1_lds);  // [32 jobs][SLOTS] transform rows, then [M][L][CW] fields
   const int fib = blockIdx.y;
   if (!b.fp[fib].live) return;  // uniform over the workgroup
   yin += fib * b.sf;
   U += fib * b.sf;
   conv += fib * b.sl;
-  cw* Ush = sh + F::FFT;  // [M][L]
+  cw* Ush = sh + F::FFT;
   const int tid = threadIdx.x;
-  const int job = tid / F::T, t = tid % F::T;
-  const int n2 = xcd_block(blockIdx.x, gridDim.x);
-  cw* fsm = sh + job * F::SLOTS;
+  const int c = tid % CW, t = (tid / CW) % F::T, jl = tid / (F::T * CW);
+  const int blk = xcd_block(blockIdx.x, gridDim.x);  // the groups of a column block are neighbours
+  const int grp = blk % groups;
+  const int n2 = (blk / groups) * CW + c;
+  const int lpg = (n_lines + groups - 1) / groups;
+  const int lbeg = grp * lpg < n_lines ? grp * lpg : n_lines;
+  const int lend = lbeg + lpg < n_lines ? lbeg + lpg : n_lines;
+  cw* fsm = sh + (jl * CW + c) * F::SLOTS;
   cw w4[8], v[8];
   // the first round's field samples are requested before the twiddles, and the first round's
   // line descriptors with them (each later round's during the round before)
   {
-    const bool valid = job < g.M;
-    const size_t base = (size_t)(valid ? job : 0) * g.N + n2;
+    const bool valid = jl < g.M;
+    const size_t base = (size_t)(valid ? jl : 0) * g.N + n2;
 #pragma unroll
     for (int i = 0; i < 8; i++) v[i] = valid ? yin[base + (size_t)(t + i * F::T) * g.N2] : cw{0, 0};
   }
   uint32_t la_n = 0, lb_n = 0, self_n = 0;
-  if (job < n_lines) {
-    la_n = pl.line_a[job];
-    lb_n = pl.line_b[job];
-    self_n = pl.line_self[job];
+  if (lbeg + jl < lend) {
+    la_n = pl.line_a[lbeg + jl];
+    lb_n = pl.line_b[lbeg + jl];
+    self_n = pl.line_self[lbeg + jl];
   }
   LineTw<L, W> lt;
   load_line_tw<L>(lt, coltw, t);
 #pragma unroll
   for (int i = 0; i < 8; i++) w4[i] = fourtw[n2 * (t + i * F::T)];
   const W sc = (W)scale;
-  const int wave_job0 = (tid / 64) * (64 / F::T);  // first line slot of this wavefront
-  for (int r0 = 0; r0 < g.M; r0 += F::NL) {
-    const int q = r0 + job;
+  const int wave_jl0 = (tid / 64) * (64 / (F::T * CW));  // first row job of this wavefront
+  for (int r0 = 0; r0 < g.M; r0 += JL) {
+    const int q = r0 + jl;
     const bool valid = q < g.M;
     const size_t base = (size_t)(valid ? q : 0) * g.N + n2;
-    if (r0 + wave_job0 >= g.M) continue;  // no field of this wavefront in the round
+    if (r0 + wave_jl0 >= g.M) continue;  // no field of this wavefront in the round
     if (r0 > 0) {
 #pragma unroll
       for (int i = 0; i < 8; i++) v[i] = valid ? yin[base + (size_t)(t + i * F::T) * g.N2] : cw{0, 0};
@@ -704,30 +719,30 @@ __global__ void __launch_bounds__(256) k_colstage_fields_pairs(Geometry g, const
       for (int i = 0; i < 8; i++) {
         const int n1 = t + i * F::T;
         const cw x = cscale(v[i], sc);
-        U[base + (size_t)n1 * g.N2] = x;
-        Ush[q * L + n1] = x;
+        if (grp == 0) U[base + (size_t)n1 * g.N2] = x;
+        Ush[(q * L + n1) * CW + c] = x;
       }
     }
   }
   __syncthreads();
-  for (int r0 = 0; r0 < n_lines; r0 += F::NL) {
-    const int q = r0 + job;
-    const bool valid = q < n_lines;
-    if (r0 + wave_job0 >= n_lines) continue;  // no line of this wavefront in this round or later
+  for (int r0 = lbeg; r0 < lend; r0 += JL) {
+    const int q = r0 + jl;
+    const bool valid = q < lend;
+    if (r0 + wave_jl0 >= lend) continue;  // no line of this wavefront in this round or later
     // cross line: U_a conj(U_b); self line: |U_a|^2 + i |U_b|^2 (b absent: 0)
     const uint32_t la = la_n, lb = lb_n, self = self_n;  // lanes past the last line read line 0's
-    if (q + F::NL < n_lines) {
-      la_n = pl.line_a[q + F::NL];
-      lb_n = pl.line_b[q + F::NL];
-      self_n = pl.line_self[q + F::NL];
+    if (q + JL < lend) {
+      la_n = pl.line_a[q + JL];
+      lb_n = pl.line_b[q + JL];
+      self_n = pl.line_self[q + JL];
     }
     const bool has_b = lb != 0xffffffffu;
-    const int ia = (int)la * L, ib = (int)(has_b ? lb : la) * L;
+    const int ia = (int)la * L * CW + c, ib = (int)(has_b ? lb : la) * L * CW + c;
     const W bmask = has_b ? (W)1 : (W)0;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
       const int n1 = t + i * F::T;
-      const cw ua = Ush[ia + n1], ub = Ush[ib + n1];
+      const cw ua = Ush[ia + n1 * CW], ub = Ush[ib + n1 * CW];
       const cw cross = cmulc(ua, ub);
       const cw selfv = {ua.re * ua.re + ua.im * ua.im, bmask * (ub.re * ub.re + ub.im * ub.im)};
       v[i] = !valid ? cw{0, 0} : (self ? selfv : cross);
@@ -1220,13 +1235,43 @@ hipError_t launch_gather(const Geometry& g, const cpx<W>* U, const cpx<W>* VR, c
   return hipGetLastError();
 }
 
+size_t colstage_pairs_lds_bytes(int M, int columns, bool mixed) {
+  return (size_t)(ColStageCfg::FFT + M * kColStageL * columns) * (mixed ? sizeof(cf) : sizeof(cd));
+}
+
+// Measured on one MI355X (M = 2..15, NT = 4096..65536, 1 to 36 fibers): four columns per
+// workgroup took the pass from 0.6-1.5 TB/s to 2-3.4 TB/s; eight gained nothing over four
+// (twice the LDS, lower occupancy). Line groups fill the CUs when the column blocks alone are
+// too few, but each group transforms every field again, so a group keeps at least 4 lines.
+void colstage_pairs_plan(int M, int N2, int n_lines, int F, bool mixed, int& columns, int& groups) {
+  columns = 1;
+  for (int cw : {4, 2})
+    if (N2 % cw == 0 && colstage_pairs_lds_bytes(M, cw, mixed) <= kColStagePairsLds) {
+      columns = cw;
+      break;
+    }
+  groups = 1;
+  while ((long)(N2 / columns) * F * groups < 512 && groups * 2 * 4 <= n_lines) groups *= 2;
+}
+
 template <class W>
 hipError_t launch_colstage_fields_pairs(const Geometry& g, const Twiddles& tw, const cpx<W>* yin, cpx<W>* U,
                                         cpx<W>* conv, int n_lines, const PairLineArgs& pl, double scale,
-                                        const Batch& b, int F, hipStream_t s) {
-  if (g.N1 != kColStageL || g.M > kColStageMaxM || F < 1 || F > 65535) return hipErrorInvalidValue;
-  MMP_LAUNCH((k_colstage_fields_pairs<W>), dim3(g.N2, F), dim3(256), 0, s, g, TwOf<W>::col(tw),
-                     TwOf<W>::four(tw), yin, U, conv, n_lines, pl, scale, b);
+                                        const Batch& b, int F, int columns, int groups, hipStream_t s) {
+  if (g.N1 != kColStageL || g.M < 1 || g.M > kColStageMaxM || F < 1 || F > 65535 || n_lines < 1) return hipErrorInvalidValue;
+  if (columns < 1 || g.N2 % columns || groups < 1 || groups > n_lines) return hipErrorInvalidValue;
+  const size_t lds = colstage_pairs_lds_bytes(g.M, columns, std::is_same<W, float>::value);
+  if (lds > kColStagePairsLds) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)(g.N2 / columns * groups), F);
+  switch (columns) {
+    case 1: MMP_LAUNCH((k_colstage_fields_pairs<W, 1>), grid, dim3(256), lds, s, g, TwOf<W>::col(tw),
+                       TwOf<W>::four(tw), yin, U, conv, n_lines, pl, scale, b, groups); break;
+    case 2: MMP_LAUNCH((k_colstage_fields_pairs<W, 2>), grid, dim3(256), lds, s, g, TwOf<W>::col(tw),
+                       TwOf<W>::four(tw), yin, U, conv, n_lines, pl, scale, b, groups); break;
+    case 4: MMP_LAUNCH((k_colstage_fields_pairs<W, 4>), grid, dim3(256), lds, s, g, TwOf<W>::col(tw),
+                       TwOf<W>::four(tw), yin, U, conv, n_lines, pl, scale, b, groups); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
@@ -1321,7 +1366,7 @@ hipError_t launch_energy(const Geometry& g, const cd* fields, double* out, doubl
                                              const GatherArgs&, const W*, hipStream_t);                            \
   template hipError_t launch_colstage_fields_pairs<W>(const Geometry&, const Twiddles&, const cpx<W>*, cpx<W>*,     \
                                                       cpx<W>*, int, const PairLineArgs&, double, const Batch&, int, \
-                                                      hipStream_t);                                                \
+                                                      int, int, hipStream_t);                                      \
   template hipError_t launch_colstage_raman_gather<W>(const Geometry&, const Twiddles&, const cpx<W>*,             \
                                                       const cpx<W>*, cpx<W>*, const GatherArgs&, const FusedArgs&, \
                                                       const W*, double, const Batch&, int, hipStream_t);

@@ -106,6 +106,8 @@ struct mmp_solver {
   bool use_fused = false;
   int chunk_lines = 0;       // column-stage gather: Raman lines per LDS chunk at most
   bool tuned = false;        // MMP_FLAG_AUTOTUNE chose fused.waves and chunk_lines
+  int pair_columns = 1;      // column-stage pair pass: columns per workgroup and line groups
+  int pair_groups = 1;
   bool chain = false;        // the step close also issues the next step's stage-0 head
   double* qdense = nullptr;  // [P][M][M] (or parity-compact) dense coupling blocks for the register gathers
   // work arrays of the step in the work precision (the FP64 buffers above in FP64 mode)
@@ -265,7 +267,8 @@ mmp_status enqueue_step_w(mmp_solver* s) {
     const int F = (int)s->F;
     if (s->use_fused) {
       MMP_TRY(run_kernel(s, "col_fields_pairs", cb * (2 * MN + LN) * F, [&] {
-        return launch_colstage_fields_pairs<W>(g, s->tw, ybuf, U, conv, s->n_lines, s->pairs, invN, s->batch, F, st);
+        return launch_colstage_fields_pairs<W>(g, s->tw, ybuf, U, conv, s->n_lines, s->pairs, invN, s->batch, F,
+                                               s->pair_columns, s->pair_groups, st);
       }));
       MMP_TRY(run_kernel(s, "row_raman", cb * (2 * LN + P.N) * F, [&] {
         return launch_row_raman<W>(g, s->tw, conv, s->n_lines, hspec, s->batch, F, st);
@@ -731,6 +734,7 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
     s->fused = FusedArgs{d_cl,   d_cs, d_meta, P.fused_chunks, P.fused_max_lines, P.fused_max_slots, compact ? 1 : 0,
                          P.M <= 6 ? 8 : 4};
     s->chunk_lines = colstage_lines_per_chunk((int)P.M);
+    colstage_pairs_plan((int)P.M, s->g.N2, s->n_lines, (int)s->F, s->mixed, s->pair_columns, s->pair_groups);
   }
   // stage heads: tableau coefficients here, each fiber's h, sgn and tables in the kernel
   // the close head also builds the next step's stage 0 unless the stage-0 Lawson table is
@@ -1183,6 +1187,8 @@ mmp_status mmp_fiber_info(mmp_solver* s, uint32_t f, mmp_info* o) {
   o->gather_waves = s->use_fused ? (uint32_t)s->fused.waves : 0u;
   o->chunk_lines = s->use_fused ? (uint32_t)s->chunk_lines : 0u;
   o->tuned = s->tuned ? 1u : 0u;
+  o->pair_columns = s->use_fused ? (uint32_t)s->pair_columns : 0u;
+  o->pair_groups = s->use_fused ? (uint32_t)s->pair_groups : 0u;
   return MMP_OK;
 }
 
