@@ -133,6 +133,7 @@ typedef struct mmp_info {
   uint32_t tuned;            /* 1 when MMP_FLAG_AUTOTUNE chose the two above */
   uint32_t pair_columns;     /* column-stage pair pass: neighbouring columns per workgroup (0: separate passes) */
   uint32_t pair_groups;      /* column-stage pair pass: workgroups sharing one column block's Raman lines */
+  uint32_t gather_hi_occupancy; /* 1 when the gather runs its three-workgroups-per-CU variant */
   uint32_t reserved;
 } mmp_info;
 

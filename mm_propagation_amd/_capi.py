@@ -84,7 +84,8 @@ class MmpInfo(ctypes.Structure):
         ("device_bytes", c_size_t),
         ("n_fibers", c_uint32), ("live", c_uint32),
         ("gather_waves", c_uint32), ("chunk_lines", c_uint32), ("tuned", c_uint32),
-        ("pair_columns", c_uint32), ("pair_groups", c_uint32), ("reserved", c_uint32),
+        ("pair_columns", c_uint32), ("pair_groups", c_uint32),
+        ("gather_hi_occupancy", c_uint32), ("reserved", c_uint32),
     ]
 
 

@@ -159,6 +159,8 @@ struct FusedArgs {
   int parity;                  // coupling blocks hold only the entries (p, k) with (p + k) % 2 == the
                                // slot's parity: M x ceil(M/2) per slot, k = 2 kk + ((p + parity) % 2)
   int waves;                   // wavefronts of the gather workgroup: 4, or 8 (colstage_wide_ok)
+  int hi_occupancy;            // with 4 wavefronts: the three-workgroups-per-CU variant where it
+                               // exists (FP32, 7 to 10 modes); same arithmetic, same results
 };
 
 // Limits of one column-stage chunk: Raman lines, slots, and coupling coefficients
@@ -166,6 +168,8 @@ struct FusedArgs {
 int colstage_max_modes();
 // Whether the 8-wavefront gather workgroup exists for M modes in this precision.
 bool colstage_wide_ok(int M, bool mixed);
+// Whether the three-workgroups-per-CU 4-wavefront gather exists (FusedArgs::hi_occupancy).
+bool colstage_hi_occupancy_ok(int M, bool mixed, bool parity);
 int colstage_lines_per_chunk(int M);
 int colstage_slots_per_chunk();
 int colstage_coefficients_per_chunk(int M, bool mixed);

@@ -612,7 +612,9 @@ constexpr int kColStageMaxM = 16;    // modes kept in registers by the gather
 // blocks, and a small LDS footprint lets more workgroups share a CU (batched fibers)
 constexpr int colstage_lines_cap(int M) { return M <= 4 ? 16 : (M <= 6 ? 24 : 40); }  // Raman lines
 // coupling coefficients held in LDS (64 KB FP64 would crowd out the lines)
-constexpr int colstage_qcap(int M, bool mixed) { return M <= 4 ? 1024 : (M <= 6 ? 2048 : (mixed ? 8192 : 4096)); }
+constexpr int colstage_qcap(int M, bool mixed) {
+  return M <= 4 ? 1024 : (M <= 6 ? 2048 : ((mixed && M > 10) ? 8192 : 4096));
+}
 template <int M, class W>
 constexpr int kColStageQCapOf = colstage_qcap(M, std::is_same<W, float>::value);
 // of those, prefetched into registers before the chunk's barrier (the rest go straight to LDS):
@@ -756,10 +758,20 @@ __global__ void __launch_bounds__(256) k_colstage_fields_pairs(Geometry g, const
   }
 }
 
+// 4-wavefront FP32 gathers of 7 to 10 modes also come in a variant held to 168 VGPRs, so
+// that three workgroups share a CU (their LDS is under 50 KB); it reads the coupling rows in
+// 64-byte groups to make room, and spills at most 12 dwords. Measured on one MI355X it pays
+// for large grids only: batched M = 10, NT = 16384 x 36 fibers 563 -> 480 us, M = 8 x 16
+// fibers 125 -> 102 us; one fiber at NT = 65536 (1024 workgroups) 65 -> 67 us. Beyond 10
+// modes (and without parity blocks at 10) the bound spills hundreds of bytes per lane.
+template <int M, class W, bool PAR>
+constexpr bool kColStageHiOcc = std::is_same<W, float>::value && M >= 7 && (M <= 9 || (M == 10 && PAR));
+bool colstage_hi_occupancy_ok(int M, bool mixed, bool parity) { return mixed && M >= 7 && (M <= 9 || (M == 10 && parity)); }
+
 // NW wavefronts (4, or 8 when M <= 6); each line transform runs inside one wavefront,
 // in place in its padded LDS row, and wavefronts without a line skip it.
-template <int M, int NW, class W, bool PAR>
-__global__ void __launch_bounds__(NW * 64) k_colstage_raman_gather(Geometry g, const cpx<W>* __restrict__ coltw,
+template <int M, int NW, class W, bool PAR, bool HI = false>
+__global__ void __launch_bounds__(NW * 64, HI ? 3 : 1) k_colstage_raman_gather(Geometry g, const cpx<W>* __restrict__ coltw,
                                                                    const cpx<W>* __restrict__ fourtw,
                                                                    const cpx<W>* __restrict__ U,
                                                                    const cpx<W>* __restrict__ conv,
@@ -900,11 +912,11 @@ __global__ void __launch_bounds__(NW * 64) k_colstage_raman_gather(Geometry g, c
 #pragma unroll
         for (int k = 0; k < M; k++) y[k] = cmul(X, u[k]);
         // the block's rows in groups of about 224 bytes (128 for FP32 beyond 10 modes, whose
-        // registers are fuller): each group's LDS reads are issued together (the scheduling
+        // registers are fuller; 64 in the 168-VGPR variant): each group's LDS reads are issued together (the scheduling
         // barrier keeps them ahead of their uses), so one LDS latency is exposed per group
         // rather than one per pair of coefficients
         constexpr int QROW = PAR ? KH : M;  // coefficients per row p
-        constexpr int QGB = (std::is_same<W, float>::value && M > 10) ? 128 : 224;
+        constexpr int QGB = HI ? 64 : ((std::is_same<W, float>::value && M > 10) ? 128 : 224);
         constexpr int PG = (int)(QGB / sizeof(W)) / QROW > 0 ? (int)(QGB / sizeof(W)) / QROW : 1;
 #pragma unroll
         for (int p0 = 0; p0 < M; p0 += PG) {
@@ -1276,6 +1288,12 @@ hipError_t launch_colstage_fields_pairs(const Geometry& g, const Twiddles& tw, c
 }
 
 #define MMP_COLSTAGE_LAUNCH(MV, PARV)                                                                             \
+  if constexpr (kColStageHiOcc<MV, W, PARV>)                                                                      \
+    if (fa.waves == 4 && fa.hi_occupancy) {                                                                       \
+      MMP_LAUNCH((k_colstage_raman_gather<MV, 4, W, PARV, true>), dim3(g.N2, F), dim3(256), 0, s, g,             \
+                 TwOf<W>::col(tw), TwOf<W>::four(tw), U, conv, yout, ga, fa, Qd, scale, b);                        \
+      break;                                                                                                      \
+    }                                                                                                             \
   if constexpr (kColStageWide<MV, W>)                                                                             \
     if (fa.waves == 8) {                                                                                          \
       MMP_LAUNCH((k_colstage_raman_gather<MV, 8, W, PARV>), dim3(g.N2, F), dim3(512), 0, s, g, TwOf<W>::col(tw), \

@@ -735,6 +735,8 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
                          P.M <= 6 ? 8 : 4};
     s->chunk_lines = colstage_lines_per_chunk((int)P.M);
     colstage_pairs_plan((int)P.M, s->g.N2, s->n_lines, (int)s->F, s->mixed, s->pair_columns, s->pair_groups);
+    // grids of 8 or more workgroups per CU
+    s->fused.hi_occupancy = colstage_hi_occupancy_ok((int)P.M, s->mixed, compact) && (long)s->g.N2 * s->F >= 2048;
   }
   // stage heads: tableau coefficients here, each fiber's h, sgn and tables in the kernel
   // the close head also builds the next step's stage 0 unless the stage-0 Lawson table is
@@ -1189,6 +1191,7 @@ mmp_status mmp_fiber_info(mmp_solver* s, uint32_t f, mmp_info* o) {
   o->tuned = s->tuned ? 1u : 0u;
   o->pair_columns = s->use_fused ? (uint32_t)s->pair_columns : 0u;
   o->pair_groups = s->use_fused ? (uint32_t)s->pair_groups : 0u;
+  o->gather_hi_occupancy = s->use_fused && s->fused.waves == 4 && s->fused.hi_occupancy ? 1u : 0u;
   return MMP_OK;
 }
 

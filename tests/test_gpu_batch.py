@@ -49,6 +49,25 @@ def test_batch_equals_single_solves(M, nt, precision):
         s.close()
 
 
+@pytest.mark.parametrize("M", [7, 10])
+def test_large_grid_variants_equal_single_solves(M):
+    """16 fibers at NT = 8192 make 2048 gather workgroups: the batch runs the three-workgroups-
+    per-CU gather and a four-column pair pass with one line group, a fiber alone the two-per-CU
+    gather and four or eight line groups. The arithmetic is the same, so the results are too."""
+    require_device()
+    F = 16
+    sps = fibers(M, 8192, F, base=40)
+    note(f"case large grid M={M} NT=8192 fibers={F}")
+    b = MultimodePropagationBatch(sps, pair_decode=1, precision=_capi.MMP_PRECISION_MIXED)
+    info = b.info(0)
+    assert (info["pair_columns"], info["gather_waves"], info["gather_hi_occupancy"]) == (4, 4, 1)
+    b.computeLawsonRK(24)
+    got = {f: b.result(f) for f in (0, 9, F - 1)}
+    b.close()
+    for f, r in got.items():
+        assert np.array_equal(r, single(sps[f], 24, _capi.MMP_PRECISION_MIXED)), f
+
+
 def test_batch_against_oracle():
     require_device()
     sps = fibers(6, 4096, 2, base=7)
