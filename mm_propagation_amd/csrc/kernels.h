@@ -6,7 +6,10 @@
 // A forward transform is a column pass (length-N1 DFT down each column n2,
 // then twiddle W_N^(n2*k1)) followed by a row pass (length-N2 DFT along each
 // row k1); the inverse runs the row pass first. Spectral state never leaves
-// the four-step order, so no transposes are needed.
+// the four-step order, so no transposes are needed. Inside one column-stage
+// step, the arrays that pass from a column pass to a column pass (the fields U,
+// and the Raman lines after their row pass) are kept in column order instead,
+// x[q][n2*N1 + n1], so that the column kernels read whole columns contiguously.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -203,6 +206,11 @@ hipError_t launch_row_inverse_phase(const Geometry& g, const Twiddles& tw, const
 template <class W>
 hipError_t launch_row_raman(const Geometry& g, const Twiddles& tw, cpx<W>* data, int lines, const cpx<W>* hspec,
                             const Batch& b, int F, hipStream_t s);
+// The column-stage step's Raman row pass: reads the lines as k_row_raman does and writes them
+// to `out` (another buffer) in column order, line q's sample (n1, n2) at q * N + n2 * N1 + n1.
+template <class W>
+hipError_t launch_row_raman_cols(const Geometry& g, const Twiddles& tw, const cpx<W>* in, cpx<W>* out, int lines,
+                                 const cpx<W>* hspec, const Batch& b, int F, hipStream_t s);
 template <class W>
 hipError_t launch_gather(const Geometry& g, const cpx<W>* U, const cpx<W>* VR, cpx<W>* sigma, const GatherArgs& a,
                          hipStream_t s);

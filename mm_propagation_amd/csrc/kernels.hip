@@ -432,6 +432,87 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_row_raman(Geometry g, co
   }
 }
 
+// Raman row pass of the column-stage step: the arithmetic of k_row_raman, with R consecutive
+// rows of one line per workgroup (n1 .. n1 + R - 1), and the result stored to `out` in column
+// order (sample (n1, n2) of a line at n2 * N1 + n1). The rows are staged in LDS as
+// [n2][R + 1] so that every store writes runs of R consecutive samples; the gather then reads
+// each column of a line as one contiguous run instead of N1 samples N2 apart. R is the
+// largest of 16 (FP32) or 8 (FP64), halved as needed, that keeps the workgroup within 512
+// threads and 50 KB of LDS: measured on one MI355X, wider workgroups lose more to occupancy
+// than the longer runs gain (FP64 rows of 1024: R = 2 60 us, R = 4 83 us; FP32 rows of 1024:
+// R = 4 21.7 us, R = 8 27.4 us; FP32 rows of 256: R = 16 111 us, R = 8 131 us).
+template <int L, class W>
+constexpr int raman_cols_rows() {
+  constexpr int slots = LineSlots<L, 1>::value;
+  int r = std::is_same<W, float>::value ? 16 : 8;
+  while (r > 1) {
+    const int lds = (r * slots > L * (r + 1) ? r * slots : L * (r + 1)) * (int)sizeof(cpx<W>);
+    if (r * (L / 8) <= 512 && lds <= 51200) break;
+    r /= 2;
+  }
+  return r;
+}
+template <int L, class W>
+struct RamanColsCfg {
+  static constexpr int T = L / 8;                              // lanes per row
+  static constexpr int R = raman_cols_rows<L, W>();            // rows per workgroup
+  static constexpr int THREADS = R * T;
+  static constexpr bool WAVE = T <= 64;                        // each row inside one wavefront
+  static constexpr int SLOTS = LineSlots<L, 1>::value;
+  static constexpr int FFT = R * SLOTS;                        // transform rows
+  static constexpr int STAGE = L * (R + 1);                    // [n2][R + 1] staging
+  static constexpr int LDS = FFT > STAGE ? FFT : STAGE;
+};
+
+template <int L, class W>
+__global__ void __launch_bounds__((RamanColsCfg<L, W>::THREADS)) k_row_raman_cols(Geometry g, const cpx<W>* __restrict__ rowtw,
+                                                                            const cpx<W>* __restrict__ in,
+                                                                            cpx<W>* __restrict__ out,
+                                                                            const cpx<W>* __restrict__ hspec, Batch b) {
+  using C = RamanColsCfg<L, W>;
+  using cw = cpx<W>;
+  __shared__ cw smem[C::LDS];
+  const int fib = blockIdx.y;
+  if (!b.fp[fib].live) return;  // uniform over the workgroup
+  in += fib * b.sl;
+  out += fib * b.sl;
+  hspec += fib * b.sh;
+  const double hscale = b.fp[fib].dt;
+  const int tid = threadIdx.x;
+  const int r = tid / C::T, t = tid % C::T;
+  // R divides N1: the rows share one line; neighbouring row groups run on one XCD, so the
+  // halves of a cache line that two of them store meet in one L2
+  const int row0 = xcd_block(blockIdx.x, gridDim.x) * C::R;
+  const int line = row0 / g.N1, n10 = row0 % g.N1, n1 = n10 + r;
+  const size_t base = (size_t)line * g.N + (size_t)n1 * g.N2;
+  const size_t hbase = (size_t)n1 * g.N2;
+  cw* sm = smem + r * C::SLOTS;
+  LineTw<L, W> lt;
+  load_line_tw<L>(lt, rowtw, t);
+  cw v[8], hs[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    v[i] = in[base + t + i * C::T];
+    hs[i] = hspec[hbase + t + i * C::T];
+  }
+  line_fft<L, -1, 1, 1, W, C::WAVE>(v, sm, lt, t);
+  const W hsc = (W)hscale;
+#pragma unroll
+  for (int i = 0; i < 8; i++) v[i] = cscale(cmul(v[i], hs[i]), hsc);
+  line_fft<L, 1, 1, 1, W, C::WAVE>(v, sm, lt, t);
+  __syncthreads();  // every row's transform is done with its LDS row before the staging reuses it
+#pragma unroll
+  for (int i = 0; i < 8; i++) smem[(t + i * C::T) * (C::R + 1) + r] = v[i];
+  __syncthreads();
+  const size_t obase = (size_t)line * g.N + n10;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const int e = tid + k * C::THREADS;  // R * L = 8 * THREADS staged samples
+    const int rr = e % C::R, n2 = e / C::R;
+    out[obase + (size_t)n2 * g.N1 + rr] = smem[n2 * (C::R + 1) + rr];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Kerr + Raman gather over a tile of 64 consecutive samples:
 //   X_slot = kv * U_a conj(U_b) + kr * VR_slot        (slot mix, into LDS)
@@ -599,11 +680,14 @@ __global__ void __launch_bounds__(512) k_gather(Geometry g, const cpx<W>* __rest
 // Column-stage kernels (Raman on, N1 = 64, M <= 16). One workgroup of 256 (or 512) lanes
 // owns one column n2 and does every column-side step of a stage there, with the
 // column of each field and Raman line held in LDS between the steps:
-//   C1  inverse column DFT of the M fields -> U (HBM and LDS) -> pair products ->
-//       forward column DFT of the Raman lines -> conv
-//   C2  inverse column DFT of the Raman lines (kept in LDS) -> dense Kerr + Raman
-//       gather of the 64 samples, forming each slot mix X_s on the way -> forward
+//   C1  inverse column DFT of the M fields -> U (HBM, column order, and LDS) -> pair
+//       products -> forward column DFT of the Raman lines -> conv (row order, for the
+//       Raman row pass, which writes them back in column order)
+//   C2  inverse column DFT of the Raman lines (column order; kept in LDS) -> dense Kerr +
+//       Raman gather of the 64 samples, forming each slot mix X_s on the way -> forward
 //       column DFT of sigma -> yout
+// Column order: sample (n1, n2) of an array at n2 * N1 + n1, so that a column is one
+// contiguous run of N1 samples.
 // Lanes form 32 line slots of 8 lanes; a round transforms up to 32 lines.
 // ---------------------------------------------------------------------------
 constexpr int kColStageL = 64;       // column length the column-stage kernels take
@@ -721,7 +805,7 @@ __global__ void __launch_bounds__(256) k_colstage_fields_pairs(Geometry g, const
       for (int i = 0; i < 8; i++) {
         const int n1 = t + i * F::T;
         const cw x = cscale(v[i], sc);
-        if (grp == 0) U[base + (size_t)n1 * g.N2] = x;
+        if (grp == 0) U[(size_t)q * g.N + (size_t)n2 * L + n1] = x;  // column order, for the gather
         Ush[(q * L + n1) * CW + c] = x;
       }
     }
@@ -811,13 +895,13 @@ __global__ void __launch_bounds__(NW * 64, HI ? 3 : 1) k_colstage_raman_gather(G
   int l0 = 0, l1 = (int)fa.chunk_line[1], s0 = 0, s1 = (int)fa.chunk_slot[1];
   {
     const bool valid = job < l1;
-    const size_t base = (size_t)(valid ? job : 0) * g.N + n2;
+    const size_t base = (size_t)(valid ? job : 0) * g.N + (size_t)n2 * L;
 #pragma unroll
-    for (int i = 0; i < 8; i++) v[i] = valid ? conv[base + (size_t)(t + i * T) * g.N2] : cw{0, 0};
+    for (int i = 0; i < 8; i++) v[i] = valid ? conv[base + t + i * T] : cw{0, 0};
 #pragma unroll
     for (int r = 0; r < UIN; r++) {
       const int e = tid + r * NT;
-      uin[r] = e < M * L ? U[(size_t)(e / L) * g.N + (size_t)(e % L) * g.N2 + n2] : cw{0, 0};
+      uin[r] = e < M * L ? U[(size_t)(e / L) * g.N + (size_t)n2 * L + e % L] : cw{0, 0};
     }
   }
   constexpr int QIN = kColStageQPreOf<M, W> / NT;
@@ -866,9 +950,9 @@ __global__ void __launch_bounds__(NW * 64, HI ? 3 : 1) k_colstage_raman_gather(G
       const bool valid = q < l1;
       if (r0 + wave * (64 / T) >= l1) continue;  // no line of this wavefront in the round
       if (c > 0 || r0 > l0) {
-        const size_t base = (size_t)(valid ? q : 0) * g.N + n2;
+        const size_t base = (size_t)(valid ? q : 0) * g.N + (size_t)n2 * L;
 #pragma unroll
-        for (int i = 0; i < 8; i++) v[i] = valid ? conv[base + (size_t)(t + i * T) * g.N2] : cw{0, 0};
+        for (int i = 0; i < 8; i++) v[i] = valid ? conv[base + t + i * T] : cw{0, 0};
       }
 #pragma unroll
       for (int i = 0; i < 8; i++) v[i] = cmulc(v[i], w4[i]);
@@ -1234,6 +1318,22 @@ hipError_t launch_row_raman(const Geometry& g, const Twiddles& tw, cpx<W>* data,
   MMP_DISPATCH_LW(g.N2, row_raman_impl, W, g, tw, data, lines, hspec, b, F, s);
 }
 
+template <int L, class W>
+static hipError_t row_raman_cols_impl(const Geometry& g, const Twiddles& tw, const cpx<W>* in, cpx<W>* out, int lines,
+                                      const cpx<W>* hspec, const Batch& b, int F, hipStream_t s) {
+  using C = RamanColsCfg<L, W>;
+  if (g.N1 % C::R) return hipErrorInvalidValue;
+  MMP_LAUNCH((k_row_raman_cols<L, W>), dim3(lines * g.N1 / C::R, F), dim3(C::THREADS), 0, s, g, TwOf<W>::row(tw), in,
+             out, hspec, b);
+  return hipGetLastError();
+}
+template <class W>
+hipError_t launch_row_raman_cols(const Geometry& g, const Twiddles& tw, const cpx<W>* in, cpx<W>* out, int lines,
+                                 const cpx<W>* hspec, const Batch& b, int F, hipStream_t s) {
+  if (F < 1 || F > 65535 || lines < 1 || in == out) return hipErrorInvalidValue;
+  MMP_DISPATCH_LW(g.N2, row_raman_cols_impl, W, g, tw, in, out, lines, hspec, b, F, s);
+}
+
 template <class W>
 hipError_t launch_gather(const Geometry& g, const cpx<W>* U, const cpx<W>* VR, cpx<W>* sigma, const GatherArgs& a,
                          hipStream_t s) {
@@ -1382,6 +1482,8 @@ hipError_t launch_energy(const Geometry& g, const cd* fields, double* out, doubl
                                        hipStream_t);                                                              \
   template hipError_t launch_gather_dense<W>(const Geometry&, const cpx<W>*, const cpx<W>*, cpx<W>*,               \
                                              const GatherArgs&, const W*, hipStream_t);                            \
+  template hipError_t launch_row_raman_cols<W>(const Geometry&, const Twiddles&, const cpx<W>*, cpx<W>*, int,         \
+                                               const cpx<W>*, const Batch&, int, hipStream_t);                      \
   template hipError_t launch_colstage_fields_pairs<W>(const Geometry&, const Twiddles&, const cpx<W>*, cpx<W>*,     \
                                                       cpx<W>*, int, const PairLineArgs&, double, const Batch&, int, \
                                                       int, int, hipStream_t);                                      \

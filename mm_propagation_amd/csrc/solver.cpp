@@ -116,6 +116,7 @@ struct mmp_solver {
   void* ybuf_w = nullptr;
   void* U_w = nullptr;
   void* conv_w = nullptr;
+  void* convt_w = nullptr;   // column-stage step: the Raman lines in column order after their row pass
   void* hspec_w = nullptr;
   const void* q_w = nullptr;
   const void* qdense_w = nullptr;
@@ -254,6 +255,7 @@ mmp_status enqueue_step_w(mmp_solver* s) {
   cw* ybuf = static_cast<cw*>(s->ybuf_w);
   cw* U = static_cast<cw*>(s->U_w);
   cw* conv = static_cast<cw*>(s->conv_w);
+  cw* convt = static_cast<cw*>(s->convt_w);
   const cw* hspec = static_cast<const cw*>(s->hspec_w);
   const W* qdense = static_cast<const W*>(s->qdense_w);
   const double cb = sizeof(cw);  // bytes per complex work value
@@ -271,11 +273,11 @@ mmp_status enqueue_step_w(mmp_solver* s) {
                                                s->pair_columns, s->pair_groups, st);
       }));
       MMP_TRY(run_kernel(s, "row_raman", cb * (2 * LN + P.N) * F, [&] {
-        return launch_row_raman<W>(g, s->tw, conv, s->n_lines, hspec, s->batch, F, st);
+        return launch_row_raman_cols<W>(g, s->tw, conv, convt, s->n_lines, hspec, s->batch, F, st);
       }));
       MMP_TRY(run_kernel(s, "col_raman_gather", cb * (2 * MN + LN) * F, [&] {
-        return launch_colstage_raman_gather<W>(g, s->tw, U, conv, ybuf, s->gather, s->fused, qdense, invN, s->batch, F,
-                                               st);
+        return launch_colstage_raman_gather<W>(g, s->tw, U, convt, ybuf, s->gather, s->fused, qdense, invN, s->batch,
+                                               F, st);
       }));
       continue;
     }
@@ -413,8 +415,10 @@ size_t planned_device_bytes(const Problem& P, const mmp_options& opt, size_t F) 
     b += 8 * (2 * MN) * F + 8 * (P.N2 + P.N1 + P.N) + 4 * c(P.nnz);
     if (P.raman) b += 8 * (size_t)P.N * F;
   }
-  if (P.fused_chunks > 0 && !(opt.flags & MMP_FLAG_SEPARATE_COLUMNS))
+  if (P.fused_chunks > 0 && !(opt.flags & MMP_FLAG_SEPARATE_COLUMNS)) {
     b += 4 * (2 * (P.line_a.size() + 1) + c(P.P));  // chunk bounds sized for the finest plan, slot metadata
+    if (P.raman) b += (mixed ? 8 : 16) * c(L) * P.N * F;  // the Raman lines in column order
+  }
   const bool fused = P.fused_chunks > 0 && !(opt.flags & MMP_FLAG_SEPARATE_COLUMNS);
   const size_t block = (fused && P.parity) ? (size_t)P.M * ((P.M + 1) / 2) : (size_t)P.M * P.M;
   if (fused || dense_gather_chunk((int)P.M) > 0) b += (mixed ? 4 : 8) * (size_t)P.P * block * F;
@@ -634,6 +638,11 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
     if (P.raman) {
       A(dalloc(s, &cw, F * conv_n));
       A(dalloc(s, &hw, F * (size_t)P.N));
+      if (s->use_fused) {
+        cf* ct;
+        A(dalloc(s, &ct, F * conv_n));
+        s->convt_w = ct;
+      }
     }
     A(dalloc(s, &rowtw32, P.N2));
     A(dalloc(s, &coltw32, P.N1));
@@ -649,6 +658,11 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
     }
   } else {
     if (P.raman) A(dalloc(s, &s->conv, F * conv_n));
+    if (P.raman && s->use_fused) {
+      cd* ct;
+      A(dalloc(s, &ct, F * conv_n));
+      s->convt_w = ct;
+    }
     s->ybuf_w = s->ybuf;
     s->U_w = s->U;
     s->conv_w = s->conv;
