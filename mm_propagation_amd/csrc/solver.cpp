@@ -451,6 +451,13 @@ void fill_info(const Problem& P, const mmp_options& opt, mmp_info* o) {
   const bool fused = P.fused_chunks > 0 && !(opt.flags & MMP_FLAG_SEPARATE_COLUMNS);
   o->gather_waves = fused ? (P.M <= 6 ? 8u : 4u) : 0u;  // the defaults, before any tuning
   o->chunk_lines = fused ? (uint32_t)colstage_lines_per_chunk((int)P.M) : 0u;
+  if (fused) {  // a solver of one fiber (a batch may choose differently; results do not depend on it)
+    int columns = 1, groups = 1;
+    colstage_pairs_plan((int)P.M, (int)P.N2, (int)P.line_a.size(), 1, opt.precision == MMP_PRECISION_MIXED, columns,
+                        groups);
+    o->pair_columns = (uint32_t)columns;
+    o->pair_groups = (uint32_t)groups;
+  }
 }
 
 // Fibers of one batch must share everything the kernels take from fiber 0's problem.

@@ -62,6 +62,25 @@ def test_plan_sizes_and_memory():
     assert 10e6 < i["device_bytes"] < 40e6
 
 
+FP64, MIXED = _capi.MMP_PRECISION_FP64, _capi.MMP_PRECISION_MIXED
+
+
+@pytest.mark.parametrize("M,nt,precision,expect", [
+    (6, 16384, FP64, (4, 4)),    # 64 blocks of 4 columns; 18 lines in 4 groups: 256 workgroups
+    (6, 16384, MIXED, (4, 4)),
+    (10, 65536, MIXED, (4, 2)),  # 256 blocks; 50 lines in 2 groups: 512 workgroups
+    (10, 65536, FP64, (2, 1)),   # 4 FP64 columns of 10 fields need 77.8 KB of LDS; 2 need 57.3 KB
+    (15, 65536, FP64, (1, 1)),   # 2 columns need 67.6 KB; 1024 one-column blocks
+    (2, 4096, FP64, (4, 1)),     # 2 lines: a second group would hold fewer than 4
+])
+def test_pair_pass_layout(M, nt, precision, expect):
+    """The column-stage pair pass of a one-fiber solver: widest column block whose LDS fits in
+    64 KB, then line groups of at least 4 lines until 512 workgroups (mmp_plan, host only)."""
+    i = plan.plan(configs.grin_fiber(M=M, nt=nt, symmetric=True), pair_decode=1, precision=precision)
+    assert (i["pair_columns"], i["pair_groups"]) == expect
+    assert i["gather_hi_occupancy"] == 0  # decided with the fiber count, by the solver
+
+
 def expect_status(sp, status, raman=True):
     with pytest.raises(_capi.MmpError) as ei:
         plan.plan(sp, raman=raman)
