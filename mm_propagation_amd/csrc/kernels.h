@@ -7,9 +7,9 @@
 // then twiddle W_N^(n2*k1)) followed by a row pass (length-N2 DFT along each
 // row k1); the inverse runs the row pass first. Spectral state never leaves
 // the four-step order, so no transposes are needed. Inside one column-stage
-// step, the arrays that pass from a column pass to a column pass (the fields U,
-// and the Raman lines after their row pass) are kept in column order instead,
-// x[q][n2*N1 + n1], so that the column kernels read whole columns contiguously.
+// step, the fields U and the Raman lines after their row pass (on large grids
+// also before it) are kept in column order instead, x[q][n2*N1 + n1], so that
+// the column kernels read and write whole columns contiguously.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -206,21 +206,24 @@ hipError_t launch_row_inverse_phase(const Geometry& g, const Twiddles& tw, const
 template <class W>
 hipError_t launch_row_raman(const Geometry& g, const Twiddles& tw, cpx<W>* data, int lines, const cpx<W>* hspec,
                             const Batch& b, int F, hipStream_t s);
-// The column-stage step's Raman row pass: reads the lines as k_row_raman does and writes them
-// to `out` (another buffer) in column order, line q's sample (n1, n2) at q * N + n2 * N1 + n1.
+// The column-stage step's Raman row pass: the transform of k_row_raman on the lines of `in`
+// (row order, or column order when in_cols: line q's sample (n1, n2) at q * N + n2 * N1 + n1),
+// written to `out` (another buffer) in column order.
 template <class W>
 hipError_t launch_row_raman_cols(const Geometry& g, const Twiddles& tw, const cpx<W>* in, cpx<W>* out, int lines,
-                                 const cpx<W>* hspec, const Batch& b, int F, hipStream_t s);
+                                 const cpx<W>* hspec, const Batch& b, int F, int in_cols, hipStream_t s);
 template <class W>
 hipError_t launch_gather(const Geometry& g, const cpx<W>* U, const cpx<W>* VR, cpx<W>* sigma, const GatherArgs& a,
                          hipStream_t s);
 // C1 takes `columns` (1, 2 or 4) neighbouring columns per workgroup and deals each column
 // block's Raman lines to `groups` workgroups; the LDS it then needs is
 // colstage_pairs_lds_bytes, at most kColStagePairsLds. Results do not depend on either.
+// conv_cols: store the Raman lines in column order (for launch_row_raman_cols with in_cols).
 template <class W>
 hipError_t launch_colstage_fields_pairs(const Geometry& g, const Twiddles& tw, const cpx<W>* yin, cpx<W>* U,
                                         cpx<W>* conv, int n_lines, const PairLineArgs& pl, double scale,
-                                        const Batch& b, int F, int columns, int groups, hipStream_t s);
+                                        const Batch& b, int F, int columns, int groups, int conv_cols,
+                                        hipStream_t s);
 template <class W>
 hipError_t launch_colstage_raman_gather(const Geometry& g, const Twiddles& tw, const cpx<W>* U, const cpx<W>* conv,
                                         cpx<W>* yout, const GatherArgs& ga, const FusedArgs& fa, const W* Qd,

@@ -433,10 +433,12 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_row_raman(Geometry g, co
 }
 
 // Raman row pass of the column-stage step: the arithmetic of k_row_raman, with R consecutive
-// rows of one line per workgroup (n1 .. n1 + R - 1), and the result stored to `out` in column
-// order (sample (n1, n2) of a line at n2 * N1 + n1). The rows are staged in LDS as
-// [n2][R + 1] so that every store writes runs of R consecutive samples; the gather then reads
-// each column of a line as one contiguous run instead of N1 samples N2 apart. R is the
+// rows of one line per workgroup (n1 .. n1 + R - 1), its lines read from `in` and stored to
+// `out` in column order (sample (n1, n2) of a line at n2 * N1 + n1). The rows are staged in
+// LDS as [n2][R + 1] on the way in and out, so that every load and store moves runs of R
+// consecutive samples; the column kernels on either side (the pair pass writing the lines,
+// the gather reading them) then move each column of a line as one contiguous run instead of
+// N1 samples N2 apart. R is the
 // largest of 16 (FP32) or 8 (FP64), halved as needed, that keeps the workgroup within 512
 // threads and 50 KB of LDS: measured on one MI355X, wider workgroups lose more to occupancy
 // than the longer runs gain (FP64 rows of 1024: R = 2 60 us, R = 4 83 us; FP32 rows of 1024:
@@ -468,7 +470,8 @@ template <int L, class W>
 __global__ void __launch_bounds__((RamanColsCfg<L, W>::THREADS)) k_row_raman_cols(Geometry g, const cpx<W>* __restrict__ rowtw,
                                                                             const cpx<W>* __restrict__ in,
                                                                             cpx<W>* __restrict__ out,
-                                                                            const cpx<W>* __restrict__ hspec, Batch b) {
+                                                                            const cpx<W>* __restrict__ hspec, Batch b,
+                                                                            int in_cols) {
   using C = RamanColsCfg<L, W>;
   using cw = cpx<W>;
   __shared__ cw smem[C::LDS];
@@ -484,16 +487,31 @@ __global__ void __launch_bounds__((RamanColsCfg<L, W>::THREADS)) k_row_raman_col
   // halves of a cache line that two of them store meet in one L2
   const int row0 = xcd_block(blockIdx.x, gridDim.x) * C::R;
   const int line = row0 / g.N1, n10 = row0 % g.N1, n1 = n10 + r;
-  const size_t base = (size_t)line * g.N + (size_t)n1 * g.N2;
+  const size_t cbase = (size_t)line * g.N + n10;  // the R rows' first sample in column order
   const size_t hbase = (size_t)n1 * g.N2;
   cw* sm = smem + r * C::SLOTS;
+  cw v[8], hs[8];
+  if (in_cols) {  // uniform branch: lines in column order, staged through LDS
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int e = tid + k * C::THREADS;  // R * L = 8 * THREADS samples
+      const int rr = e % C::R, n2 = e / C::R;
+      smem[n2 * (C::R + 1) + rr] = in[cbase + (size_t)n2 * g.N1 + rr];
+    }
+  } else {  // lines in row order: each row read directly
+    const size_t rbase = (size_t)line * g.N + (size_t)n1 * g.N2;
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = in[rbase + t + i * C::T];
+  }
+#pragma unroll
+  for (int i = 0; i < 8; i++) hs[i] = hspec[hbase + t + i * C::T];
   LineTw<L, W> lt;
   load_line_tw<L>(lt, rowtw, t);
-  cw v[8], hs[8];
+  if (in_cols) {
+    __syncthreads();
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    v[i] = in[base + t + i * C::T];
-    hs[i] = hspec[hbase + t + i * C::T];
+    for (int i = 0; i < 8; i++) v[i] = smem[(t + i * C::T) * (C::R + 1) + r];
+    __syncthreads();  // the staging area becomes the rows' transform slots
   }
   line_fft<L, -1, 1, 1, W, C::WAVE>(v, sm, lt, t);
   const W hsc = (W)hscale;
@@ -504,12 +522,11 @@ __global__ void __launch_bounds__((RamanColsCfg<L, W>::THREADS)) k_row_raman_col
 #pragma unroll
   for (int i = 0; i < 8; i++) smem[(t + i * C::T) * (C::R + 1) + r] = v[i];
   __syncthreads();
-  const size_t obase = (size_t)line * g.N + n10;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
     const int e = tid + k * C::THREADS;  // R * L = 8 * THREADS staged samples
     const int rr = e % C::R, n2 = e / C::R;
-    out[obase + (size_t)n2 * g.N1 + rr] = smem[n2 * (C::R + 1) + rr];
+    out[cbase + (size_t)n2 * g.N1 + rr] = smem[n2 * (C::R + 1) + rr];
   }
 }
 
@@ -681,8 +698,8 @@ __global__ void __launch_bounds__(512) k_gather(Geometry g, const cpx<W>* __rest
 // owns one column n2 and does every column-side step of a stage there, with the
 // column of each field and Raman line held in LDS between the steps:
 //   C1  inverse column DFT of the M fields -> U (HBM, column order, and LDS) -> pair
-//       products -> forward column DFT of the Raman lines -> conv (row order, for the
-//       Raman row pass, which writes them back in column order)
+//       products -> forward column DFT of the Raman lines -> conv (row order, or column
+//       order on large grids; the Raman row pass reads them so and writes column order)
 //   C2  inverse column DFT of the Raman lines (column order; kept in LDS) -> dense Kerr +
 //       Raman gather of the 64 samples, forming each slot mix X_s on the way -> forward
 //       column DFT of sigma -> yout
@@ -744,7 +761,8 @@ __global__ void __launch_bounds__(256) k_colstage_fields_pairs(Geometry g, const
                                                                const cpx<W>* __restrict__ fourtw,
                                                                const cpx<W>* __restrict__ yin, cpx<W>* __restrict__ U,
                                                                cpx<W>* __restrict__ conv, int n_lines,
-                                                               PairLineArgs pl, double scale, Batch b, int groups) {
+                                                               PairLineArgs pl, double scale, Batch b, int groups,
+                                                               int conv_cols) {
   using F = ColStageCfg;
   using cw = cpx<W>;
   constexpr int L = F::L;
@@ -835,9 +853,15 @@ __global__ void __launch_bounds__(256) k_colstage_fields_pairs(Geometry g, const
     }
     line_fft<L, -1, 1, 1, W, true>(v, fsm, lt, t);
     if (valid) {
-      const size_t base = (size_t)q * g.N + n2;
+      if (conv_cols) {  // column order (uniform branch)
+        const size_t base = (size_t)q * g.N + (size_t)n2 * L;
 #pragma unroll
-      for (int i = 0; i < 8; i++) conv[base + (size_t)(t + i * F::T) * g.N2] = cmul(v[i], w4[i]);
+        for (int i = 0; i < 8; i++) conv[base + t + i * F::T] = cmul(v[i], w4[i]);
+      } else {
+        const size_t base = (size_t)q * g.N + n2;
+#pragma unroll
+        for (int i = 0; i < 8; i++) conv[base + (size_t)(t + i * F::T) * g.N2] = cmul(v[i], w4[i]);
+      }
     }
   }
 }
@@ -1320,18 +1344,18 @@ hipError_t launch_row_raman(const Geometry& g, const Twiddles& tw, cpx<W>* data,
 
 template <int L, class W>
 static hipError_t row_raman_cols_impl(const Geometry& g, const Twiddles& tw, const cpx<W>* in, cpx<W>* out, int lines,
-                                      const cpx<W>* hspec, const Batch& b, int F, hipStream_t s) {
+                                      const cpx<W>* hspec, const Batch& b, int F, int in_cols, hipStream_t s) {
   using C = RamanColsCfg<L, W>;
   if (g.N1 % C::R) return hipErrorInvalidValue;
   MMP_LAUNCH((k_row_raman_cols<L, W>), dim3(lines * g.N1 / C::R, F), dim3(C::THREADS), 0, s, g, TwOf<W>::row(tw), in,
-             out, hspec, b);
+             out, hspec, b, in_cols);
   return hipGetLastError();
 }
 template <class W>
 hipError_t launch_row_raman_cols(const Geometry& g, const Twiddles& tw, const cpx<W>* in, cpx<W>* out, int lines,
-                                 const cpx<W>* hspec, const Batch& b, int F, hipStream_t s) {
+                                 const cpx<W>* hspec, const Batch& b, int F, int in_cols, hipStream_t s) {
   if (F < 1 || F > 65535 || lines < 1 || in == out) return hipErrorInvalidValue;
-  MMP_DISPATCH_LW(g.N2, row_raman_cols_impl, W, g, tw, in, out, lines, hspec, b, F, s);
+  MMP_DISPATCH_LW(g.N2, row_raman_cols_impl, W, g, tw, in, out, lines, hspec, b, F, in_cols, s);
 }
 
 template <class W>
@@ -1369,7 +1393,7 @@ void colstage_pairs_plan(int M, int N2, int n_lines, int F, bool mixed, int& col
 template <class W>
 hipError_t launch_colstage_fields_pairs(const Geometry& g, const Twiddles& tw, const cpx<W>* yin, cpx<W>* U,
                                         cpx<W>* conv, int n_lines, const PairLineArgs& pl, double scale,
-                                        const Batch& b, int F, int columns, int groups, hipStream_t s) {
+                                        const Batch& b, int F, int columns, int groups, int conv_cols, hipStream_t s) {
   if (g.N1 != kColStageL || g.M < 1 || g.M > kColStageMaxM || F < 1 || F > 65535 || n_lines < 1) return hipErrorInvalidValue;
   if (columns < 1 || g.N2 % columns || groups < 1 || groups > n_lines) return hipErrorInvalidValue;
   const size_t lds = colstage_pairs_lds_bytes(g.M, columns, std::is_same<W, float>::value);
@@ -1377,11 +1401,11 @@ hipError_t launch_colstage_fields_pairs(const Geometry& g, const Twiddles& tw, c
   const dim3 grid((unsigned)(g.N2 / columns * groups), F);
   switch (columns) {
     case 1: MMP_LAUNCH((k_colstage_fields_pairs<W, 1>), grid, dim3(256), lds, s, g, TwOf<W>::col(tw),
-                       TwOf<W>::four(tw), yin, U, conv, n_lines, pl, scale, b, groups); break;
+                       TwOf<W>::four(tw), yin, U, conv, n_lines, pl, scale, b, groups, conv_cols); break;
     case 2: MMP_LAUNCH((k_colstage_fields_pairs<W, 2>), grid, dim3(256), lds, s, g, TwOf<W>::col(tw),
-                       TwOf<W>::four(tw), yin, U, conv, n_lines, pl, scale, b, groups); break;
+                       TwOf<W>::four(tw), yin, U, conv, n_lines, pl, scale, b, groups, conv_cols); break;
     case 4: MMP_LAUNCH((k_colstage_fields_pairs<W, 4>), grid, dim3(256), lds, s, g, TwOf<W>::col(tw),
-                       TwOf<W>::four(tw), yin, U, conv, n_lines, pl, scale, b, groups); break;
+                       TwOf<W>::four(tw), yin, U, conv, n_lines, pl, scale, b, groups, conv_cols); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -1483,10 +1507,10 @@ hipError_t launch_energy(const Geometry& g, const cd* fields, double* out, doubl
   template hipError_t launch_gather_dense<W>(const Geometry&, const cpx<W>*, const cpx<W>*, cpx<W>*,               \
                                              const GatherArgs&, const W*, hipStream_t);                            \
   template hipError_t launch_row_raman_cols<W>(const Geometry&, const Twiddles&, const cpx<W>*, cpx<W>*, int,         \
-                                               const cpx<W>*, const Batch&, int, hipStream_t);                      \
+                                               const cpx<W>*, const Batch&, int, int, hipStream_t);                 \
   template hipError_t launch_colstage_fields_pairs<W>(const Geometry&, const Twiddles&, const cpx<W>*, cpx<W>*,     \
                                                       cpx<W>*, int, const PairLineArgs&, double, const Batch&, int, \
-                                                      int, int, hipStream_t);                                      \
+                                                      int, int, int, hipStream_t);                                 \
   template hipError_t launch_colstage_raman_gather<W>(const Geometry&, const Twiddles&, const cpx<W>*,             \
                                                       const cpx<W>*, cpx<W>*, const GatherArgs&, const FusedArgs&, \
                                                       const W*, double, const Batch&, int, hipStream_t);

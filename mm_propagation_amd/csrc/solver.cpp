@@ -108,6 +108,7 @@ struct mmp_solver {
   bool tuned = false;        // MMP_FLAG_AUTOTUNE chose fused.waves and chunk_lines
   int pair_columns = 1;      // column-stage pair pass: columns per workgroup and line groups
   int pair_groups = 1;
+  int conv_cols = 0;         // the pair pass stores the Raman lines in column order (large grids)
   bool chain = false;        // the step close also issues the next step's stage-0 head
   double* qdense = nullptr;  // [P][M][M] (or parity-compact) dense coupling blocks for the register gathers
   // work arrays of the step in the work precision (the FP64 buffers above in FP64 mode)
@@ -270,10 +271,10 @@ mmp_status enqueue_step_w(mmp_solver* s) {
     if (s->use_fused) {
       MMP_TRY(run_kernel(s, "col_fields_pairs", cb * (2 * MN + LN) * F, [&] {
         return launch_colstage_fields_pairs<W>(g, s->tw, ybuf, U, conv, s->n_lines, s->pairs, invN, s->batch, F,
-                                               s->pair_columns, s->pair_groups, st);
+                                               s->pair_columns, s->pair_groups, s->conv_cols, st);
       }));
       MMP_TRY(run_kernel(s, "row_raman", cb * (2 * LN + P.N) * F, [&] {
-        return launch_row_raman_cols<W>(g, s->tw, conv, convt, s->n_lines, hspec, s->batch, F, st);
+        return launch_row_raman_cols<W>(g, s->tw, conv, convt, s->n_lines, hspec, s->batch, F, s->conv_cols, st);
       }));
       MMP_TRY(run_kernel(s, "col_raman_gather", cb * (2 * MN + LN) * F, [&] {
         return launch_colstage_raman_gather<W>(g, s->tw, U, convt, ybuf, s->gather, s->fused, qdense, invN, s->batch,
@@ -758,6 +759,10 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
     colstage_pairs_plan((int)P.M, s->g.N2, s->n_lines, (int)s->F, s->mixed, s->pair_columns, s->pair_groups);
     // grids of 8 or more workgroups per CU
     s->fused.hi_occupancy = colstage_hi_occupancy_ok((int)P.M, s->mixed, compact) && (long)s->g.N2 * s->F >= 2048;
+    // lines in column order before the Raman row pass too: its LDS staging costs latency that
+    // only large grids hide (measured: batched M = 10, NT = 16384 x 36 fibers row pass 140 ->
+    // 111 us; one grin6 fiber 7.0 -> 7.6 us)
+    s->conv_cols = (long)s->g.N2 * s->F >= 2048 ? 1 : 0;
   }
   // stage heads: tableau coefficients here, each fiber's h, sgn and tables in the kernel
   // the close head also builds the next step's stage 0 unless the stage-0 Lawson table is

@@ -52,8 +52,9 @@ def test_batch_equals_single_solves(M, nt, precision):
 @pytest.mark.parametrize("M", [7, 10])
 def test_large_grid_variants_equal_single_solves(M):
     """16 fibers at NT = 8192 make 2048 gather workgroups: the batch runs the three-workgroups-
-    per-CU gather and a four-column pair pass with one line group, a fiber alone the two-per-CU
-    gather and four or eight line groups. The arithmetic is the same, so the results are too."""
+    per-CU gather, a four-column pair pass with one line group and the Raman lines in column
+    order before their row pass; a fiber alone the two-per-CU gather, four or eight line groups
+    and the lines in row order. The arithmetic is the same, so the results are too."""
     require_device()
     F = 16
     sps = fibers(M, 8192, F, base=40)
