@@ -41,7 +41,7 @@ def test_two_mode_mixed_short(raman):
     assert np.all(err <= MIXED_SHORT), err
 
 
-@pytest.mark.parametrize("M,nt", [(3, 4096), (6, 16384), (10, 8192)])
+@pytest.mark.parametrize("M,nt", [(3, 4096), (6, 16384), (10, 8192), (4, 32768), (3, 65536)])
 def test_grin_mixed_short(M, nt):
     require_device()
     sp = configs.grin_fiber(M=M, nt=nt, symmetric=True)
