@@ -483,9 +483,11 @@ __global__ void __launch_bounds__((RamanColsCfg<L, W>::THREADS)) k_row_raman_col
   const double hscale = b.fp[fib].dt;
   const int tid = threadIdx.x;
   const int r = tid / C::T, t = tid % C::T;
-  // R divides N1: the rows share one line; neighbouring row groups run on one XCD, so the
-  // halves of a cache line that two of them store meet in one L2
-  const int row0 = xcd_block(blockIdx.x, gridDim.x) * C::R;
+  // R divides N1: the rows share one line. In FP32 neighbouring row groups run on one XCD, so
+  // the parts of a cache line that several of them store meet in one L2 (measured: rows of 1024
+  // 23.7 -> 21.7 us); FP64 runs of 8 rows already fill 128-byte lines, and there the plain
+  // order measured slightly faster (rows of 256: 5.09 vs 5.21 us)
+  const int row0 = (std::is_same<W, float>::value ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x) * C::R;
   const int line = row0 / g.N1, n10 = row0 % g.N1, n1 = n10 + r;
   const size_t cbase = (size_t)line * g.N + n10;  // the R rows' first sample in column order
   const size_t hbase = (size_t)n1 * g.N2;
