@@ -153,7 +153,8 @@ def main() -> int:
              (_capi.MMP_FLAG_SEPARATE_COLUMNS if args.separate_columns else 0) |
              (_capi.MMP_FLAG_AUTOTUNE if args.autotune else 0))
     s = MultimodePropagationGPU_RamanON(sp, pair_decode=decode, flags=flags, ctx=ctx, precision=prec)
-    tuned = {k: s.info()[k] for k in ("gather_waves", "chunk_lines", "tuned")}
+    tuned = {k: s.info()[k] for k in ("gather_waves", "chunk_lines", "tuned", "pair_columns", "pair_groups",
+                                      "gather_hi_occupancy")}
     psi0, _ = s.get_state()
     if args.warmup:
         s.advance(args.warmup)
