@@ -59,6 +59,20 @@ __device__ __forceinline__ int xcd_block(int bid, int nb) {
   return (bid % 8) * (nb / 8) + bid / 8;
 }
 
+// Streaming store for arrays the next kernel reads (it runs on other XCDs, whose L2s do not
+// share lines with this one).
+template <class T>
+__device__ __forceinline__ void store_stream(cpx<T>* p, cpx<T> v) {
+  __builtin_nontemporal_store(v.re, &p->re);
+  __builtin_nontemporal_store(v.im, &p->im);
+}
+// The same in FP64 only: FP32 stores of the column kernels measured slower streamed.
+template <class T>
+__device__ __forceinline__ void store_stream64(cpx<T>* p, cpx<T> v) {
+  if constexpr (std::is_same<T, double>::value) store_stream(p, v);
+  else *p = v;
+}
+
 __device__ __forceinline__ cd phase(double imL, double tau) {
   double s, c;
   sincos(imL * tau, &s, &c);
@@ -141,7 +155,7 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_row_head(Geometry g, con
   // the Lawson table of stage i-1 moves on by E1 (it is not read again in this step)
   if (a.has_prev && a.update_prev && valid) {
 #pragma unroll
-    for (int i = 0; i < 8; i++) e_prev[base + i * C::T] = cmul_exact(ep[i], e1[i]);
+    for (int i = 0; i < 8; i++) store_stream(e_prev + base + i * C::T, cmul_exact(ep[i], e1[i]));
   }
   if (a.has_prev) {
     const double bh = hc[0];
@@ -155,9 +169,9 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_row_head(Geometry g, con
       cd acc = accb[i];
       if (a.b_prev != 0.) acc = cadd(acc, cscale(nh, bh));
       if (valid) {
-        if (a.final_stage) psi_hat[idx] = acc;
-        else accp[idx] = acc;
-        if (a.stash_prev) stash[(size_t)a.prev_stage * MN + idx] = nh;
+        if (a.final_stage) store_stream(psi_hat + idx, acc);
+        else store_stream(accp + idx, acc);
+        if (a.stash_prev) store_stream(stash + (size_t)a.prev_stage * MN + idx, nh);
       }
       if (a.final_stage) x[i] = acc;  // chain_next: stage 0 of the next step from the new psi_hat
       else if (prev_term) x[i] = cadd(x[i], cscale(nh, cprev));
@@ -169,7 +183,7 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_row_head(Geometry g, con
   line_fft<L, 1, 1, 1>(v, sm, lt, t);
   if (valid) {
 #pragma unroll
-    for (int i = 0; i < 8; i++) yout[base + i * C::T] = v[i];
+    for (int i = 0; i < 8; i++) store_stream(yout + base + i * C::T, v[i]);
   }
 }
 
@@ -528,7 +542,8 @@ __global__ void __launch_bounds__((RamanColsCfg<L, W>::THREADS)) k_row_raman_col
   for (int k = 0; k < 8; k++) {
     const int e = tid + k * C::THREADS;  // R * L = 8 * THREADS staged samples
     const int rr = e % C::R, n2 = e / C::R;
-    out[cbase + (size_t)n2 * g.N1 + rr] = smem[n2 * (C::R + 1) + rr];
+    if constexpr (C::R * sizeof(cw) >= 128) store_stream(out + cbase + (size_t)n2 * g.N1 + rr, smem[n2 * (C::R + 1) + rr]);
+    else out[cbase + (size_t)n2 * g.N1 + rr] = smem[n2 * (C::R + 1) + rr];
   }
 }
 
@@ -825,7 +840,7 @@ __global__ void __launch_bounds__(256) k_colstage_fields_pairs(Geometry g, const
       for (int i = 0; i < 8; i++) {
         const int n1 = t + i * F::T;
         const cw x = cscale(v[i], sc);
-        if (grp == 0) U[(size_t)q * g.N + (size_t)n2 * L + n1] = x;  // column order, for the gather
+        if (grp == 0) store_stream64(U + (size_t)q * g.N + (size_t)n2 * L + n1, x);  // column order, for the gather
         Ush[(q * L + n1) * CW + c] = x;
       }
     }
@@ -858,11 +873,11 @@ __global__ void __launch_bounds__(256) k_colstage_fields_pairs(Geometry g, const
       if (conv_cols) {  // column order (uniform branch)
         const size_t base = (size_t)q * g.N + (size_t)n2 * L;
 #pragma unroll
-        for (int i = 0; i < 8; i++) conv[base + t + i * F::T] = cmul(v[i], w4[i]);
+        for (int i = 0; i < 8; i++) store_stream64(conv + base + t + i * F::T, cmul(v[i], w4[i]));
       } else {
         const size_t base = (size_t)q * g.N + n2;
 #pragma unroll
-        for (int i = 0; i < 8; i++) conv[base + (size_t)(t + i * F::T) * g.N2] = cmul(v[i], w4[i]);
+        for (int i = 0; i < 8; i++) store_stream64(conv + base + (size_t)(t + i * F::T) * g.N2, cmul(v[i], w4[i]));
       }
     }
   }
@@ -1080,7 +1095,7 @@ __global__ void __launch_bounds__(NW * 64, HI ? 3 : 1) k_colstage_raman_gather(G
     if (valid) {
       const size_t base = (size_t)q * g.N + n2;
 #pragma unroll
-      for (int i = 0; i < 8; i++) yout[base + (size_t)(t + i * T) * g.N2] = cmul(v[i], w4[i]);
+      for (int i = 0; i < 8; i++) store_stream64(yout + base + (size_t)(t + i * T) * g.N2, cmul(v[i], w4[i]));
     }
   }
 }
