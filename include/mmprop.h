@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MMPROP_ABI_VERSION 4
+#define MMPROP_ABI_VERSION 5
 
 /* Only the entry points below are exported; the library is built with hidden visibility. */
 #if defined(__GNUC__) || defined(__clang__)
@@ -263,6 +263,19 @@ MMP_API mmp_status mmp_set_state(mmp_solver* solver, const double* psi);
  * in W*ps, computed on the device in FP64. energy_out is [M]. */
 MMP_API mmp_status mmp_energy(mmp_solver* solver, double* energy_out);
 
+/* Per-mode photon number of the physical field (no reference counterpart; the reference writes
+ * only fields, matlab/mm_propagation.cpp:72-75, simulations/main.cpp:101-103):
+ * N_p = sum_k E_p,k / (hbar * omega_k), with E_p,k = (T_w / NT^2) |(F A_p)_k|^2 the energy of
+ * frequency bin k (W ps; F the unnormalised forward DFT) and omega_k = omega0 - 2 pi k / T_w its
+ * absolute angular frequency, k in (-NT/2, NT/2] as the reference's frequency grid
+ * (src/MultimodePropagation.cpp:169-175) and omega0 = 2 pi c / wavelength. The carrier wavelength
+ * (metres) is an argument because SimulationParameters has none. Computed on the device in FP64
+ * from the spectral state. photons_out is [M]. MMP_ERR_INVALID_ARGUMENT when wavelength is not
+ * positive or the grid reaches omega_k <= 0 (omega0 <= pi NT / T_w). */
+MMP_API mmp_status mmp_photon_number(mmp_solver* solver, double wavelength, double* photons_out);
+MMP_API mmp_status mmp_fiber_photon_number(mmp_solver* solver, uint32_t fiber, double wavelength,
+                                           double* photons_out);
+
 /* Solver scalars and state. */
 MMP_API mmp_status mmp_solver_info(mmp_solver* solver, mmp_info* info_out);
 
@@ -294,8 +307,10 @@ MMP_API mmp_status mmp_ctx_query(mmp_ctx* ctx);
  * solver of its own. Every group and solo solver has its own stream, so their work overlaps;
  * a group advances by the fewest steps any of its live requests still needs (at most
  * max_chunk), finished requests are read back and their slots parked. Each result is
- * bit-identical to the same solve alone with the same options. One host thread drives a
- * server; mmp_serve_poll never blocks.
+ * bit-identical to the same solve alone with the same options, unless the flags hold
+ * MMP_FLAG_AUTOTUNE: a group then runs the gather variant tuned for its slot count, which may
+ * sum over the slots in another order than the solve alone, and results agree to rounding
+ * (about 1e-15 relative in FP64). One host thread drives a server; mmp_serve_poll never blocks.
  * ---------------------------------------------------------------------------------------- */
 typedef struct mmp_server mmp_server;
 
@@ -305,7 +320,8 @@ typedef struct mmp_serve_options {
   uint32_t max_chunk;    /* steps a group or solver takes between scheduling points, >= 1 */
   uint32_t precision;    /* MMP_PRECISION_* for every solver */
   uint32_t pair_decode;  /* MMP_PAIRS_* */
-  uint32_t flags;        /* MMP_FLAG_* for every solver (MMP_FLAG_AUTOTUNE tunes each structure once) */
+  uint32_t flags;        /* MMP_FLAG_* for every solver (MMP_FLAG_AUTOTUNE tunes each structure once;
+                            results are then rounding-equal, not bit-identical, to solves alone) */
   double hbm_budget;     /* bytes of device memory the server's solvers may hold */
 } mmp_serve_options;
 

@@ -32,7 +32,7 @@ typedef SimulationParameters InputParameters;
 
 class MultimodePropagation {
  public:
-  virtual ~MultimodePropagation() {}
+  virtual ~MultimodePropagation();
 
   // Restart at z = 0 (fresh Lawson operators, current field kept) and take nz steps.
   virtual void computeLawsonRK(const unsigned int nz) = 0;

@@ -14,7 +14,7 @@ struct mmp_server;
 
 struct MultimodePropagationBatchOptions {
   unsigned int precision = 0;    // MMP_PRECISION_FP64 (0) or MMP_PRECISION_MIXED (1)
-  unsigned int pair_decode = 0;  // MMP_PAIRS_REFERENCE_CPU (0) or MMP_PAIRS_PHYSICAL (1)
+  unsigned int pair_decode = 1;  // MMP_PAIRS_PHYSICAL (1), as MultimodePropagationGPU_*, or MMP_PAIRS_REFERENCE_CPU (0)
   unsigned int flags = 0;        // MMP_FLAG_* (e.g. MMP_FLAG_AUTOTUNE)
   int device = 0;
 };

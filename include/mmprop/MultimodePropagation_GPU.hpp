@@ -24,9 +24,13 @@ class MmpropError : public std::runtime_error {
   int status_;
 };
 
-// Choices with no counterpart in the reference (defaults reproduce its CPU behaviour).
+// Choices with no counterpart in the reference. The default pair decode is the one of the class
+// these replace: MultimodePropagationGPU_* reads slot j's modes from the stored pair code
+// (l = pair % M, m = pair / M; src/MultimodePropagation_GPU.cu:188-190,278-280). The reference
+// CPU class's counter decode (j % M, j / M; src/MultimodePropagation_CPU.cpp:126-133) is
+// pair_decode = 0.
 struct MultimodePropagationGPUOptions {
-  unsigned int pair_decode = 0;  // MMP_PAIRS_REFERENCE_CPU (0) or MMP_PAIRS_PHYSICAL (1)
+  unsigned int pair_decode = 1;  // MMP_PAIRS_PHYSICAL (1) or MMP_PAIRS_REFERENCE_CPU (0)
   unsigned int flags = 0;        // MMP_FLAG_* of include/mmprop.h
   int device = 0;
 };
@@ -41,9 +45,11 @@ class MultimodePropagationGPU : public MultimodePropagation {
   void computeLawsonRK() override;
   ComplexArraysContainerCPU getResult() override;
 
-  // Extensions: resumable stepping, per-mode energy (W ps), current z.
+  // Extensions: resumable stepping, per-mode energy (W ps), per-mode photon number for a carrier
+  // wavelength in metres, current z.
   void advance(const unsigned int n);
   std::vector<double> getEnergy();
+  std::vector<double> getPhotonNumber(double wavelength);
   double getZ();
 
   // The MEX gateway's n_save loop (matlab/mm_propagation.cpp:61-101): n_steps in n_save chunks

@@ -154,6 +154,8 @@ EXPORTS = [
     ("mmp_get_state", c_int, [c_void_p, POINTER(c_double), POINTER(c_double)]),
     ("mmp_set_state", c_int, [c_void_p, POINTER(c_double)]),
     ("mmp_energy", c_int, [c_void_p, POINTER(c_double)]),
+    ("mmp_photon_number", c_int, [c_void_p, c_double, POINTER(c_double)]),
+    ("mmp_fiber_photon_number", c_int, [c_void_p, c_uint32, c_double, POINTER(c_double)]),
     ("mmp_solver_info", c_int, [c_void_p, POINTER(MmpInfo)]),
     ("mmp_kernel_times", c_int, [c_void_p, POINTER(MmpKernelTime), c_uint32, POINTER(c_uint32)]),
     ("mmp_ctx_synchronize", c_int, [c_void_p]),

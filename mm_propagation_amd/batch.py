@@ -26,7 +26,7 @@ def _dp(a: np.ndarray):
 
 class MultimodePropagationBatch:
     def __init__(self, params: Sequence[SimulationParameters], raman: bool = True,
-                 pair_decode: int = _capi.MMP_PAIRS_REFERENCE_CPU, flags: int = 0,
+                 pair_decode: int = _capi.MMP_PAIRS_PHYSICAL, flags: int = 0,
                  precision: int = _capi.MMP_PRECISION_FP64, ctx: Optional[Context] = None):
         if not params:
             raise ValueError("a batch needs at least one fiber")
@@ -76,6 +76,13 @@ class MultimodePropagationBatch:
     def energy(self, f: int) -> np.ndarray:
         out = np.empty(self.M, dtype=np.float64)
         _capi.check(self.lib.mmp_fiber_energy(self.h, int(f), _dp(out)), "mmp_fiber_energy")
+        return out
+
+    def photon_number(self, f: int, wavelength: float) -> np.ndarray:
+        """Fiber f's per-mode photon number for a carrier wavelength in metres."""
+        out = np.empty(self.M, dtype=np.float64)
+        _capi.check(self.lib.mmp_fiber_photon_number(self.h, int(f), float(wavelength), _dp(out)),
+                    "mmp_fiber_photon_number")
         return out
 
     def info(self, f: int = 0) -> Dict[str, float]:

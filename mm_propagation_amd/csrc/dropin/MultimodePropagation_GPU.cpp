@@ -58,6 +58,12 @@ std::vector<double> MultimodePropagationGPU::getEnergy() {
   return e;
 }
 
+std::vector<double> MultimodePropagationGPU::getPhotonNumber(double wavelength) {
+  std::vector<double> n(M_);
+  check(mmp_photon_number(solver_, wavelength, n.data()), "mmp_photon_number");
+  return n;
+}
+
 std::vector<MultimodePropagationGPU::Snapshot> MultimodePropagationGPU::propagateWithSnapshots(
     unsigned int n_steps, unsigned int n_save, bool resumable) {
   const size_t per = 2 * (size_t)M_ * nt_;

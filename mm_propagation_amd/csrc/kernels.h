@@ -237,5 +237,9 @@ hipError_t launch_gather_dense(const Geometry& g, const cpx<W>* U, const cpx<W>*
 // out[j] = in[j] rounded to FP32, j < n
 hipError_t launch_narrow(const cd* in, cf* out, size_t n, hipStream_t s);
 hipError_t launch_energy(const Geometry& g, const cd* fields, double* out, double dt, hipStream_t s);
+// out[p] = scale * sum_k |spectra_p[k]|^2 / (omega0 - dalpha * k_signed) over the M spectra in
+// four-step order (k_signed in (-N/2, N/2], the reference's frequency grid).
+hipError_t launch_photon_number(const Geometry& g, const cd* spectra, double* out, double omega0, double dalpha,
+                                double scale, hipStream_t s);
 
 }  // namespace mmp

@@ -1189,6 +1189,31 @@ __global__ void __launch_bounds__(256) k_energy(Geometry g, const cd* __restrict
   if (threadIdx.x == 0) out[p] = red[0] * dt;
 }
 
+// Per-mode photon number of a spectral state in four-step order (S[k1*N2 + k2] holds bin
+// k = k1 + N1*k2): out[p] = scale * sum_k |S_p[k]|^2 / (omega0 - dalpha * k_signed), k_signed = k
+// for k <= N/2, else k - N (the reference's frequency grid, src/MultimodePropagation.cpp:171-175).
+// One workgroup per mode, fixed reduction order; FP64 throughout.
+__global__ void __launch_bounds__(256) k_photon_number(Geometry g, const cd* __restrict__ S, double* __restrict__ out,
+                                                       double omega0, double dalpha, double scale) {
+  __shared__ double red[256];
+  const int p = blockIdx.x;
+  double s = 0.;
+  for (int idx = threadIdx.x; idx < g.N; idx += 256) {
+    const int k1 = idx / g.N2, k2 = idx - k1 * g.N2;
+    const int k = k1 + g.N1 * k2;
+    const int ks = k <= g.N / 2 ? k : k - g.N;
+    const cd x = S[(size_t)p * g.N + idx];
+    s += (x.re * x.re + x.im * x.im) / (omega0 - dalpha * (double)ks);
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[p] = red[0] * scale;
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -1505,6 +1530,13 @@ hipError_t launch_narrow(const cd* in, cf* out, size_t n, hipStream_t s) {
 
 hipError_t launch_energy(const Geometry& g, const cd* fields, double* out, double dt, hipStream_t s) {
   MMP_LAUNCH(k_energy, dim3(g.M), dim3(256), 0, s, g, fields, out, dt);
+  return hipGetLastError();
+}
+
+hipError_t launch_photon_number(const Geometry& g, const cd* spectra, double* out, double omega0, double dalpha,
+                                double scale, hipStream_t s) {
+  if (g.N1 * g.N2 != g.N || g.M < 1) return hipErrorInvalidValue;
+  MMP_LAUNCH(k_photon_number, dim3(g.M), dim3(256), 0, s, g, spectra, out, omega0, dalpha, scale);
   return hipGetLastError();
 }
 

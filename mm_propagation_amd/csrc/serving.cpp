@@ -9,7 +9,8 @@
 // Other requests get a solver of their own. Every group and solo solver has its own context
 // (stream). A group advances by the fewest steps any of its live requests still needs, capped
 // at max_chunk, so no request takes a step too many; a request's result is therefore the
-// result of the same solve alone.
+// result of the same solve alone (bit for bit; to rounding with MMP_FLAG_AUTOTUNE, whose gather
+// variant for a group's slot count may sum over the pair slots in another order).
 
 #include <algorithm>
 #include <chrono>

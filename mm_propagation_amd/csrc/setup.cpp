@@ -144,7 +144,7 @@ mmp_status prepare_problem(const mmp_params* in, const mmp_options& opt, Problem
   P.mixed = opt.precision == MMP_PRECISION_MIXED;
   if (opt.precision != MMP_PRECISION_FP64 && opt.precision != MMP_PRECISION_MIXED) {
     err = "unknown precision";
-    return MMP_ERR_UNSUPPORTED;
+    return MMP_ERR_INVALID_ARGUMENT;
   }
   if (opt.pair_decode != MMP_PAIRS_REFERENCE_CPU && opt.pair_decode != MMP_PAIRS_PHYSICAL) {
     err = "unknown pair_decode";

@@ -124,8 +124,10 @@ struct mmp_solver {
   std::vector<uint32_t> steps;  // per fiber, since its last restart
   std::vector<uint8_t> live;    // per fiber: advanced by the steps (1) or parked (0)
   bool graphs = true;
-  hipGraphExec_t graph_exec = nullptr;
+  hipGraphExec_t graph_exec = nullptr;   // kGraphSteps steps
   hipGraph_t graph = nullptr;
+  hipGraphExec_t graph1_exec = nullptr;  // one step (the remainder of a call)
+  hipGraph_t graph1 = nullptr;
   // profiling
   bool profile = false;
   std::vector<std::string> kernel_names;
@@ -306,11 +308,11 @@ mmp_status enqueue_step_w(mmp_solver* s) {
 
 mmp_status enqueue_step(mmp_solver* s) { return s->mixed ? enqueue_step_w<float>(s) : enqueue_step_w<double>(s); }
 
-mmp_status build_graph(mmp_solver* s) {
+mmp_status capture_steps(mmp_solver* s, int steps, hipGraph_t* graph_out, hipGraphExec_t* exec_out) {
   hipStream_t st = s->ctx->stream;
   HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
   mmp_status status = MMP_OK;
-  for (int k = 0; k < kGraphSteps && status == MMP_OK; k++) status = enqueue_step(s);
+  for (int k = 0; k < steps && status == MMP_OK; k++) status = enqueue_step(s);
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(st, &graph);
   if (status != MMP_OK) {
@@ -324,8 +326,41 @@ mmp_status build_graph(mmp_solver* s) {
     hipGraphDestroy(graph);
     return hip_fail(e, "hipGraphInstantiate");
   }
-  s->graph = graph;
-  s->graph_exec = exec;
+  // upload once here, so that the first replay does not carry it
+  e = hipGraphUpload(exec, st);
+  if (e != hipSuccess) {
+    hipGraphExecDestroy(exec);
+    hipGraphDestroy(graph);
+    return hip_fail(e, "hipGraphUpload");
+  }
+  *graph_out = graph;
+  *exec_out = exec;
+  return MMP_OK;
+}
+
+void drop_graph(mmp_solver* s) {
+  if (s->graph_exec) hipGraphExecDestroy(s->graph_exec);
+  if (s->graph) hipGraphDestroy(s->graph);
+  if (s->graph1_exec) hipGraphExecDestroy(s->graph1_exec);
+  if (s->graph1) hipGraphDestroy(s->graph1);
+  s->graph_exec = nullptr;
+  s->graph = nullptr;
+  s->graph1_exec = nullptr;
+  s->graph1 = nullptr;
+}
+
+// The replayable graphs of kGraphSteps steps and of one step. Built when the solver is created
+// (and again after the autotuner changes the kernels' plan), so no call that steps pays for them.
+// When capture is unavailable the solver keeps launching kernel by kernel.
+mmp_status build_graph(mmp_solver* s) {
+  if (!s->graphs || s->graph_exec) return MMP_OK;
+  mmp_status st = capture_steps(s, kGraphSteps, &s->graph, &s->graph_exec);
+  if (st == MMP_OK) st = capture_steps(s, 1, &s->graph1, &s->graph1_exec);
+  if (st != MMP_OK) {
+    drop_graph(s);
+    s->graphs = false;
+    hipGetLastError();
+  }
   return MMP_OK;
 }
 
@@ -392,8 +427,7 @@ void release(mmp_solver* s) {
     hipEventDestroy(r.stop);
   }
   for (auto e : s->free_events) hipEventDestroy(e);
-  if (s->graph_exec) hipGraphExecDestroy(s->graph_exec);
-  if (s->graph) hipGraphDestroy(s->graph);
+  drop_graph(s);
   for (void* p : s->allocations) hipFree(p);
   delete s;
 }
@@ -813,20 +847,14 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
   }
   for (uint32_t f = 0; f < F && st == MMP_OK; f++) A(load_fiber(s, f, s->fib[f]));
   if (st == MMP_OK && (opt.flags & MMP_FLAG_AUTOTUNE) && s->use_fused) A(autotune(s));
+  if (st == MMP_OK) A(build_graph(s));
   if (st != MMP_OK) return cleanup(st);
   *out = s;
   return MMP_OK;
 }
 
-void drop_graph(mmp_solver* s) {
-  if (s->graph_exec) hipGraphExecDestroy(s->graph_exec);
-  if (s->graph) hipGraphDestroy(s->graph);
-  s->graph_exec = nullptr;
-  s->graph = nullptr;
-}
-
 // Column-stage gather variant: `waves` wavefronts per workgroup, at most `lines` Raman lines per
-// LDS chunk. The next step rebuilds the graph.
+// LDS chunk. The graphs are captured again.
 mmp_status apply_colstage_plan(mmp_solver* s, int waves, int lines) {
   if (!s->use_fused) return fail(MMP_ERR_UNSUPPORTED, "no column-stage kernels in this solver");
   if (waves != 4 && !(waves == 8 && colstage_wide_ok((int)s->P.M, s->mixed)))
@@ -936,19 +964,11 @@ mmp_status advance_impl(mmp_solver* s, uint32_t n) {
   MMP_TRY(set_device(s));
   if (n > 0 && s->chain) MMP_TRY(enqueue_head(s, 0));  // stage 0 of the first step
   uint32_t left = n;
-  if (s->graphs && left >= (uint32_t)kGraphSteps) {
-    if (!s->graph_exec) {
-      mmp_status bs = build_graph(s);
-      if (bs != MMP_OK) {
-        // graphs unavailable: keep launching step by step
-        s->graphs = false;
-        hipGetLastError();
-      }
-    }
-    while (s->graphs && left >= (uint32_t)kGraphSteps) {
+  if (n > 0) MMP_TRY(build_graph(s));
+  if (s->graphs) {
+    for (; left >= (uint32_t)kGraphSteps; left -= kGraphSteps)
       HIP_TRY(hipGraphLaunch(s->graph_exec, s->ctx->stream), "hipGraphLaunch");
-      left -= kGraphSteps;
-    }
+    for (; left > 0; left--) HIP_TRY(hipGraphLaunch(s->graph1_exec, s->ctx->stream), "hipGraphLaunch");
   }
   for (uint32_t k = 0; k < left; k++) MMP_TRY(enqueue_step(s));
   for (uint32_t f = 0; f < s->F; f++) {
@@ -1196,6 +1216,45 @@ mmp_status mmp_fiber_energy(mmp_solver* s, uint32_t f, double* energy_out) {
   MMP_TRY(inverse_state_to_U(s, f, true));
   HIP_TRY(launch_energy(s->g, s->U, s->energy, s->time_window[f] / s->P.N, s->ctx->stream), "energy");
   HIP_TRY(hipMemcpyAsync(energy_out, s->energy, sizeof(double) * s->P.M, hipMemcpyDeviceToHost, s->ctx->stream),
+          "hipMemcpyAsync D2H");
+  HIP_TRY(hipStreamSynchronize(s->ctx->stream), "hipStreamSynchronize");
+  return MMP_OK;
+}
+
+mmp_status mmp_photon_number(mmp_solver* s, double wavelength, double* photons_out) {
+  return mmp_fiber_photon_number(s, 0, wavelength, photons_out);
+}
+
+// N_p = sum_k E_p,k / (hbar omega_k): E_p,k = (dt / NT) |F A_p|_k^2 (W ps) is bin k's share of the
+// energy (Parseval, unnormalised forward DFT) and omega_k = omega0 - alpha_k (rad/ps) its absolute
+// angular frequency, alpha_k = 2 pi k / T_w on the reference's grid and the carrier
+// exp(-i omega0 t) (the operator i d/dt of src/MultimodePropagation.cpp:166 maps bin alpha to
+// omega0 - alpha). |F A_p| = |psi_hat_p| / conversion_factor, the Lawson phase exp(L z) having
+// modulus one, so the device state is reduced directly.
+mmp_status mmp_fiber_photon_number(mmp_solver* s, uint32_t f, double wavelength, double* photons_out) {
+  MMP_TRY(check_fiber(s, f));
+  if (!photons_out) return fail(MMP_ERR_INVALID_ARGUMENT, "photons_out is required");
+  if (!(wavelength > 0.) || !std::isfinite(wavelength))
+    return fail(MMP_ERR_INVALID_ARGUMENT, "the carrier wavelength must be a positive number of metres");
+  const double c_light = 299792458.0;          // m/s
+  const double hbar = 1.054571817e-34;         // J s
+  const double two_pi = 6.283185307179586476925286766559;
+  const double tw = s->time_window[f];         // ps
+  const double omega0 = two_pi * c_light / wavelength * 1e-12;  // rad/ps
+  const double dalpha = two_pi / tw;                            // rad/ps per bin
+  if (!(omega0 - dalpha * (double)(s->P.N / 2) > 0.))
+    return fail(MMP_ERR_INVALID_ARGUMENT,
+                "the time grid reaches negative absolute frequencies: the carrier frequency must exceed "
+                "pi * nt / time_window");
+  const double conv = s->fib[f].conversion_factor;
+  const double dt = tw / s->P.N;  // ps
+  // W ps -> J (1e-12), rad/ps -> rad/s (1e12)
+  const double scale = dt / ((double)s->P.N * conv * conv) * 1e-24 / hbar;
+  MMP_TRY(set_device(s));
+  HIP_TRY(launch_photon_number(s->g, s->psi_hat + (size_t)f * s->batch.sf, s->energy, omega0, dalpha, scale,
+                               s->ctx->stream),
+          "photon_number");
+  HIP_TRY(hipMemcpyAsync(photons_out, s->energy, sizeof(double) * s->P.M, hipMemcpyDeviceToHost, s->ctx->stream),
           "hipMemcpyAsync D2H");
   HIP_TRY(hipStreamSynchronize(s->ctx->stream), "hipStreamSynchronize");
   return MMP_OK;

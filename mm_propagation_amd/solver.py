@@ -65,7 +65,7 @@ class MultimodePropagationGPU:
 
     raman = True
 
-    def __init__(self, params: SimulationParameters, pair_decode: int = _capi.MMP_PAIRS_REFERENCE_CPU,
+    def __init__(self, params: SimulationParameters, pair_decode: int = _capi.MMP_PAIRS_PHYSICAL,
                  flags: int = 0, ctx: Optional[Context] = None, precision: int = _capi.MMP_PRECISION_FP64):
         self.ctx = ctx or default_context()
         self.lib = self.ctx.lib
@@ -130,6 +130,14 @@ class MultimodePropagationGPU:
     def energy(self) -> np.ndarray:
         out = np.empty(self.M, dtype=np.float64)
         _capi.check(self.lib.mmp_energy(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "mmp_energy")
+        return out
+
+    def photon_number(self, wavelength: float) -> np.ndarray:
+        """Per-mode photon number of the physical field for a carrier wavelength in metres."""
+        out = np.empty(self.M, dtype=np.float64)
+        _capi.check(self.lib.mmp_photon_number(self.h, float(wavelength),
+                                               out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                    "mmp_photon_number")
         return out
 
     def info(self) -> Dict[str, float]:

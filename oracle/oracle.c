@@ -180,6 +180,51 @@ static void fft_inverse(const fft_plan* p, const cx* in, cx* out, int variant, c
   for (unsigned i = 0; i < p->n; i++) out[i] = cx_mul(out[i], f);
 }
 
+/* FP32 emulation (error-budget studies, ORC_FP32_*): every operation's result rounded to single
+ * precision. Products and sums of two FP32 values are exact in FP64, so rounding the FP64 result
+ * gives the FP32 operation's result. */
+static inline double f32(double x) { return (double)(float)x; }
+static inline cx cx_f32(cx a) { cx r = {f32(a.re), f32(a.im)}; return r; }
+static inline cx cx_mul_f32(cx a, cx b) {
+  cx r = {f32(f32(a.re * b.re) - f32(a.im * b.im)), f32(f32(a.re * b.im) + f32(a.im * b.re))};
+  return r;
+}
+static inline cx cx_add_f32(cx a, cx b) { cx r = {f32(a.re + b.re), f32(a.im + b.im)}; return r; }
+static inline cx cx_scale_f32(cx a, double d) { cx r = {f32(a.re * d), f32(a.im * d)}; return r; }
+
+/* The radix-2 decimation-in-time transform with FP32 arithmetic and twiddles rounded to FP32;
+ * the input is rounded to FP32 first. sign -1 forward unnormalised, +1 inverse with 1/n. */
+static void fft_f32(const fft_plan* p, const cx* in, cx* out, int sign, cx* scratch) {
+  const unsigned n = p->n;
+  if (!p->pow2) {  /* no FP32 variant of the O(n^2) transform: the FP64 one, rounded */
+    fft_raw(p, in, out, sign, 0, scratch);
+    for (unsigned i = 0; i < n; i++) out[i] = cx_f32(sign > 0 ? cx_scale(out[i], 1.0 / n) : out[i]);
+    return;
+  }
+  if (in == out) {
+    memcpy(scratch, in, sizeof(cx) * n);
+    in = scratch;
+  }
+  for (unsigned j = 0; j < n; j++) out[p->rev[j]] = cx_f32(in[j]);
+  for (unsigned half = 1; half < n; half <<= 1) {
+    unsigned stride = n / (2 * half);
+    for (unsigned base = 0; base < n; base += 2 * half) {
+      for (unsigned j = 0; j < half; j++) {
+        cx w = cx_f32(twiddle(p, j * stride, sign));
+        cx u = out[base + j];
+        cx v = cx_mul_f32(out[base + j + half], w);
+        out[base + j] = cx_add_f32(u, v);
+        out[base + j + half].re = f32(u.re - v.re);
+        out[base + j + half].im = f32(u.im - v.im);
+      }
+    }
+  }
+  if (sign > 0) {
+    const double f = f32(1.0 / n);
+    for (unsigned i = 0; i < n; i++) out[i] = cx_scale_f32(out[i], f);
+  }
+}
+
 void orc_dft(unsigned n, const double* in, double* out, int sign, int variant) {
   fft_plan p;
   if (!fft_plan_init(&p, n)) return;
@@ -531,6 +576,7 @@ static void nonlinearity(orc_solver* s, cx* N_i, const cx* E_i, const cx* psi_i)
   const unsigned M = s->M, nt = s->nt, P = s->P;
   const int var = s->opt.fft_variant;
   const int nth = s->nthreads;
+  const int f32m = s->opt.fp32_stages;
   cx* U = s->U;
   /* Step 1: U_p = F^-1[E_i,p * F psi_i,p] */
 #pragma omp parallel for num_threads(nth) schedule(static)
@@ -540,7 +586,8 @@ static void nonlinearity(orc_solver* s, cx* N_i, const cx* E_i, const cx* psi_i)
     fft_forward(&s->plan, psi_i + (size_t)p * nt, buf, var, scr);
     const cx* Ep = E_i + (size_t)p * nt;
     for (unsigned j = 0; j < nt; j++) buf[j] = cx_mul(buf[j], Ep[j]);
-    fft_inverse(&s->plan, buf, U + (size_t)p * nt, var, scr);
+    if (f32m & ORC_FP32_FIELDS) fft_f32(&s->plan, buf, U + (size_t)p * nt, +1, scr);
+    else fft_inverse(&s->plan, buf, U + (size_t)p * nt, var, scr);
   }
   /* Step 2: V_j = U_a * conj(U_b) for slot j; RamanON also V_R,j = dt * F^-1[hR_fft * F V_j] */
 #pragma omp parallel for num_threads(s->opt.parallel_pairs ? nth : 1) schedule(static)
@@ -548,15 +595,26 @@ static void nonlinearity(orc_solver* s, cx* N_i, const cx* E_i, const cx* psi_i)
     const cx* Ua = U + (size_t)s->pair_a[j] * nt;
     const cx* Ub = U + (size_t)s->pair_b[j] * nt;
     cx* Vj = s->V + (size_t)j * nt;
-    for (unsigned t = 0; t < nt; t++) Vj[t] = cx_mul(Ua[t], cx_conj(Ub[t]));
+    if (f32m & ORC_FP32_PAIRS)
+      for (unsigned t = 0; t < nt; t++) Vj[t] = cx_mul_f32(cx_f32(Ua[t]), cx_conj(cx_f32(Ub[t])));
+    else
+      for (unsigned t = 0; t < nt; t++) Vj[t] = cx_mul(Ua[t], cx_conj(Ub[t]));
     if (s->raman) {
       cx* buf = s->tmp + (size_t)thread_id() * 3 * nt;
       cx* scr = buf + nt;
       cx* VRj = s->VR + (size_t)j * nt;
-      fft_forward(&s->plan, Vj, buf, var, scr);
-      for (unsigned t = 0; t < nt; t++) buf[t] = cx_mul(buf[t], s->hR_fft[t]);
-      fft_inverse(&s->plan, buf, VRj, var, scr);
-      for (unsigned t = 0; t < nt; t++) VRj[t] = cx_scale(VRj[t], s->delta_t);
+      if (f32m & ORC_FP32_RAMAN) {
+        fft_f32(&s->plan, Vj, buf, -1, scr);
+        for (unsigned t = 0; t < nt; t++) buf[t] = cx_mul_f32(buf[t], cx_f32(s->hR_fft[t]));
+        fft_f32(&s->plan, buf, VRj, +1, scr);
+        const double dtf = f32(s->delta_t);
+        for (unsigned t = 0; t < nt; t++) VRj[t] = cx_scale_f32(VRj[t], dtf);
+      } else {
+        fft_forward(&s->plan, Vj, buf, var, scr);
+        for (unsigned t = 0; t < nt; t++) buf[t] = cx_mul(buf[t], s->hR_fft[t]);
+        fft_inverse(&s->plan, buf, VRj, var, scr);
+        for (unsigned t = 0; t < nt; t++) VRj[t] = cx_scale(VRj[t], s->delta_t);
+      }
     }
   }
   /* Steps 3 and 4 per mode */
@@ -572,7 +630,17 @@ static void nonlinearity(orc_solver* s, cx* N_i, const cx* E_i, const cx* psi_i)
       const unsigned slot = s->nnz_slot[e];
       const cx* Vs = s->V + (size_t)slot * nt;
       const cx* Uk = U + (size_t)k * nt;
-      if (s->raman) {
+      if (f32m & ORC_FP32_GATHER) {
+        const double c1 = f32(s->raman ? Q * (1 - fR) : Q);
+        const double c2 = f32(Q * fR * s->T0);
+        const cx* VRs = s->VR + (size_t)slot * nt;
+        for (unsigned t = 0; t < nt; t++) {
+          cx W = cx_scale_f32(cx_f32(Vs[t]), c1);
+          if (s->raman) W = cx_add_f32(W, cx_scale_f32(cx_f32(VRs[t]), c2));
+          W = cx_mul_f32(W, cx_f32(Uk[t]));
+          sigma[t] = cx_add_f32(sigma[t], W);
+        }
+      } else if (s->raman) {
         const cx* VRs = s->VR + (size_t)slot * nt;
         const double c1 = Q * (1 - fR);
         const double c2 = Q * fR * s->T0;
@@ -593,7 +661,8 @@ static void nonlinearity(orc_solver* s, cx* N_i, const cx* E_i, const cx* psi_i)
     /* Step 4: N_i,p = F^-1[conj(E_i,p) * F sigma_p] * i*sign_gamma */
     cx* buf = s->tmp + (size_t)thread_id() * 3 * nt;
     cx* scr = buf + nt;
-    fft_forward(&s->plan, sigma, buf, var, scr);
+    if (f32m & ORC_FP32_SIGMA) fft_f32(&s->plan, sigma, buf, -1, scr);
+    else fft_forward(&s->plan, sigma, buf, var, scr);
     const cx* Ep = E_i + (size_t)p * nt;
     for (unsigned t = 0; t < nt; t++) buf[t] = cx_mul(buf[t], cx_conj(Ep[t]));
     cx* Np = N_i + (size_t)p * nt;

@@ -25,7 +25,19 @@ typedef struct orc_options {
   int fft_variant;      /* 0: radix-2 decimation in time; 1: radix-2 decimation in frequency */
   int parallel_pairs;   /* 0: serial Raman pair loop as the reference; 1: OpenMP over pairs (same result) */
   int n_threads;        /* OpenMP threads (0: omp_get_max_threads()) */
+  int fp32_stages;      /* 0 for the FP64 restatement. Error-budget studies only: ORC_FP32_* bits
+                           round the named stage's arithmetic to FP32 (each operation rounded to
+                           single precision), to measure what each stage of a mixed-precision
+                           solver contributes to its error */
 } orc_options;
+
+enum {
+  ORC_FP32_FIELDS = 1,  /* U = F^-1[E Fpsi]: the product rounded to FP32, the inverse transform in FP32 */
+  ORC_FP32_PAIRS = 2,   /* V_j = U_a conj(U_b) in FP32 */
+  ORC_FP32_RAMAN = 4,   /* the Raman convolution (transforms, h_R spectrum product, dt) in FP32 */
+  ORC_FP32_GATHER = 8,  /* the Kerr + Raman sum over the nonzeros in FP32 */
+  ORC_FP32_SIGMA = 16   /* F sigma in FP32 (widened to FP64 before the conj(E) product) */
+};
 
 /* Constructor (src/MultimodePropagation.cpp:4-12 + MultimodePropagationCPU_RamanON/OFF
  * constructors, src/MultimodePropagation_CPU.cpp:4-21,108,170-177). NULL on invalid input;

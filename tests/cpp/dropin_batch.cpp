@@ -57,7 +57,7 @@ static bool same(const ComplexArraysContainerCPU& a, const ComplexArraysContaine
 }
 
 static ComplexArraysContainerCPU alone(const SimulationParameters& in, unsigned int steps) {
-  MultimodePropagationGPU_RamanON s(in);  // FP64, the reference's pair decode
+  MultimodePropagationGPU_RamanON s(in);  // FP64, the GPU class's pair decode (by pair value)
   s.computeLawsonRK(steps);
   return s.getResult();
 }
@@ -79,7 +79,7 @@ int main() {
     o.slots = 2;  // the third request waits for a free slot
     o.max_chunk = 16;
     o.precision = 0;
-    o.pair_decode = 0;
+    o.pair_decode = 1;
     PropagationServer server(o);
     uint64_t id[3];
     for (unsigned int f = 0; f < 3; f++) id[f] = server.submit(fibers[f], steps[f]);

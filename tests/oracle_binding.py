@@ -17,7 +17,12 @@ REF_SO = os.path.join(REPO, "oracle", "_ref", "libref_setup.so")
 
 
 class OrcOptions(ctypes.Structure):
-    _fields_ = [("pair_decode", c_int), ("fft_variant", c_int), ("parallel_pairs", c_int), ("n_threads", c_int)]
+    _fields_ = [("pair_decode", c_int), ("fft_variant", c_int), ("parallel_pairs", c_int), ("n_threads", c_int),
+                ("fp32_stages", c_int)]
+
+
+# ORC_FP32_* of oracle/oracle.h: stages whose arithmetic an error-budget study rounds to FP32
+ORC_FP32_FIELDS, ORC_FP32_PAIRS, ORC_FP32_RAMAN, ORC_FP32_GATHER, ORC_FP32_SIGMA = 1, 2, 4, 8, 16
 
 
 class OrcSetupInfo(ctypes.Structure):
@@ -106,7 +111,7 @@ class Oracle:
     """The C restatement of the reference CPU solver."""
 
     def __init__(self, sp: SimulationParameters, raman: bool = True, pair_decode: int = 0, fft_variant: int = 0,
-                 parallel_pairs: bool = True, n_threads: int = -1):
+                 parallel_pairs: bool = True, n_threads: int = -1, fp32_stages: int = 0):
         self.sp = sp
         self.cp = CParams(sp, raman)
         self.M = int(sp.n_modes)
@@ -117,7 +122,7 @@ class Oracle:
             # cost more in barriers than they gain
             work = int(sp.n_modes) * int(sp.nt)
             n_threads = 1 if work < 65536 else min(8, os.cpu_count() or 1)
-        opt = OrcOptions(pair_decode, fft_variant, 1 if parallel_pairs else 0, n_threads)
+        opt = OrcOptions(pair_decode, fft_variant, 1 if parallel_pairs else 0, n_threads, fp32_stages)
         self.h = lib().orc_create(ctypes.byref(self.cp.c), ctypes.byref(opt))
         if not self.h:
             raise ValueError(lib().orc_last_error().decode())

@@ -20,7 +20,7 @@ def test_restart_semantics_match_oracle():
     sp = configs.two_mode_gaussian(nt=1024)
     note("case restart semantics M=2 NT=1024")
     g = MultimodePropagationGPU_RamanON(sp)
-    o = ob.Oracle(sp)
+    o = ob.Oracle(sp, pair_decode=1)  # the GPU classes decode pairs as the reference GPU class
     for nz in (7, 5, 11):
         g.computeLawsonRK(nz)
         o.compute_lawson_rk(nz)
@@ -57,7 +57,7 @@ def test_default_step_count_and_info():
     assert i["default_steps"] == math.floor(i["z_final"] / i["delta_z"])
     g.computeLawsonRK()
     assert g.info()["steps_taken"] == i["default_steps"]
-    o = ob.Oracle(sp)
+    o = ob.Oracle(sp, pair_decode=1)  # the GPU classes decode pairs as the reference GPU class
     o.compute_lawson_rk(o.default_steps())
     assert np.all(configs.rel_l2_per_mode(g.getResult(), o.get_result()) < 1e-12)
     g.close()
@@ -139,7 +139,7 @@ def test_snapshots_match_oracle(resumable):
     s = MultimodePropagationGPU_RamanON(sp)
     fields, z = s.propagate_snapshots(n_steps, n_save, resumable=resumable)
     s.close()
-    o = ob.Oracle(sp)
+    o = ob.Oracle(sp, pair_decode=1)  # the GPU classes decode pairs as the reference GPU class
     Z = 0.0
     for i, n in enumerate([12, 12, 12, 14]):
         if resumable and i > 0:
