@@ -194,7 +194,7 @@ static inline cx cx_scale_f32(cx a, double d) { cx r = {f32(a.re * d), f32(a.im 
 
 /* The radix-2 decimation-in-time transform with FP32 arithmetic and twiddles rounded to FP32;
  * the input is rounded to FP32 first. sign -1 forward unnormalised, +1 inverse with 1/n. */
-static void fft_f32(const fft_plan* p, const cx* in, cx* out, int sign, cx* scratch) {
+static void fft_f32(const fft_plan* p, const cx* in, cx* out, int sign, cx* scratch, int keep) {
   const unsigned n = p->n;
   if (!p->pow2) {  /* no FP32 variant of the O(n^2) transform: the FP64 one, rounded */
     fft_raw(p, in, out, sign, 0, scratch);
@@ -210,7 +210,7 @@ static void fft_f32(const fft_plan* p, const cx* in, cx* out, int sign, cx* scra
     unsigned stride = n / (2 * half);
     for (unsigned base = 0; base < n; base += 2 * half) {
       for (unsigned j = 0; j < half; j++) {
-        cx w = cx_f32(twiddle(p, j * stride, sign));
+        cx w = keep ? twiddle(p, j * stride, sign) : cx_f32(twiddle(p, j * stride, sign));
         cx u = out[base + j];
         cx v = cx_mul_f32(out[base + j + half], w);
         out[base + j] = cx_add_f32(u, v);
@@ -220,7 +220,7 @@ static void fft_f32(const fft_plan* p, const cx* in, cx* out, int sign, cx* scra
     }
   }
   if (sign > 0) {
-    const double f = f32(1.0 / n);
+    const double f = keep ? 1.0 / n : f32(1.0 / n);
     for (unsigned i = 0; i < n; i++) out[i] = cx_scale_f32(out[i], f);
   }
 }
@@ -577,6 +577,8 @@ static void nonlinearity(orc_solver* s, cx* N_i, const cx* E_i, const cx* psi_i)
   const int var = s->opt.fft_variant;
   const int nth = s->nthreads;
   const int f32m = s->opt.fp32_stages;
+  const int keep = (f32m & ORC_FP32_KEEP_CONSTANTS) != 0;
+  const int acc64 = (f32m & ORC_FP32_GATHER_ACC64) != 0;
   cx* U = s->U;
   /* Step 1: U_p = F^-1[E_i,p * F psi_i,p] */
 #pragma omp parallel for num_threads(nth) schedule(static)
@@ -586,7 +588,7 @@ static void nonlinearity(orc_solver* s, cx* N_i, const cx* E_i, const cx* psi_i)
     fft_forward(&s->plan, psi_i + (size_t)p * nt, buf, var, scr);
     const cx* Ep = E_i + (size_t)p * nt;
     for (unsigned j = 0; j < nt; j++) buf[j] = cx_mul(buf[j], Ep[j]);
-    if (f32m & ORC_FP32_FIELDS) fft_f32(&s->plan, buf, U + (size_t)p * nt, +1, scr);
+    if (f32m & ORC_FP32_FIELDS) fft_f32(&s->plan, buf, U + (size_t)p * nt, +1, scr, keep);
     else fft_inverse(&s->plan, buf, U + (size_t)p * nt, var, scr);
   }
   /* Step 2: V_j = U_a * conj(U_b) for slot j; RamanON also V_R,j = dt * F^-1[hR_fft * F V_j] */
@@ -604,10 +606,10 @@ static void nonlinearity(orc_solver* s, cx* N_i, const cx* E_i, const cx* psi_i)
       cx* scr = buf + nt;
       cx* VRj = s->VR + (size_t)j * nt;
       if (f32m & ORC_FP32_RAMAN) {
-        fft_f32(&s->plan, Vj, buf, -1, scr);
-        for (unsigned t = 0; t < nt; t++) buf[t] = cx_mul_f32(buf[t], cx_f32(s->hR_fft[t]));
-        fft_f32(&s->plan, buf, VRj, +1, scr);
-        const double dtf = f32(s->delta_t);
+        fft_f32(&s->plan, Vj, buf, -1, scr, keep);
+        for (unsigned t = 0; t < nt; t++) buf[t] = cx_mul_f32(buf[t], keep ? s->hR_fft[t] : cx_f32(s->hR_fft[t]));
+        fft_f32(&s->plan, buf, VRj, +1, scr, keep);
+        const double dtf = keep ? s->delta_t : f32(s->delta_t);
         for (unsigned t = 0; t < nt; t++) VRj[t] = cx_scale_f32(VRj[t], dtf);
       } else {
         fft_forward(&s->plan, Vj, buf, var, scr);
@@ -631,14 +633,14 @@ static void nonlinearity(orc_solver* s, cx* N_i, const cx* E_i, const cx* psi_i)
       const cx* Vs = s->V + (size_t)slot * nt;
       const cx* Uk = U + (size_t)k * nt;
       if (f32m & ORC_FP32_GATHER) {
-        const double c1 = f32(s->raman ? Q * (1 - fR) : Q);
-        const double c2 = f32(Q * fR * s->T0);
+        const double c1 = keep ? (s->raman ? Q * (1 - fR) : Q) : f32(s->raman ? Q * (1 - fR) : Q);
+        const double c2 = keep ? Q * fR * s->T0 : f32(Q * fR * s->T0);
         const cx* VRs = s->VR + (size_t)slot * nt;
         for (unsigned t = 0; t < nt; t++) {
           cx W = cx_scale_f32(cx_f32(Vs[t]), c1);
           if (s->raman) W = cx_add_f32(W, cx_scale_f32(cx_f32(VRs[t]), c2));
           W = cx_mul_f32(W, cx_f32(Uk[t]));
-          sigma[t] = cx_add_f32(sigma[t], W);
+          sigma[t] = acc64 ? cx_add(sigma[t], W) : cx_add_f32(sigma[t], W);
         }
       } else if (s->raman) {
         const cx* VRs = s->VR + (size_t)slot * nt;
@@ -661,7 +663,7 @@ static void nonlinearity(orc_solver* s, cx* N_i, const cx* E_i, const cx* psi_i)
     /* Step 4: N_i,p = F^-1[conj(E_i,p) * F sigma_p] * i*sign_gamma */
     cx* buf = s->tmp + (size_t)thread_id() * 3 * nt;
     cx* scr = buf + nt;
-    if (f32m & ORC_FP32_SIGMA) fft_f32(&s->plan, sigma, buf, -1, scr);
+    if (f32m & ORC_FP32_SIGMA) fft_f32(&s->plan, sigma, buf, -1, scr, keep);
     else fft_forward(&s->plan, sigma, buf, var, scr);
     const cx* Ep = E_i + (size_t)p * nt;
     for (unsigned t = 0; t < nt; t++) buf[t] = cx_mul(buf[t], cx_conj(Ep[t]));

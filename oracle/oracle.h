@@ -36,7 +36,11 @@ enum {
   ORC_FP32_PAIRS = 2,   /* V_j = U_a conj(U_b) in FP32 */
   ORC_FP32_RAMAN = 4,   /* the Raman convolution (transforms, h_R spectrum product, dt) in FP32 */
   ORC_FP32_GATHER = 8,  /* the Kerr + Raman sum over the nonzeros in FP32 */
-  ORC_FP32_SIGMA = 16   /* F sigma in FP32 (widened to FP64 before the conj(E) product) */
+  ORC_FP32_SIGMA = 16,  /* F sigma in FP32 (widened to FP64 before the conj(E) product) */
+  ORC_FP32_KEEP_CONSTANTS = 32, /* with the bits above: twiddles, coupling values, the h_R spectrum and
+                                   the slot-mix factors stay FP64 (only operation results are rounded),
+                                   to separate the rounding of the constants from that of the arithmetic */
+  ORC_FP32_GATHER_ACC64 = 64  /* with ORC_FP32_GATHER: the products in FP32, their sum over the nonzeros in FP64 */
 };
 
 /* Constructor (src/MultimodePropagation.cpp:4-12 + MultimodePropagationCPU_RamanON/OFF
