@@ -214,6 +214,24 @@ MMP_API mmp_status mmp_fiber_load(mmp_solver* solver, uint32_t fiber, const mmp_
  * solver that does not run the column-stage kernels (batches always do). */
 MMP_API mmp_status mmp_fiber_set_live(mmp_solver* solver, uint32_t fiber, uint32_t live);
 
+/* Device memory of one mmp_solve_static call: every device array of the solve must fit in it. */
+#define MMP_STATIC_SCRATCH_BYTES ((size_t)64 << 20)
+
+/* No-allocation solve. No counterpart in the reference: for callers that cannot allocate device
+ * memory (code running under HIP graph capture, tight serving loops, hosts that forbid hipMalloc).
+ * Equivalent to mmp_solver_create(ctx, params, options) + mmp_compute_lawson_rk(nz) +
+ * mmp_get_result(fields_out) + mmp_solver_destroy, with the same kernels and the same results bit
+ * for bit, but every device array lives in a static per-device scratch of MMP_STATIC_SCRATCH_BYTES
+ * that the library holds until the process ends: the call makes no hipMalloc, hipFree, graph
+ * instantiation or event. Kernels are launched one by one on the context's stream; the call returns
+ * once the result is on the host. The scratch keeps the last call's contents (each call uploads
+ * everything it reads). Calls are serialised by a process-wide lock: calls from several threads
+ * wait for each other, since overlapping calls on one device would race on the one scratch.
+ * MMP_ERR_UNSUPPORTED when the problem's device footprint (mmp_plan's device_bytes plus alignment)
+ * exceeds the scratch; MMP_ERR_INVALID_ARGUMENT with MMP_FLAG_PROFILE or MMP_FLAG_AUTOTUNE. */
+MMP_API mmp_status mmp_solve_static(mmp_ctx* ctx, const mmp_params* params, const mmp_options* options, uint32_t nz,
+                                    double* fields_out);
+
 /* Replaces computeLawsonRK(nz) (include/MultimodePropagation_GPU.cuh:143;
  * semantics of src/MultimodePropagation_CPU.cpp:24-34): restarts at z = 0
  * with freshly built Lawson operators, keeps the current field, then takes

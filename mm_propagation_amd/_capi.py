@@ -142,6 +142,7 @@ EXPORTS = [
     ("mmp_get_result", c_int, [c_void_p, POINTER(c_double)]),
     ("mmp_batch_create", c_int, [c_void_p, c_uint32, c_void_p, c_void_p, POINTER(c_void_p)]),
     ("mmp_ctx_query", c_int, [c_void_p]),
+    ("mmp_solve_static", c_int, [c_void_p, POINTER(MmpParams), POINTER(MmpOptions), c_uint32, POINTER(c_double)]),
     ("mmp_fiber_result", c_int, [c_void_p, c_uint32, POINTER(c_double)]),
     ("mmp_fiber_state", c_int, [c_void_p, c_uint32, POINTER(c_double), POINTER(c_double)]),
     ("mmp_fiber_set_state", c_int, [c_void_p, c_uint32, POINTER(c_double)]),

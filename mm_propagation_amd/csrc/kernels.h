@@ -234,6 +234,14 @@ int dense_gather_chunk(int M);
 template <class W>
 hipError_t launch_gather_dense(const Geometry& g, const cpx<W>* U, const cpx<W>* VR, cpx<W>* sigma,
                                const GatherArgs& a, const W* Qd, hipStream_t s);
+// The no-allocation entry point's device scratch: a file-scope __device__ array of
+// kStaticScratchBytes per device (kernels.hip). static_scratch_resolve looks up its address on
+// `device` (which must be current) once; static_scratch returns it (nullptr before that).
+constexpr size_t kStaticScratchBytes = (size_t)64 << 20;
+constexpr int kStaticScratchDevices = 64;
+hipError_t static_scratch_resolve(int device);
+void* static_scratch(int device);
+
 // out[j] = in[j] rounded to FP32, j < n
 hipError_t launch_narrow(const cd* in, cf* out, size_t n, hipStream_t s);
 hipError_t launch_energy(const Geometry& g, const cd* fields, double* out, double dt, hipStream_t s);

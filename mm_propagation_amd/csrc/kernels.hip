@@ -17,6 +17,7 @@
 
 #include <hip/hip_ext.h>
 
+#include <mutex>
 #include <type_traits>
 
 namespace mmp {
@@ -1519,6 +1520,38 @@ hipError_t launch_gather_dense(const Geometry& g, const cpx<W>* U, const cpx<W>*
 __global__ void __launch_bounds__(256) k_narrow(const cd* __restrict__ in, cf* __restrict__ out, size_t n) {
   for (size_t j = (size_t)blockIdx.x * 256 + threadIdx.x; j < n; j += (size_t)gridDim.x * 256)
     out[j] = narrow<float>(in[j]);
+}
+
+}  // namespace mmp
+
+// Device memory of the no-allocation entry point (mmp_solve_static): every device array of one
+// solve is carved from this array, so the call needs no hipMalloc. One copy per device, held
+// until the process ends; its host-side name is not a device address (the address comes from
+// hipGetSymbolAddress, once per device, in static_scratch_resolve).
+static __device__ __align__(256) unsigned char scratch[mmp::kStaticScratchBytes];
+
+namespace mmp {
+
+namespace {
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kStaticScratchDevices] = {};
+}  // namespace
+
+hipError_t static_scratch_resolve(int device) {
+  if (device < 0 || device >= kStaticScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_ptr[device] = p;
+  return hipSuccess;
+}
+
+void* static_scratch(int device) {
+  if (device < 0 || device >= kStaticScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  return g_scratch_ptr[device];
 }
 
 hipError_t launch_narrow(const cd* in, cf* out, size_t n, hipStream_t s) {

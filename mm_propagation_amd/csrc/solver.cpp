@@ -85,6 +85,9 @@ struct mmp_solver {
   Twiddles tw{};
   std::vector<void*> allocations;
   size_t device_bytes = 0;
+  // no-allocation solve: every device array is carved from the static scratch instead
+  unsigned char* arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
   cd* psi_hat = nullptr;
   cd* acc = nullptr;
   cd* ybuf = nullptr;
@@ -140,12 +143,23 @@ struct mmp_solver {
 
 namespace {
 
+constexpr size_t kArenaAlign = 256;
+
 template <class T>
 mmp_status dalloc(mmp_solver* s, T** out, size_t count) {
   void* p = nullptr;
   const size_t bytes = sizeof(T) * (count ? count : 1);
-  HIP_TRY(hipMalloc(&p, bytes), "hipMalloc");
-  s->allocations.push_back(p);
+  if (s->arena) {
+    const size_t off = (s->arena_used + kArenaAlign - 1) & ~(kArenaAlign - 1);
+    if (off > s->arena_cap || bytes > s->arena_cap - off)
+      return fail(MMP_ERR_UNSUPPORTED, "the problem needs more device memory than the static scratch holds (" +
+                                           std::to_string(s->arena_cap) + " bytes)");
+    p = s->arena + off;
+    s->arena_used = off + bytes;
+  } else {
+    HIP_TRY(hipMalloc(&p, bytes), "hipMalloc");
+    s->allocations.push_back(p);
+  }
   s->device_bytes += bytes;
   *out = static_cast<T*>(p);
   return MMP_OK;
@@ -574,8 +588,11 @@ mmp_status load_fiber(mmp_solver* s, uint32_t f, Problem& Pf) {
 
 mmp_status autotune(mmp_solver* s);
 
+// arena (optional): carve every device array from [arena, arena + arena_cap) instead of
+// allocating; such a solver launches kernel by kernel (no graph instantiation) and takes neither
+// MMP_FLAG_PROFILE nor MMP_FLAG_AUTOTUNE (both allocate).
 mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params, const mmp_options* options,
-                       mmp_solver** out) {
+                       mmp_solver** out, unsigned char* arena = nullptr, size_t arena_cap = 0) {
   if (!ctx || !out || !params) return fail(MMP_ERR_INVALID_ARGUMENT, "ctx, params and out are required");
   *out = nullptr;
   if (n_fibers < 1 || n_fibers > 65535) return fail(MMP_ERR_INVALID_ARGUMENT, "n_fibers must be in [1, 65535]");
@@ -586,6 +603,8 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
   s->ctx = ctx;
   s->opt = opt;
   s->F = n_fibers;
+  s->arena = arena;
+  s->arena_cap = arena_cap;
   s->fib.resize(n_fibers);
   std::string err;
   for (uint32_t f = 0; f < n_fibers; f++) {
@@ -611,7 +630,7 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
   s->mixed = opt.precision == MMP_PRECISION_MIXED;
   s->wbytes = s->mixed ? sizeof(cf) : sizeof(cd);
   s->profile = (opt.flags & MMP_FLAG_PROFILE) != 0;
-  s->graphs = !s->profile && (opt.flags & MMP_FLAG_NO_GRAPH) == 0;
+  s->graphs = !s->profile && (opt.flags & MMP_FLAG_NO_GRAPH) == 0 && !s->arena;
   Problem& P = s->P;
   const uint32_t F = n_fibers;
   s->g = Geometry{(int)P.N, (int)P.N1, (int)P.N2, (int)P.M};
@@ -1026,6 +1045,8 @@ mmp_status mmp_ctx_create(int device, mmp_ctx** out) {
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     return fail(MMP_ERR_NO_DEVICE, std::string("kernels are built for gfx950, device is ") + prop.gcnArchName);
   HIP_TRY(hipSetDevice(device), "hipSetDevice");
+  // the no-allocation entry point's scratch: its address on this device, looked up once
+  HIP_TRY(static_scratch_resolve(device), "hipGetSymbolAddress(scratch)");
   mmp_ctx* c = new (std::nothrow) mmp_ctx();
   if (!c) return fail(MMP_ERR_OUT_OF_MEMORY, "host allocation failed");
   c->device = device;
@@ -1087,6 +1108,45 @@ mmp_status mmp_batch_create(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* p
 }
 
 void mmp_solver_destroy(mmp_solver* s) { release(s); }
+
+// One solve whose device arrays all live in the device's static scratch (no allocation). Calls are
+// serialised by this lock: the scratch is one array per device, and a call ends only when its
+// result has been copied to the host.
+static std::mutex g_static_solve_mutex;
+static_assert(MMP_STATIC_SCRATCH_BYTES == kStaticScratchBytes, "the header states the scratch size");
+
+mmp_status mmp_solve_static(mmp_ctx* ctx, const mmp_params* params, const mmp_options* options, uint32_t nz,
+                            double* fields_out) {
+  if (!ctx || !params || !fields_out) return fail(MMP_ERR_INVALID_ARGUMENT, "ctx, params and fields_out are required");
+  mmp_options opt{};
+  if (options) opt = *options;
+  if (opt.flags & (MMP_FLAG_PROFILE | MMP_FLAG_AUTOTUNE))
+    return fail(MMP_ERR_INVALID_ARGUMENT, "mmp_solve_static takes neither MMP_FLAG_PROFILE nor MMP_FLAG_AUTOTUNE");
+  try {
+    // the footprint first, on the host, so that a problem too large is refused before any device work
+    mmp_info plan{};
+    MMP_TRY(mmp_plan(params, &opt, &plan));
+    const size_t slack = 64 * kArenaAlign;  // alignment of the solver's arrays within the scratch
+    if (plan.device_bytes + slack > kStaticScratchBytes)
+      return fail(MMP_ERR_UNSUPPORTED, "mmp_solve_static: the problem needs " + std::to_string(plan.device_bytes) +
+                                           " bytes of device memory; the static scratch holds " +
+                                           std::to_string(kStaticScratchBytes) + " (use mmp_solver_create)");
+    std::lock_guard<std::mutex> lock(g_static_solve_mutex);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    unsigned char* base = static_cast<unsigned char*>(static_scratch(ctx->device));
+    if (!base) return fail(MMP_ERR_STATE, "the static scratch of this device is not resolved (mmp_ctx_create)");
+    mmp_solver* s = nullptr;
+    MMP_TRY(create_impl(ctx, 1, params, &opt, &s, base, kStaticScratchBytes));
+    mmp_status st = mmp_compute_lawson_rk(s, nz);
+    if (st == MMP_OK) st = mmp_fiber_result(s, 0, fields_out);
+    release(s);  // waits for the stream; frees no device memory
+    return st;
+  } catch (const std::bad_alloc&) {
+    return fail(MMP_ERR_OUT_OF_MEMORY, "host allocation failed");
+  } catch (...) {
+    return fail(MMP_ERR_DEVICE, "unexpected exception");
+  }
+}
 
 mmp_status mmp_compute_lawson_rk(mmp_solver* s, uint32_t nz) {
   if (!s) return fail(MMP_ERR_INVALID_ARGUMENT, "solver is NULL");

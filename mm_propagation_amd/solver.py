@@ -177,3 +177,19 @@ def make_solver(params: SimulationParameters, **kw) -> MultimodePropagationGPU:
     """Class choice of matlab/mm_propagation.cpp:59: Raman off when raman_proportion == 0."""
     cls = MultimodePropagationGPU_RamanOFF if params.raman_proportion == 0.0 else MultimodePropagationGPU_RamanON
     return cls(params, **kw)
+
+
+def solve_static(params: SimulationParameters, nz: int, raman: bool = True,
+                 pair_decode: int = _capi.MMP_PAIRS_PHYSICAL, precision: int = _capi.MMP_PRECISION_FP64,
+                 flags: int = 0, ctx: Optional[Context] = None) -> np.ndarray:
+    """mmp_solve_static: computeLawsonRK(nz) from the initial field and the physical field at the
+    end, with every device array in the library's static per-device scratch (no device
+    allocation). Same result as the solver classes, bit for bit."""
+    ctx = ctx or default_context()
+    cp = CParams(params, raman)
+    opt = _capi.MmpOptions(precision=precision, pair_decode=pair_decode, flags=flags, reserved=0)
+    out = np.empty((int(params.n_modes), int(params.nt)), dtype=np.complex128)
+    _capi.check(ctx.lib.mmp_solve_static(ctx.h, ctypes.byref(cp.c), ctypes.byref(opt), int(nz),
+                                         out.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                "mmp_solve_static")
+    return out
