@@ -109,8 +109,10 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels one by one instead of replaying HIP graphs (for kernel tracers)")
-    ap.add_argument("--autotune", action="store_true",
-                    help="MMP_FLAG_AUTOTUNE: each solver times its gather variants at creation and keeps the fastest")
+    ap.add_argument("--autotune", dest="autotune", action="store_true", default=None,
+                    help="MMP_FLAG_AUTOTUNE: each solver times its column-stage variants at creation and keeps the "
+                         "fastest (default for the batched workload, off for the others)")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false")
     ap.add_argument("--separate-columns", action="store_true",
                     help="every column pass a kernel of its own (7 per stage instead of 4)")
     ap.add_argument("--mixed", action="store_true",
@@ -287,7 +289,7 @@ def run_batched(args) -> int:
     Timed: exactly --steps steps of every fiber, fields resident, after --warmup steps and a
     restart of every fiber at z = 0 from its initial field."""
     from mm_propagation_amd import _capi, configs
-    from mm_propagation_amd.batch import MultimodePropagationBatch
+    from mm_propagation_amd.batch import MultimodePropagationBatch, SolverGroup
     from mm_propagation_amd.solver import Context, MultimodePropagationGPU_RamanON
 
     prec = _capi.MMP_PRECISION_FP64 if args.fp64_batch else _capi.MMP_PRECISION_MIXED
@@ -295,7 +297,8 @@ def run_batched(args) -> int:
     classes = {}
     for i, sp in enumerate(fibers):
         classes.setdefault((sp.n_modes, sp.nt), []).append(i)
-    bflags = (_capi.MMP_FLAG_AUTOTUNE if args.autotune else 0) | (_capi.MMP_FLAG_NO_GRAPH if args.no_graph else 0)
+    autotune = True if args.autotune is None else args.autotune
+    bflags = (_capi.MMP_FLAG_AUTOTUNE if autotune else 0) | (_capi.MMP_FLAG_NO_GRAPH if args.no_graph else 0)
     one_stream = args.batch_streams == "one"
     shared = Context(0) if one_stream else None
     batches = []
@@ -313,14 +316,20 @@ def run_batched(args) -> int:
     contexts = [shared] if one_stream else [ctx for _, _, ctx, _ in batches]
     psi0 = {(bi, f): b.state(f)[0] for bi, (_, idx, _, b) in enumerate(batches) for f in range(len(idx))}
 
+    # one stream: every class in one graph (mmp_group: one launch per 16 steps of all 256 fibers)
+    group = SolverGroup([b for _, _, _, b in batches], shared) if one_stream and not args.no_graph else None
+
     def run(n):
-        """n steps of every fiber: the classes' 16-step graphs interleaved, then the remainder."""
-        left = n
-        while left > 0:
-            k = min(16, left)
-            for _, _, _, b in batches:
-                b.advance(k)
-            left -= k
+        """n steps of every fiber: the group's graph, or the classes' 16-step graphs interleaved."""
+        if group is not None:
+            group.advance(n)
+        else:
+            left = n
+            while left > 0:
+                k = min(16, left)
+                for _, _, _, b in batches:
+                    b.advance(k)
+                left -= k
         for c in contexts:
             c.synchronize()
 
@@ -339,17 +348,19 @@ def run_batched(args) -> int:
     run(steps)
     elapsed = time.perf_counter() - t0
 
-    # fibers whose field is still finite count; a non-finite fiber's steps are not throughput
-    finite = {}
+    # only the fibers whose field stays finite count: the device parks a fiber in the step its
+    # field stops being finite (its later steps skip it), and none of its steps is throughput
+    finite = set()
+    halted_at = []
     for bi, (_, idx, _, b) in enumerate(batches):
         for f in range(len(idx)):
-            try:
-                finite[(bi, f)] = b.result(f)
-            except _capi.MmpError as e:
-                if e.status != _capi.MMP_ERR_NONFINITE:
-                    raise
-    units = sum(fibers[idx[f]].n_modes * fibers[idx[f]].nt
-                for bi, (_, idx, _, _) in enumerate(batches) for f in range(len(idx)) if (bi, f) in finite)
+            info = b.info(f)
+            if info["halted"]:
+                halted_at.append(int(info["steps_taken"]))
+            else:
+                finite.add((bi, f))
+    finite_fibers = [idx[f] for bi, (_, idx, _, _) in enumerate(batches) for f in range(len(idx)) if (bi, f) in finite]
+    units = sum(fibers[i].n_modes * fibers[i].nt for i in finite_fibers)
     value = units * steps / elapsed
 
     # accuracy and the CPU baseline: in every class the lowest-power fiber, after the timed
@@ -379,13 +390,14 @@ def run_batched(args) -> int:
             acc[f"M{key[0]}_NT{key[1]}_fiber{idx[f]}"] = {"rel_l2_max": float(err.max()),
                                                          "energy_drift": float((e_got - e_ref) / e_ref)}
             cpu_per_step[f"M{key[0]}_NT{key[1]}"] = round(dt / acc_steps * 1e3, 2)
-            cpu_time += dt / acc_steps * len(idx)  # the class's fibers one after another, per step
-            cpu_units += sp.n_modes * sp.nt * len(idx)
+            n_fin = sum(1 for f2 in range(len(idx)) if (bi, f2) in finite)
+            cpu_time += dt / acc_steps * n_fin  # the class's finite fibers one after another, per step
+            cpu_units += sp.n_modes * sp.nt * n_fin
     cpu = None
     if cpu_time > 0:
         cpu = {"value": cpu_units / cpu_time, "unit": "M·NT·steps/s", "cores": threads, "kind": "port",
                "sample": f"the lowest-power fiber of each of the {len(batches)} classes, {acc_steps} RK4 steps each "
-                         f"(ms per step: {cpu_per_step}), extrapolated to all {args.fibers} fibers; the oracle "
+                         f"(ms per step: {cpu_per_step}), extrapolated to the {len(finite)} finite fibers; the oracle "
                          f"(FP64 C restatement of the reference CPU path) with {threads} OpenMP threads over "
                          "modes and the Raman pair loop serial as in the reference; radix-2 FFT (no FFTW here)"}
 
@@ -394,39 +406,67 @@ def run_batched(args) -> int:
     plain_time = 0.0
     plain = {}
     pctx = shared or batches[0][2]
-    for key, idx, _, _ in batches:
-        s1 = MultimodePropagationGPU_RamanON(fibers[idx[0]], ctx=pctx)
-        s1.advance(16)
-        pctx.synchronize()
-        t1 = time.perf_counter()
-        s1.advance(160)
-        pctx.synchronize()
-        per_step = (time.perf_counter() - t1) / 160
-        s1.close()
-        plain[f"M{key[0]}_NT{key[1]}"] = round(per_step * 1e6, 2)
-        plain_time += per_step * len(idx)
-    plain_value = sum(fibers[i].n_modes * fibers[i].nt for i in range(args.fibers)) / plain_time
+    def power(i):
+        return float(np.max(np.abs(fibers[i].initial_fields) ** 2))
 
-    # per-kernel dispatch-recorded times of every class (eager launches of the same kernels)
+    for bi, (key, idx, _, _) in enumerate(batches):
+        fin = sorted((idx[f] for f in range(len(idx)) if (bi, f) in finite), key=power)
+        per_step = None
+        # a fiber that stays finite over the timing window (a parked fiber skips its steps); the
+        # window is the timed run's own length at most, over which these fibers stayed finite
+        n_time = max(1, min(128, steps - 4))
+        for i in fin:
+            s1 = MultimodePropagationGPU_RamanON(fibers[i], ctx=pctx,
+                                                 flags=_capi.MMP_FLAG_NO_GRAPH if args.no_graph else 0)
+            s1.advance(4)
+            pctx.synchronize()
+            t1 = time.perf_counter()
+            s1.advance(n_time)
+            pctx.synchronize()
+            dt = (time.perf_counter() - t1) / n_time
+            halted = s1.info()["halted"]
+            s1.close()
+            if not halted:
+                per_step = dt
+                break
+        if per_step is None:
+            continue
+        plain[f"M{key[0]}_NT{key[1]}"] = round(per_step * 1e6, 2)
+        plain_time += per_step * sum(1 for f in range(len(idx)) if (bi, f) in finite)
+    plain_value = units / plain_time if plain_time > 0 else None
+
+    # per-kernel dispatch-recorded times (eager launches of the same kernels): every class replays
+    # the timed run (from the initial fields, the same steps, the same fibers parked on the way),
+    # and their summed kernel time per step stands beside ms_per_step
     kernel_us_per_step = 0.0
     per_class = {}
-    dom = None
     for key, idx, ctx, _ in batches:
         pb = MultimodePropagationBatch([fibers[i] for i in idx], precision=prec, ctx=ctx,
                                        flags=_capi.MMP_FLAG_PROFILE | (bflags & _capi.MMP_FLAG_AUTOTUNE))
-        pb.advance(4)
         pb.kernel_times()
-        nprof = 16
-        pb.advance(nprof)
+        pb.advance(steps)
         kt = pb.kernel_times()
-        info = pb.info(0)
         pb.close()
-        cls_us = sum(k["total_ms"] for k in kt) * 1e3 / nprof
+        cls_us = sum(k["total_ms"] for k in kt) * 1e3 / steps
         kernel_us_per_step += cls_us
         per_class[f"M{key[0]}_NT{key[1]}"] = round(cls_us, 1)
-        for k in kt:
-            if dom is None or k["total_ms"] > dom[0]["total_ms"]:
-                dom = (k, key, idx, info)
+    # the dominant kernel: the class with the most finite work, its finite fibers only (every
+    # fiber of each launch live)
+    def finite_units(t):
+        bi, (key, idx, _, _) = t
+        return key[0] * key[1] * sum(1 for f in range(len(idx)) if (bi, f) in finite)
+
+    bi, (key, idx, ctx, _) = max(enumerate(batches), key=finite_units)
+    fin = [idx[f] for f in range(len(idx)) if (bi, f) in finite] or idx
+    pb = MultimodePropagationBatch([fibers[i] for i in fin], precision=prec, ctx=ctx,
+                                   flags=_capi.MMP_FLAG_PROFILE | (bflags & _capi.MMP_FLAG_AUTOTUNE))
+    pb.advance(min(4, steps))
+    pb.kernel_times()
+    pb.advance(max(1, min(16, steps - 4)))  # within the steps over which these fibers stayed finite
+    kt = pb.kernel_times()
+    info = pb.info(0)
+    pb.close()
+    dom = (max(kt, key=lambda k: k["total_ms"]), key, fin, info)
     k, key, idx, info = dom
     avg_s = k["total_ms"] / 1e3 / max(k["launches"], 1)
     gbps = k["bytes_per_launch"] / avg_s / 1e9
@@ -436,10 +476,15 @@ def run_batched(args) -> int:
                                       coupling_parity(fibers[idx[0]])) * len(idx)
         tflops = flops / avg_s / 1e12
         peak = VALU_PEAK_TFLOPS["f64" if args.fp64_batch else "f32"]
-        roofline = {"bound": "valu", "kernel": k["name"], "achieved": round(tflops, 3), "peak": peak,
-                    "unit": "TFLOP/s", "frac": round(tflops / peak, 5), "traffic": None,
-                    "hbm_gbps": round(gbps, 1), "hbm_frac": round(gbps / HBM_PEAK_GBS, 4),
-                    "flops_per_launch": flops, "bytes_per_launch": k["bytes_per_launch"]}
+        if tflops / peak >= gbps / HBM_PEAK_GBS:
+            roofline = {"bound": "valu", "kernel": k["name"], "achieved": round(tflops, 3), "peak": peak,
+                        "unit": "TFLOP/s", "frac": round(tflops / peak, 5), "traffic": None,
+                        "hbm_gbps": round(gbps, 1), "hbm_frac": round(gbps / HBM_PEAK_GBS, 4)}
+        else:
+            roofline = {"bound": "hbm", "kernel": k["name"], "achieved": round(gbps, 2), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBS, 5), "traffic": None,
+                        "valu_tflops": round(tflops, 3), "valu_frac": round(tflops / peak, 4)}
+        roofline.update({"flops_per_launch": flops, "bytes_per_launch": k["bytes_per_launch"]})
     else:
         roofline = {"bound": "hbm", "kernel": k["name"], "achieved": round(gbps, 2), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBS, 5), "traffic": None,
@@ -456,10 +501,18 @@ def run_batched(args) -> int:
                                "P log-uniform [10 W, 10 kW], symmetric GRIN coupling, RK4, Raman on, "
                                + ("FP64" if args.fp64_batch else "FP32 transforms and sums, FP64 state") +
                                f"), {len(batches)} batches by (M, NT), "
-                               + ("one stream" if one_stream else "one stream per batch"),
+                               + ("one graph for all classes (one stream)" if group is not None
+                                  else ("one stream" if one_stream else "one stream per batch"))
+                               + (", autotuned" if autotune else ""),
                    "global_batch": args.fibers, "seq_len": "4096-16384", "parallelism": "single GPU"},
-        "fibers_finite": len(finite), "setup_s": round(t_setup, 2),
-        "plain_fp64_single_fiber": {"value": plain_value, "per_step_us": plain, "ratio": value / plain_value},
+        "fibers_finite": len(finite),
+        "fibers_parked": {"count": len(halted_at), "steps_before_parking_median":
+                          float(np.median(halted_at)) if halted_at else None},
+        "setup_s": round(t_setup, 2),
+        "plain_fp64_single_fiber": {"value": plain_value, "per_step_us": plain,
+                                    "ratio": value / plain_value if plain_value else None,
+                                    "note": "the plain FP64 solver's per-step time for one fiber of each class, "
+                                            "over the same finite fibers solved one after another"},
         "cpu_ratio": (value / cpu["value"]) if cpu else None,
         "accuracy": {"rel_l2_vs_fp64_oracle": acc, "steps": acc_steps},
         "kernel_sum_ms_per_step": round(kernel_us_per_step / 1e3, 4),
@@ -468,6 +521,8 @@ def run_batched(args) -> int:
         "plans": plans,
     }
     print(json.dumps(out), flush=True)
+    if group is not None:
+        group.close()
     for _, _, ctx, b in batches:
         b.close()
     for c in contexts:

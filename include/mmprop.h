@@ -134,7 +134,8 @@ typedef struct mmp_info {
   uint32_t pair_columns;     /* column-stage pair pass: neighbouring columns per workgroup (0: separate passes) */
   uint32_t pair_groups;      /* column-stage pair pass: workgroups sharing one column block's Raman lines */
   uint32_t gather_hi_occupancy; /* 1 when the gather runs its three-workgroups-per-CU variant */
-  uint32_t reserved;
+  uint32_t halted;           /* 1 when the device parked this fiber because its field stopped being finite
+                                (after the step that made it so: z and steps_taken count that step) */
 } mmp_info;
 
 typedef struct mmp_ctx mmp_ctx;
@@ -211,7 +212,12 @@ MMP_API mmp_status mmp_fiber_load(mmp_solver* solver, uint32_t fiber, const mmp_
  * Lawson tables untouched, and its workgroups exit at once, so a batch whose slots are partly
  * empty costs about what its live fibers cost. mmp_fiber_load makes a fiber live. Waits for the
  * work already queued on the context's stream. Fibers start live. MMP_ERR_UNSUPPORTED for a
- * solver that does not run the column-stage kernels (batches always do). */
+ * solver that does not run the column-stage kernels (batches always do).
+ * A fiber whose spectral state stops being finite is parked by the device itself at the end of
+ * that step (no reference counterpart: the reference keeps stepping a NaN field): its later steps
+ * cost nothing, its z and step count stop there, mmp_info.halted reports it, and resuming it with
+ * live = 1 returns MMP_ERR_NONFINITE; mmp_fiber_load or mmp_fiber_set_state gives it a finite
+ * field and makes it live again. */
 MMP_API mmp_status mmp_fiber_set_live(mmp_solver* solver, uint32_t fiber, uint32_t live);
 
 /* Device memory of one mmp_solve_static call: every device array of the solve must fit in it. */
@@ -231,6 +237,17 @@ MMP_API mmp_status mmp_fiber_set_live(mmp_solver* solver, uint32_t fiber, uint32
  * exceeds the scratch; MMP_ERR_INVALID_ARGUMENT with MMP_FLAG_PROFILE or MMP_FLAG_AUTOTUNE. */
 MMP_API mmp_status mmp_solve_static(mmp_ctx* ctx, const mmp_params* params, const mmp_options* options, uint32_t nz,
                                     double* fields_out);
+
+/* Solvers of different shapes advanced together (no reference counterpart: the reference runs one
+ * fiber per object). A group holds solvers and batches of any n_modes, nt and precision, all made
+ * on ctx with HIP graphs (neither MMP_FLAG_PROFILE nor MMP_FLAG_NO_GRAPH); one graph, captured at
+ * mmp_group_create, holds 16 steps of every member, so mmp_group_advance(g, n) steps all of them
+ * with one graph launch per 16 steps (and one per remaining step). It is mmp_advance(member, n)
+ * for every member, bit for bit. The members keep their own calls; destroy the group before them. */
+typedef struct mmp_group mmp_group;
+MMP_API mmp_status mmp_group_create(mmp_ctx* ctx, uint32_t n_solvers, mmp_solver* const* solvers, mmp_group** out);
+MMP_API mmp_status mmp_group_advance(mmp_group* group, uint32_t n_steps);
+MMP_API void mmp_group_destroy(mmp_group* group);
 
 /* Replaces computeLawsonRK(nz) (include/MultimodePropagation_GPU.cuh:143;
  * semantics of src/MultimodePropagation_CPU.cpp:24-34): restarts at z = 0

@@ -85,7 +85,7 @@ class MmpInfo(ctypes.Structure):
         ("n_fibers", c_uint32), ("live", c_uint32),
         ("gather_waves", c_uint32), ("chunk_lines", c_uint32), ("tuned", c_uint32),
         ("pair_columns", c_uint32), ("pair_groups", c_uint32),
-        ("gather_hi_occupancy", c_uint32), ("reserved", c_uint32),
+        ("gather_hi_occupancy", c_uint32), ("halted", c_uint32),
     ]
 
 
@@ -142,6 +142,9 @@ EXPORTS = [
     ("mmp_get_result", c_int, [c_void_p, POINTER(c_double)]),
     ("mmp_batch_create", c_int, [c_void_p, c_uint32, c_void_p, c_void_p, POINTER(c_void_p)]),
     ("mmp_ctx_query", c_int, [c_void_p]),
+    ("mmp_group_create", c_int, [c_void_p, c_uint32, POINTER(c_void_p), POINTER(c_void_p)]),
+    ("mmp_group_advance", c_int, [c_void_p, c_uint32]),
+    ("mmp_group_destroy", None, [c_void_p]),
     ("mmp_solve_static", c_int, [c_void_p, POINTER(MmpParams), POINTER(MmpOptions), c_uint32, POINTER(c_double)]),
     ("mmp_fiber_result", c_int, [c_void_p, c_uint32, POINTER(c_double)]),
     ("mmp_fiber_state", c_int, [c_void_p, c_uint32, POINTER(c_double), POINTER(c_double)]),

@@ -119,3 +119,31 @@ class MultimodePropagationBatch:
             self.close()
         except Exception:
             pass
+
+
+class SolverGroup:
+    """Solvers and batches of different shapes on one context, stepped together by one HIP graph
+    (mmp_group_*): advance(n) equals advance(n) on every member, bit for bit."""
+
+    def __init__(self, members: Sequence, ctx: Context):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.members = list(members)
+        arr = (ctypes.c_void_p * len(self.members))(*[m.h.value for m in self.members])
+        h = ctypes.c_void_p()
+        _capi.check(self.lib.mmp_group_create(ctx.h, len(self.members), arr, ctypes.byref(h)), "mmp_group_create")
+        self.h = h
+
+    def advance(self, n: int) -> None:
+        _capi.check(self.lib.mmp_group_advance(self.h, int(n)), "mmp_group_advance")
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.mmp_group_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

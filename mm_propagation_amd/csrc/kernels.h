@@ -61,10 +61,12 @@ struct TwOf<float> {
 };
 
 // Per-step device state updated on the stream (z by repeated FP64 addition).
+// halted: set by the step-closing kernel when the fiber's new spectral state is not finite; it
+// then also parks the fiber (FiberParams::live = 0), so later kernels skip it.
 struct StepState {
   double z;
   uint32_t steps;
-  uint32_t pad;
+  uint32_t halted;
 };
 
 // Per-fiber scalars of a batch of fibers that share M, NT, the tableau and the pair structure.
@@ -83,7 +85,7 @@ struct FiberParams {
 constexpr int kHeadCoef = kMaxStages + 1;
 
 struct Batch {
-  const FiberParams* fp;  // [F]
+  FiberParams* fp;        // [F] (the step-closing kernel parks a fiber whose state is not finite)
   const double* coef;     // [F][nheads][kHeadCoef] stage-head coefficients times the fiber's h
   size_t nheads;          // s + 1
   StepState* st;          // [F]

@@ -162,6 +162,7 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_row_head(Geometry g, con
     const double bh = hc[0];
     const double cprev = prev_term ? hc[a.n_a] : 0.;
     line_fft<L, -1, 1, 1>(v, sm, lt, t);
+    bool finite = true;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
       const size_t idx = base + i * C::T;
@@ -169,6 +170,7 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_row_head(Geometry g, con
       const cd nh = {-y.im * fp.sgn, y.re * fp.sgn};  // * (i sgn) = N_{i-1}
       cd acc = accb[i];
       if (a.b_prev != 0.) acc = cadd(acc, cscale(nh, bh));
+      finite = finite && __builtin_isfinite(acc.re) && __builtin_isfinite(acc.im);
       if (valid) {
         if (a.final_stage) store_stream(psi_hat + idx, acc);
         else store_stream(accp + idx, acc);
@@ -176,6 +178,12 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_row_head(Geometry g, con
       }
       if (a.final_stage) x[i] = acc;  // chain_next: stage 0 of the next step from the new psi_hat
       else if (prev_term) x[i] = cadd(x[i], cscale(nh, cprev));
+    }
+    // a state that is no longer finite stays so: the fiber is parked (every lane that sees it
+    // stores the same values), its later kernels skip it and its z and step count stop here
+    if (a.final_stage && valid && !finite) {
+      b.st[fib].halted = 1u;
+      b.fp[fib].live = 0u;
     }
   }
   if (a.final_stage && !a.chain_next) return;

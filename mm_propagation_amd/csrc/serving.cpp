@@ -10,7 +10,9 @@
 // (stream). A group advances by the fewest steps any of its live requests still needs, capped
 // at max_chunk, so no request takes a step too many; a request's result is therefore the
 // result of the same solve alone (bit for bit; to rounding with MMP_FLAG_AUTOTUNE, whose gather
-// variant for a group's slot count may sum over the pair slots in another order).
+// variant for a group's slot count may sum over the pair slots in another order). A request whose
+// field stops being finite is parked by the device at that step and ends at the next chunk
+// boundary as MMP_REQ_NONFINITE, so it holds its slot no longer than that.
 
 #include <algorithm>
 #include <chrono>
@@ -299,7 +301,13 @@ mmp_status poll_impl(mmp_server* s) {
     for (uint32_t i = 0; i < g.slot_req.size(); i++) {
       if (g.slot_req[i] < 0) continue;
       g.remaining[i] -= g.chunk;
-      if (g.remaining[i] > 0) continue;
+      if (g.remaining[i] > 0) {
+        // a field that stopped being finite was parked by the device: the request ends now
+        mmp_info fi{};
+        SRV_TRY(mmp_fiber_info(g.batch, i, &fi));
+        if (!fi.halted) continue;
+        g.remaining[i] = 0;
+      }
       SRV_TRY(finish(s, *s->reqs[(size_t)g.slot_req[i]], g.batch, i));
       g.slot_req[i] = -1;
       SRV_TRY(mmp_fiber_set_live(g.batch, i, 0));
@@ -316,6 +324,9 @@ mmp_status poll_impl(mmp_server* s) {
       SRV_TRY(q);
       o.busy = false;
       o.remaining -= o.chunk;
+      mmp_info fi{};
+      SRV_TRY(mmp_solver_info(o.solver, &fi));
+      if (fi.halted) o.remaining = 0;  // parked by the device: its field is not finite
     }
     if (o.remaining == 0) {
       SRV_TRY(finish(s, *s->reqs[o.req], o.solver, 0));

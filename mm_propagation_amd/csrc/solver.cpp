@@ -126,6 +126,7 @@ struct mmp_solver {
   const void* qdense_w = nullptr;
   std::vector<uint32_t> steps;  // per fiber, since its last restart
   std::vector<uint8_t> live;    // per fiber: advanced by the steps (1) or parked (0)
+  std::vector<uint8_t> halted;  // per fiber: parked by the device because its state stopped being finite
   bool graphs = true;
   hipGraphExec_t graph_exec = nullptr;   // kGraphSteps steps
   hipGraph_t graph = nullptr;
@@ -139,6 +140,14 @@ struct mmp_solver {
   std::vector<uint64_t> kernel_count;
   std::vector<KernelRecord> pending;
   std::vector<hipEvent_t> free_events;
+};
+
+// Solvers of different shapes on one context advanced together: one graph replay steps them all.
+struct mmp_group {
+  mmp_ctx* ctx = nullptr;
+  std::vector<mmp_solver*> members;
+  hipGraph_t graph = nullptr, graph1 = nullptr;
+  hipGraphExec_t exec = nullptr, exec1 = nullptr;
 };
 
 namespace {
@@ -385,8 +394,12 @@ mmp_status set_device(mmp_solver* s) {
 
 // initLawsonRK (src/MultimodePropagation.cpp:158-163): z = 0 and fresh Lawson tables.
 // z = 0 and the initial Lawson tables for every fiber (zeroed on the host: no memset node).
+mmp_status refresh_halted(mmp_solver* s);
+
 mmp_status reset_z(mmp_solver* s) {
-  const std::vector<StepState> zero(s->F, StepState{0., 0u, 0u});
+  MMP_TRY(refresh_halted(s));
+  std::vector<StepState> zero(s->F, StepState{0., 0u, 0u});
+  for (uint32_t f = 0; f < s->F; f++) zero[f].halted = s->halted[f];  // a non-finite field stays parked
   MMP_TRY(upload(s, s->st, zero.data(), zero.size() * sizeof(StepState)));
   HIP_TRY(hipMemcpyAsync(s->etab, s->etab0, (size_t)s->F * s->ntab * s->P.M * s->P.N * sizeof(cd),
                          hipMemcpyDeviceToDevice, s->ctx->stream),
@@ -394,6 +407,23 @@ mmp_status reset_z(mmp_solver* s) {
   HIP_TRY(hipStreamSynchronize(s->ctx->stream), "hipStreamSynchronize");  // `zero` is a host staging copy
   std::fill(s->z_host.begin(), s->z_host.end(), 0.);
   std::fill(s->steps.begin(), s->steps.end(), 0u);
+  return MMP_OK;
+}
+
+// The device parks a fiber whose state stops being finite (StepState::halted, FiberParams::live):
+// bring the host's view of live, z and the step count in line with it. Waits for the stream.
+mmp_status refresh_halted(mmp_solver* s) {
+  std::vector<StepState> st(s->F);
+  HIP_TRY(hipMemcpyAsync(st.data(), s->st, s->F * sizeof(StepState), hipMemcpyDeviceToHost, s->ctx->stream),
+          "hipMemcpyAsync D2H");
+  HIP_TRY(hipStreamSynchronize(s->ctx->stream), "hipStreamSynchronize");
+  for (uint32_t f = 0; f < s->F; f++)
+    if (st[f].halted && !s->halted[f]) {
+      s->halted[f] = 1;
+      s->live[f] = 0;
+      s->z_host[f] = st[f].z;
+      s->steps[f] = st[f].steps;
+    }
   return MMP_OK;
 }
 
@@ -582,6 +612,7 @@ mmp_status load_fiber(mmp_solver* s, uint32_t f, Problem& Pf) {
   s->z_host[f] = 0.;
   s->steps[f] = 0;
   s->live[f] = 1;
+  s->halted[f] = 0;
   for (auto* v : {&Pf.imL_fourstep, &Pf.e1, &Pf.etab0, &Pf.psi0, &Pf.raman_response}) std::vector<double>().swap(*v);
   return MMP_OK;
 }
@@ -624,6 +655,7 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
   s->z_host.assign(n_fibers, 0.);
   s->steps.assign(n_fibers, 0u);
   s->live.assign(n_fibers, 1);
+  s->halted.assign(n_fibers, 0);
   s->P = s->fib[0];  // structure of the batch (fiber 0's large arrays are uploaded from s->fib[0])
   for (auto* v : {&s->P.imL_fourstep, &s->P.e1, &s->P.etab0, &s->P.psi0, &s->P.raman_response})
     std::vector<double>().swap(*v);
@@ -872,40 +904,137 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
   return MMP_OK;
 }
 
-// Column-stage gather variant: `waves` wavefronts per workgroup, at most `lines` Raman lines per
-// LDS chunk. The graphs are captured again.
-mmp_status apply_colstage_plan(mmp_solver* s, int waves, int lines) {
+// The tunable shape of the column-stage step: the gather's wavefronts per workgroup, Raman lines
+// per LDS chunk and three-workgroups-per-CU instance; the pair pass's neighbouring columns per
+// workgroup and line groups per column block (the column DFTs' work layout); and whether the
+// Raman lines travel in column order into their row pass. Only the gather variants change the
+// order of a sum (over the pair slots); the other choices leave every result bit-identical.
+struct ColPlan {
+  int waves, lines, hi, columns, groups, conv_cols;
+  bool operator==(const ColPlan& o) const {
+    return waves == o.waves && lines == o.lines && hi == o.hi && columns == o.columns && groups == o.groups &&
+           conv_cols == o.conv_cols;
+  }
+};
+
+ColPlan current_plan(const mmp_solver* s) {
+  return ColPlan{s->fused.waves, s->chunk_lines, s->fused.hi_occupancy, s->pair_columns, s->pair_groups,
+                 s->conv_cols};
+}
+
+bool plan_valid(const mmp_solver* s, const ColPlan& c) {
+  const int M = (int)s->P.M;
+  if (c.waves != 4 && !(c.waves == 8 && colstage_wide_ok(M, s->mixed))) return false;
+  if (c.hi && !(c.waves == 4 && colstage_hi_occupancy_ok(M, s->mixed, s->fused.parity != 0))) return false;
+  if (c.columns != 1 && c.columns != 2 && c.columns != 4) return false;
+  if (s->g.N2 % c.columns || colstage_pairs_lds_bytes(M, c.columns, s->mixed) > kColStagePairsLds) return false;
+  if (c.groups < 1 || c.groups > s->n_lines || (c.groups & (c.groups - 1))) return false;
+  return c.conv_cols == 0 || c.conv_cols == 1;
+}
+
+// Installs plan c (chunk tables uploaded); the graphs are captured again.
+mmp_status apply_colstage_plan(mmp_solver* s, const ColPlan& c) {
   if (!s->use_fused) return fail(MMP_ERR_UNSUPPORTED, "no column-stage kernels in this solver");
-  if (waves != 4 && !(waves == 8 && colstage_wide_ok((int)s->P.M, s->mixed)))
-    return fail(MMP_ERR_INVALID_ARGUMENT, "gather wavefronts must be 4, or 8 where that variant exists");
+  if (!plan_valid(s, c)) return fail(MMP_ERR_INVALID_ARGUMENT, "no such column-stage variant for this problem");
   Problem& P = s->P;
-  if (!plan_colstage_chunks(P, lines)) return fail(MMP_ERR_INVALID_ARGUMENT, "no chunk plan for these limits");
+  if (!plan_colstage_chunks(P, c.lines)) return fail(MMP_ERR_INVALID_ARGUMENT, "no chunk plan for these limits");
   MMP_TRY(upload(s, const_cast<uint32_t*>(s->fused.chunk_line), P.fused_line.data(), P.fused_line.size() * 4));
   MMP_TRY(upload(s, const_cast<uint32_t*>(s->fused.chunk_slot), P.fused_slot.data(), P.fused_slot.size() * 4));
   HIP_TRY(hipStreamSynchronize(s->ctx->stream), "hipStreamSynchronize");  // host staging copies
   s->fused.n_chunks = P.fused_chunks;
   s->fused.max_chunk_lines = P.fused_max_lines;
   s->fused.max_chunk_slots = P.fused_max_slots;
-  s->fused.waves = waves;
-  s->chunk_lines = lines;
+  s->fused.waves = c.waves;
+  s->fused.hi_occupancy = c.hi;
+  s->chunk_lines = c.lines;
+  s->pair_columns = c.columns;
+  s->pair_groups = c.groups;
+  s->conv_cols = c.conv_cols;
   drop_graph(s);
   return MMP_OK;
 }
 
 mmp_status advance_impl(mmp_solver* s, uint32_t n);
 
-// Autotuner: every (wavefronts, lines per chunk) variant of the column-stage gather that gives a
-// distinct plan runs 2 graphs of steps on this solver's own fibers after one warm-up graph;
-// the fastest is kept, and remembered per structure (modes, samples, stages, slots, lines,
-// parity, precision, fibers) for later solvers of the process. The fibers' spectra are saved
-// first and restored after, and z, steps and Lawson tables are reset, so the solver ends as
+// Autotuner: the column-stage plan is chosen in three rounds, each keeping the best of the rounds
+// before: the gather variants (wavefronts, lines per chunk that give distinct chunk plans, the
+// three-workgroups-per-CU instance), then the pair pass's columns per workgroup and line groups,
+// then the line order into the Raman row pass. Every candidate runs one warm-up graph and two
+// timed graphs of steps on this solver's own fibers (HIP events); the fastest is kept and
+// remembered per structure (modes, samples, stages, slots, lines, parity, precision, fibers) for
+// later solvers of the process. The fibers' spectra are saved first and restored after; z, steps,
+// Lawson tables and any fiber the device parked during the runs are reset, so the solver ends as
 // created.
 struct TuneKey {
   uint32_t v[8];
   bool operator<(const TuneKey& o) const { return std::lexicographical_compare(v, v + 8, o.v, o.v + 8); }
 };
 std::mutex g_tune_mutex;
-std::map<TuneKey, std::pair<int, int>> g_tune_cache;
+std::map<TuneKey, ColPlan> g_tune_cache;
+
+mmp_status time_plan(mmp_solver* s, const ColPlan& c, hipEvent_t e0, hipEvent_t e1, float& ms) {
+  MMP_TRY(apply_colstage_plan(s, c));
+  MMP_TRY(advance_impl(s, kGraphSteps));  // warm-up, graph build
+  HIP_TRY(hipEventRecord(e0, s->ctx->stream), "hipEventRecord");
+  MMP_TRY(advance_impl(s, 2 * kGraphSteps));
+  HIP_TRY(hipEventRecord(e1, s->ctx->stream), "hipEventRecord");
+  HIP_TRY(hipEventSynchronize(e1), "hipEventSynchronize");
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+  return MMP_OK;
+}
+
+mmp_status tune_rounds(mmp_solver* s, hipEvent_t e0, hipEvent_t e1, ColPlan& best) {
+  const Problem& P = s->P;
+  const int M = (int)P.M;
+  best = current_plan(s);
+  float best_ms = 0.f;
+  MMP_TRY(time_plan(s, best, e0, e1, best_ms));
+  auto consider = [&](const ColPlan& c) -> mmp_status {
+    if (!plan_valid(s, c) || c == best) return MMP_OK;
+    float ms = 0.f;
+    MMP_TRY(time_plan(s, c, e0, e1, ms));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = c;
+    }
+    return MMP_OK;
+  };
+  // round 1: gather variants (lines per chunk only where they give distinct chunk plans)
+  std::vector<std::vector<uint32_t>> seen;
+  std::vector<int> lines_opts;
+  for (int lines : {colstage_lines_per_chunk(M), 24, 16}) {
+    if (lines > colstage_lines_per_chunk(M)) continue;
+    Problem probe = P;
+    if (!plan_colstage_chunks(probe, lines)) continue;
+    if (std::find(seen.begin(), seen.end(), probe.fused_line) != seen.end()) continue;
+    seen.push_back(probe.fused_line);
+    lines_opts.push_back(lines);
+  }
+  const ColPlan base1 = best;
+  for (int waves : {4, 8})
+    for (int lines : lines_opts)
+      for (int hi : {0, 1}) {
+        ColPlan c = base1;
+        c.waves = waves;
+        c.lines = lines;
+        c.hi = hi;
+        MMP_TRY(consider(c));
+      }
+  // round 2: the pair pass's columns per workgroup and line groups
+  const ColPlan base2 = best;
+  for (int columns : {4, 2, 1})
+    for (int groups = 1; groups <= 16 && (groups == 1 || groups * 4 <= s->n_lines); groups *= 2) {
+      ColPlan c = base2;
+      c.columns = columns;
+      c.groups = groups;
+      MMP_TRY(consider(c));
+    }
+  // round 3: the Raman lines' order into their row pass
+  ColPlan c = best;
+  c.conv_cols = 1 - c.conv_cols;
+  MMP_TRY(consider(c));
+  return MMP_OK;
+}
 
 mmp_status autotune(mmp_solver* s) {
   const Problem& P = s->P;
@@ -915,12 +1044,13 @@ mmp_status autotune(mmp_solver* s) {
     std::lock_guard<std::mutex> lock(g_tune_mutex);
     auto it = g_tune_cache.find(key);
     if (it != g_tune_cache.end()) {
-      MMP_TRY(apply_colstage_plan(s, it->second.first, it->second.second));
+      MMP_TRY(apply_colstage_plan(s, it->second));
       s->tuned = true;
       return MMP_OK;
     }
   }
   const size_t state_bytes = (size_t)s->F * s->batch.sf * sizeof(cd);
+  const std::vector<uint8_t> live_before = s->live;
   void* saved = nullptr;
   HIP_TRY(hipMalloc(&saved, state_bytes), "hipMalloc");
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -928,39 +1058,22 @@ mmp_status autotune(mmp_solver* s) {
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) st = fail(MMP_ERR_DEVICE, "hipEventCreate");
   if (st == MMP_OK && hipMemcpyAsync(saved, s->psi_hat, state_bytes, hipMemcpyDeviceToDevice, s->ctx->stream) != hipSuccess)
     st = fail(MMP_ERR_DEVICE, "hipMemcpyAsync D2D");
-  std::vector<std::pair<int, int>> cands;
-  std::vector<std::vector<uint32_t>> seen;
-  for (int waves : {4, 8}) {
-    if (waves == 8 && !colstage_wide_ok((int)P.M, s->mixed)) continue;
-    seen.clear();
-    for (int lines : {colstage_lines_per_chunk((int)P.M), 24, 16}) {
-      if (lines > colstage_lines_per_chunk((int)P.M)) continue;
-      Problem probe = P;
-      if (!plan_colstage_chunks(probe, lines)) continue;
-      if (std::find(seen.begin(), seen.end(), probe.fused_line) != seen.end()) continue;  // same plan
-      seen.push_back(probe.fused_line);
-      cands.push_back({waves, lines});
-    }
-  }
-  std::pair<int, int> best{s->fused.waves, s->chunk_lines};
-  float best_ms = 0.f;
-  for (size_t c = 0; c < cands.size() && st == MMP_OK; c++) {
-    st = apply_colstage_plan(s, cands[c].first, cands[c].second);
-    if (st == MMP_OK) st = advance_impl(s, kGraphSteps);  // warm-up, graph build
-    if (st == MMP_OK && hipEventRecord(e0, s->ctx->stream) != hipSuccess) st = fail(MMP_ERR_DEVICE, "hipEventRecord");
-    if (st == MMP_OK) st = advance_impl(s, 2 * kGraphSteps);
-    if (st == MMP_OK && hipEventRecord(e1, s->ctx->stream) != hipSuccess) st = fail(MMP_ERR_DEVICE, "hipEventRecord");
-    float ms = 0.f;
-    if (st == MMP_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-      st = fail(MMP_ERR_DEVICE, "hipEventElapsedTime");
-    if (st == MMP_OK && (c == 0 || ms < best_ms)) {
-      best_ms = ms;
-      best = cands[c];
-    }
-  }
-  if (st == MMP_OK) st = apply_colstage_plan(s, best.first, best.second);
+  ColPlan best = current_plan(s);
+  if (st == MMP_OK) st = tune_rounds(s, e0, e1, best);
+  if (st == MMP_OK) st = apply_colstage_plan(s, best);
   if (st == MMP_OK && hipMemcpyAsync(s->psi_hat, saved, state_bytes, hipMemcpyDeviceToDevice, s->ctx->stream) != hipSuccess)
     st = fail(MMP_ERR_DEVICE, "hipMemcpyAsync D2D");
+  // fibers the device parked during the runs get their live flag back (their state is restored)
+  if (st == MMP_OK) st = refresh_halted(s);
+  for (uint32_t f = 0; f < s->F && st == MMP_OK; f++) {
+    if (!s->halted[f]) continue;
+    const uint32_t v = live_before[f];
+    st = upload(s, reinterpret_cast<char*>(s->d_fp + f) + offsetof(FiberParams, live), &v, sizeof v);
+    static const StepState zero{0., 0u, 0u};  // clears StepState::halted before reset_z reads it
+    if (st == MMP_OK) st = upload(s, s->st + f, &zero, sizeof zero);
+    s->halted[f] = 0;
+    s->live[f] = live_before[f];
+  }
   if (st == MMP_OK) st = reset_z(s);  // synchronises the stream before `saved` is freed
   if (st == MMP_OK) st = drain_profile(s);
   if (st == MMP_OK) {
@@ -979,6 +1092,16 @@ mmp_status autotune(mmp_solver* s) {
   return MMP_OK;
 }
 
+// Host bookkeeping of n steps: z += h (src/MultimodePropagation_CPU.cpp:32) and the step count of
+// every live fiber.
+void account_steps(mmp_solver* s, uint32_t n) {
+  for (uint32_t f = 0; f < s->F; f++) {
+    if (!s->live[f]) continue;
+    for (uint32_t k = 0; k < n; k++) s->z_host[f] += s->fib[f].delta_z;
+    s->steps[f] += n;
+  }
+}
+
 mmp_status advance_impl(mmp_solver* s, uint32_t n) {
   MMP_TRY(set_device(s));
   if (n > 0 && s->chain) MMP_TRY(enqueue_head(s, 0));  // stage 0 of the first step
@@ -990,12 +1113,49 @@ mmp_status advance_impl(mmp_solver* s, uint32_t n) {
     for (; left > 0; left--) HIP_TRY(hipGraphLaunch(s->graph1_exec, s->ctx->stream), "hipGraphLaunch");
   }
   for (uint32_t k = 0; k < left; k++) MMP_TRY(enqueue_step(s));
-  for (uint32_t f = 0; f < s->F; f++) {
-    if (!s->live[f]) continue;
-    for (uint32_t k = 0; k < n; k++) s->z_host[f] += s->fib[f].delta_z;  // z += h (src/MultimodePropagation_CPU.cpp:32)
-    s->steps[f] += n;
-  }
+  account_steps(s, n);
   return MMP_OK;
+}
+
+// One graph of `steps` steps of every member of the group, members interleaved step by step.
+mmp_status capture_group(mmp_group* g, int steps, hipGraph_t* graph_out, hipGraphExec_t* exec_out) {
+  hipStream_t st = g->ctx->stream;
+  HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  mmp_status status = MMP_OK;
+  for (int k = 0; k < steps && status == MMP_OK; k++)
+    for (mmp_solver* s : g->members)
+      if (status == MMP_OK) status = enqueue_step(s);
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(st, &graph);
+  if (status != MMP_OK) {
+    if (graph) hipGraphDestroy(graph);
+    return status;
+  }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e == hipSuccess) e = hipGraphUpload(exec, st);
+  if (e != hipSuccess) {
+    if (exec) hipGraphExecDestroy(exec);
+    hipGraphDestroy(graph);
+    return hip_fail(e, "hipGraphInstantiate");
+  }
+  *graph_out = graph;
+  *exec_out = exec;
+  return MMP_OK;
+}
+
+void group_release(mmp_group* g) {
+  if (!g) return;
+  if (g->ctx) {
+    hipSetDevice(g->ctx->device);
+    hipStreamSynchronize(g->ctx->stream);
+  }
+  if (g->exec) hipGraphExecDestroy(g->exec);
+  if (g->graph) hipGraphDestroy(g->graph);
+  if (g->exec1) hipGraphExecDestroy(g->exec1);
+  if (g->graph1) hipGraphDestroy(g->graph1);
+  delete g;
 }
 
 mmp_status check_fiber(mmp_solver* s, uint32_t f) {
@@ -1109,6 +1269,53 @@ mmp_status mmp_batch_create(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* p
 
 void mmp_solver_destroy(mmp_solver* s) { release(s); }
 
+mmp_status mmp_group_create(mmp_ctx* ctx, uint32_t n, mmp_solver* const* solvers, mmp_group** out) {
+  if (!ctx || !solvers || !out || n < 1) return fail(MMP_ERR_INVALID_ARGUMENT, "ctx, solvers (n >= 1) and out are required");
+  *out = nullptr;
+  for (uint32_t i = 0; i < n; i++) {
+    const mmp_solver* s = solvers[i];
+    if (!s) return fail(MMP_ERR_INVALID_ARGUMENT, "a solver of the group is NULL");
+    if (s->ctx != ctx) return fail(MMP_ERR_INVALID_ARGUMENT, "every solver of a group must be made on the group's context");
+    if (!s->graphs) return fail(MMP_ERR_UNSUPPORTED, "a solver of the group launches kernel by kernel (profiling, "
+                                                      "MMP_FLAG_NO_GRAPH or no graph support)");
+    for (uint32_t j = 0; j < i; j++)
+      if (solvers[j] == s) return fail(MMP_ERR_INVALID_ARGUMENT, "a solver appears twice in the group");
+  }
+  try {
+    mmp_group* g = new mmp_group();
+    g->ctx = ctx;
+    g->members.assign(solvers, solvers + n);
+    hipError_t e = hipSetDevice(ctx->device);
+    mmp_status st = e == hipSuccess ? MMP_OK : hip_fail(e, "hipSetDevice");
+    if (st == MMP_OK) st = capture_group(g, kGraphSteps, &g->graph, &g->exec);
+    if (st == MMP_OK) st = capture_group(g, 1, &g->graph1, &g->exec1);
+    if (st != MMP_OK) {
+      group_release(g);
+      return st;
+    }
+    *out = g;
+    return MMP_OK;
+  } catch (const std::bad_alloc&) {
+    return fail(MMP_ERR_OUT_OF_MEMORY, "host allocation failed");
+  }
+}
+
+mmp_status mmp_group_advance(mmp_group* g, uint32_t n) {
+  if (!g) return fail(MMP_ERR_INVALID_ARGUMENT, "group is NULL");
+  HIP_TRY(hipSetDevice(g->ctx->device), "hipSetDevice");
+  if (n == 0) return MMP_OK;
+  for (mmp_solver* s : g->members)
+    if (s->chain) MMP_TRY(enqueue_head(s, 0));  // stage 0 of each member's first step
+  uint32_t left = n;
+  for (; left >= (uint32_t)kGraphSteps; left -= kGraphSteps)
+    HIP_TRY(hipGraphLaunch(g->exec, g->ctx->stream), "hipGraphLaunch");
+  for (; left > 0; left--) HIP_TRY(hipGraphLaunch(g->exec1, g->ctx->stream), "hipGraphLaunch");
+  for (mmp_solver* s : g->members) account_steps(s, n);
+  return MMP_OK;
+}
+
+void mmp_group_destroy(mmp_group* g) { group_release(g); }
+
 // One solve whose device arrays all live in the device's static scratch (no allocation). Calls are
 // serialised by this lock: the scratch is one array per device, and a call ends only when its
 // result has been copied to the host.
@@ -1205,6 +1412,7 @@ mmp_status mmp_get_state(mmp_solver* s, double* psi_out, double* z_out) { return
 mmp_status mmp_fiber_state(mmp_solver* s, uint32_t f, double* psi_out, double* z_out) {
   MMP_TRY(check_fiber(s, f));
   MMP_TRY(set_device(s));
+  MMP_TRY(refresh_halted(s));
   if (z_out) *z_out = s->z_host[f];
   if (psi_out) {
     MMP_TRY(inverse_state_to_U(s, f, false));
@@ -1222,9 +1430,16 @@ mmp_status mmp_fiber_set_state(mmp_solver* s, uint32_t f, const double* psi) {
   for (size_t i = 0; i < n; i++)
     if (!std::isfinite(psi[i])) return fail(MMP_ERR_NONFINITE, "psi is not finite");
   MMP_TRY(set_device(s));
+  MMP_TRY(refresh_halted(s));
   MMP_TRY(upload(s, s->U, psi, n * 8));
   MMP_TRY(forward_state_from_U(s, f));
-  MMP_TRY(reset_fiber(s, f));
+  MMP_TRY(reset_fiber(s, f));  // clears StepState::halted
+  if (s->halted[f]) {  // a finite field again: the fiber is live again
+    const uint32_t v = 1;
+    MMP_TRY(upload(s, reinterpret_cast<char*>(s->d_fp + f) + offsetof(FiberParams, live), &v, sizeof v));
+    s->halted[f] = 0;
+    s->live[f] = 1;
+  }
   HIP_TRY(hipStreamSynchronize(s->ctx->stream), "hipStreamSynchronize");
   return MMP_OK;
 }
@@ -1255,6 +1470,10 @@ mmp_status mmp_fiber_load(mmp_solver* s, uint32_t f, const mmp_params* params) {
 mmp_status mmp_fiber_set_live(mmp_solver* s, uint32_t f, uint32_t live) {
   MMP_TRY(check_fiber(s, f));
   if (live > 1) return fail(MMP_ERR_INVALID_ARGUMENT, "live must be 0 or 1");
+  MMP_TRY(set_device(s));
+  MMP_TRY(refresh_halted(s));
+  if (live && s->halted[f])
+    return fail(MMP_ERR_NONFINITE, "the fiber's field is not finite (load a new problem or set a new state)");
   if (s->live[f] == live) return MMP_OK;
   // only the column-stage step is made of kernels that all honour the flag
   if (!s->use_fused) return fail(MMP_ERR_UNSUPPORTED, "parking a fiber needs the column-stage kernels");
@@ -1325,6 +1544,8 @@ mmp_status mmp_solver_info(mmp_solver* s, mmp_info* o) { return mmp_fiber_info(s
 mmp_status mmp_fiber_info(mmp_solver* s, uint32_t f, mmp_info* o) {
   MMP_TRY(check_fiber(s, f));
   if (!o) return fail(MMP_ERR_INVALID_ARGUMENT, "info_out is required");
+  MMP_TRY(set_device(s));
+  MMP_TRY(refresh_halted(s));
   fill_info(s->fib[f], s->opt, o);
   o->steps_taken = s->steps[f];
   o->z = s->z_host[f];
@@ -1337,6 +1558,7 @@ mmp_status mmp_fiber_info(mmp_solver* s, uint32_t f, mmp_info* o) {
   o->pair_columns = s->use_fused ? (uint32_t)s->pair_columns : 0u;
   o->pair_groups = s->use_fused ? (uint32_t)s->pair_groups : 0u;
   o->gather_hi_occupancy = s->use_fused && s->fused.waves == 4 && s->fused.hi_occupancy ? 1u : 0u;
+  o->halted = s->halted[f];
   return MMP_OK;
 }
 

@@ -1,7 +1,9 @@
-"""MMP_FLAG_AUTOTUNE: the tuner times the column-stage gather variants on the solver's own data,
-keeps the fastest, and leaves the solver exactly as created (z = 0, the initial field); a later
-solver of the same structure takes the cached choice. Tuned results stay within rounding of the
-oracle (the variants differ only in the order of the sums over slots)."""
+"""MMP_FLAG_AUTOTUNE: the tuner times the column-stage variants (gather wavefronts, lines per
+chunk and instance; pair-pass columns per workgroup and line groups; the line order into the Raman
+row pass) on the solver's own data, keeps the fastest, and leaves the solver exactly as created
+(z = 0, the initial field, no fiber parked); a later solver of the same structure takes the cached
+choice. Tuned results stay within rounding of the oracle (only the gather variants change an
+order of summation, over the pair slots)."""
 import numpy as np
 import pytest
 
@@ -33,7 +35,8 @@ def test_tuned_solver_starts_as_created_and_matches_the_oracle():
     # the second solver of this structure takes the cached choice: the same arithmetic, bit for bit
     b = MultimodePropagationGPU_RamanON(sp, pair_decode=1, flags=_capi.MMP_FLAG_AUTOTUNE)
     ib = b.info()
-    assert (ib["gather_waves"], ib["chunk_lines"]) == (info["gather_waves"], info["chunk_lines"])
+    keys = ("gather_waves", "chunk_lines", "gather_hi_occupancy", "pair_columns", "pair_groups")
+    assert [ib[k] for k in keys] == [info[k] for k in keys]
     b.computeLawsonRK(20)
     assert np.array_equal(b.getResult(), ra)
     b.close()
@@ -65,3 +68,20 @@ def test_tuned_batch_matches_untuned_and_takes_new_fibers(precision):
         assert np.all(err <= tol), (f, err)
     t.close()
     u.close()
+
+
+def test_tuning_leaves_no_fiber_parked():
+    """A fiber whose field overflows during the tuning runs (1-10 MW pulses) is parked by the device
+    there; the tuner restores its state and live flag, and it overflows again only when stepped."""
+    require_device()
+    sps = [configs.grin_fiber(M=3, nt=4096, n_steps=20000, seed=61, symmetric=True),
+           configs.grin_fiber(M=3, nt=4096, n_steps=20000, fiber_length=1.0, seed=5, power_range=(1e6, 1e7),
+                              symmetric=True)]
+    note("case autotune with an overflowing fiber")
+    b = MultimodePropagationBatch(sps, precision=_capi.MMP_PRECISION_MIXED, flags=_capi.MMP_FLAG_AUTOTUNE)
+    for f in range(2):
+        i = b.info(f)
+        assert (i["halted"], i["live"], i["steps_taken"], i["z"]) == (0, 1, 0, 0.0), (f, i)
+    b.computeLawsonRK(8)
+    assert b.info(1)["halted"] == 1 and b.info(0)["halted"] == 0
+    b.close()

@@ -143,3 +143,98 @@ def test_batch_rejects_mixed_structures_and_ineligible_shapes():
     with pytest.raises(_capi.MmpError) as ei:
         MultimodePropagationBatch([configs.grin_fiber(M=3, nt=2048, symmetric=True)] * 2, pair_decode=1)
     assert ei.value.status == _capi.MMP_ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("precision", [_capi.MMP_PRECISION_FP64, _capi.MMP_PRECISION_MIXED])
+def test_nonfinite_fiber_is_parked_by_the_device(precision):
+    """A fiber whose pulses (1-10 MW) are far beyond what its step resolves overflows within a
+    few steps: the step-closing kernel parks it (its z and step count stop, later steps skip it),
+    the other fibers' results are those of their solves alone, resuming it is refused, and a new
+    finite state makes it live again."""
+    require_device()
+    sps = fibers(3, 4096, 3, base=60)
+    sps[1] = configs.grin_fiber(M=3, nt=4096, n_steps=20000, fiber_length=1.0, seed=5, power_range=(1e6, 1e7),
+                                symmetric=True)
+    note(f"case device parking precision={precision}")
+    b = MultimodePropagationBatch(sps, precision=precision)
+    psi1, _ = b.state(1)
+    b.computeLawsonRK(32)
+    i0, i1 = b.info(0), b.info(1)
+    assert i1["halted"] == 1 and i1["live"] == 0 and 1 <= i1["steps_taken"] < 32, i1
+    assert i0["halted"] == 0 and i0["steps_taken"] == 32
+    assert b.state(1)[1] < b.state(0)[1]  # its z stopped
+    with pytest.raises(_capi.MmpError) as ei:
+        b.result(1)
+    assert ei.value.status == _capi.MMP_ERR_NONFINITE
+    with pytest.raises(_capi.MmpError) as ei:
+        _capi.check(b.lib.mmp_fiber_set_live(b.h, 1, 1), "set_live")
+    assert ei.value.status == _capi.MMP_ERR_NONFINITE
+    got = [b.result(f) for f in (0, 2)]
+    for f, r in zip((0, 2), got):
+        assert np.array_equal(r, single(sps[f], 32, precision)), f
+    # a finite state again: live, restarted at z = 0, and it steps with the others
+    b.set_state(1, psi1 * 1e-3)
+    i1 = b.info(1)
+    assert i1["halted"] == 0 and i1["live"] == 1 and i1["steps_taken"] == 0
+    b.advance(2)
+    assert b.info(1)["steps_taken"] == 2
+    b.close()
+
+
+def test_single_solver_parks_a_nonfinite_field():
+    """The same on a single solver: getResult reports MMP_ERR_NONFINITE and z stops at the step
+    that overflowed (the reference would keep adding h to a NaN field)."""
+    require_device()
+    sp = configs.grin_fiber(M=3, nt=4096, n_steps=20000, fiber_length=1.0, seed=5, power_range=(1e6, 1e7),
+                            symmetric=True)
+    note("case single solver parking")
+    s = MultimodePropagationGPU_RamanON(sp)
+    s.computeLawsonRK(20)
+    info = s.info()
+    assert info["halted"] == 1 and info["steps_taken"] < 20
+    with pytest.raises(_capi.MmpError) as ei:
+        s.getResult()
+    assert ei.value.status == _capi.MMP_ERR_NONFINITE
+    s.close()
+
+
+def test_group_of_mixed_shapes_equals_members_stepped_alone():
+    """mmp_group: a mixed-precision batch (M = 3, NT = 4096), an FP64 batch (M = 6, NT = 8192) and a
+    single FP64 fiber (M = 2, NT = 1024) on one context, stepped by one graph per 16 steps, give
+    the results of each stepped on its own, bit for bit, and their z and step counts."""
+    from mm_propagation_amd.batch import SolverGroup
+    from mm_propagation_amd.solver import Context
+    require_device()
+    ctx = Context(0)
+    a_sps = fibers(3, 4096, 3, base=90)
+    b_sps = fibers(6, 8192, 2, base=95)
+    c_sp = configs.two_mode_gaussian(nt=1024)
+    note("case solver group of three shapes")
+
+    def make():
+        return (MultimodePropagationBatch(a_sps, precision=_capi.MMP_PRECISION_MIXED, ctx=ctx),
+                MultimodePropagationBatch(b_sps, ctx=ctx),
+                MultimodePropagationGPU_RamanON(c_sp, ctx=ctx))
+
+    members = make()
+    g = SolverGroup(members, ctx)
+    g.advance(37)
+    ctx.synchronize()
+    got = [[members[0].result(f) for f in range(3)], [members[1].result(f) for f in range(2)], members[2].getResult()]
+    zs = [members[0].info(0)["z"], members[1].info(1)["z"], members[2].info()["z"]]
+    steps = [members[0].info(2)["steps_taken"], members[1].info(0)["steps_taken"], members[2].info()["steps_taken"]]
+    g.close()
+    alone = make()
+    for m in alone:
+        m.advance(37)
+    ctx.synchronize()
+    for f in range(3):
+        assert np.array_equal(got[0][f], alone[0].result(f)), f
+    for f in range(2):
+        assert np.array_equal(got[1][f], alone[1].result(f)), f
+    assert np.array_equal(got[2], alone[2].getResult())
+    assert zs == [alone[0].info(0)["z"], alone[1].info(1)["z"], alone[2].info()["z"]]
+    assert steps == [37, 37, 37]
+    for m in members + alone:
+        m.close()
+    ctx.close()
