@@ -40,13 +40,14 @@ METRIC = "M·NT·n_steps/s at stated rel-L2 vs FP64 oracle; p50/p99 solve latenc
 HBM_PEAK_GBS = 8000.0
 VALU_PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}  # MI355X vector peaks (FP32 with packed instructions)
 TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "hbm_traffic_grin6_fp64.md")
+BATCHED_TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "hbm_traffic_batched.md")
 
 
-def measured_traffic():
-    """Bytes per launch of each kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of
-    this workload (FP64 default path), or {} when it is absent."""
+def measured_traffic(path=TRAFFIC_PROFILE):
+    """Bytes per launch of each kernel from a committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    (copy-calibrated), or {} when it is absent."""
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(path) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
                     return json.loads(line.split(":", 1)[1])
@@ -458,8 +459,8 @@ def run_batched(args) -> int:
 
     bi, (key, idx, ctx, _) = max(enumerate(batches), key=finite_units)
     fin = [idx[f] for f in range(len(idx)) if (bi, f) in finite] or idx
-    pb = MultimodePropagationBatch([fibers[i] for i in fin], precision=prec, ctx=ctx,
-                                   flags=_capi.MMP_FLAG_PROFILE | (bflags & _capi.MMP_FLAG_AUTOTUNE))
+    # untuned: a tuning run would step these fibers past the steps over which they stayed finite
+    pb = MultimodePropagationBatch([fibers[i] for i in fin], precision=prec, ctx=ctx, flags=_capi.MMP_FLAG_PROFILE)
     pb.advance(min(4, steps))
     pb.kernel_times()
     pb.advance(max(1, min(16, steps - 4)))  # within the steps over which these fibers stayed finite
@@ -489,7 +490,12 @@ def run_batched(args) -> int:
         roofline = {"bound": "hbm", "kernel": k["name"], "achieved": round(gbps, 2), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBS, 5), "traffic": None,
                     "bytes_per_launch": k["bytes_per_launch"]}
-    roofline.update({"class": f"M={key[0]} NT={key[1]} x {len(idx)} fibers", "avg_launch_us": round(avg_s * 1e6, 3),
+    cls = f"M={key[0]} NT={key[1]} x {len(idx)} fibers"
+    traffic = measured_traffic(BATCHED_TRAFFIC_PROFILE).get(f"{k['name']} {cls}") if not args.fp64_batch else None
+    if traffic:
+        roofline.update({"traffic": traffic, "traffic_source": "profiles/hbm_traffic_batched.md (rocprofv3 FETCH_SIZE "
+                                                               "x2.0 + WRITE_SIZE x1.0, copy-calibrated)"})
+    roofline.update({"class": cls, "avg_launch_us": round(avg_s * 1e6, 3),
                      "peak_note": "FP32 vector peak 157.3 TFLOP/s counts packed FP32 instructions, which this build "
                                   "does not emit (78.6 TFLOP/s without them, as FP64)"})
     out = {
@@ -527,6 +533,8 @@ def run_batched(args) -> int:
         b.close()
     for c in contexts:
         c.close()
+    if args.copy_calibration_mib > 0:
+        copy_calibration(args.copy_calibration_mib)
     return 0
 
 

@@ -150,6 +150,11 @@ MMP_API const char* mmp_last_error(void);
 /* ABI version compiled into the library (MMPROP_ABI_VERSION). */
 MMP_API int mmp_abi_version(void);
 
+/* Digest of the sources the library was built from (the first 16 hex digits of a SHA-256 over the
+ * library's source files and this header, as mm_propagation_amd/build.py computes it), so that a
+ * caller can tell a library built from other sources; "unknown" when the build did not set it. */
+MMP_API const char* mmp_source_hash(void);
+
 /* Host-only planning (no device needed, no reference counterpart): validates
  * params and options exactly as mmp_solver_create does and fills the solver
  * scalars and the device memory a solver would hold (device_bytes). Used for

@@ -132,6 +132,7 @@ EXPORTS = [
     ("mmp_status_string", c_char_p, [c_int]),
     ("mmp_last_error", c_char_p, []),
     ("mmp_abi_version", c_int, []),
+    ("mmp_source_hash", c_char_p, []),
     ("mmp_ctx_create", c_int, [c_int, POINTER(c_void_p)]),
     ("mmp_ctx_destroy", None, [c_void_p]),
     ("mmp_solver_create", c_int, [c_void_p, POINTER(MmpParams), POINTER(MmpOptions), POINTER(c_void_p)]),
@@ -188,7 +189,9 @@ class MmpError(RuntimeError):
 
 
 def load_library(path: str | None = None) -> ctypes.CDLL:
-    """Load libmmprop_hip.so (built in-tree by mm_propagation_amd.build). Raises if absent."""
+    """Load libmmprop_hip.so (built in-tree by mm_propagation_amd.build). Raises if it is absent,
+    or if it was built from other sources than the tree's (its mmp_source_hash differs from the
+    digest of the sources here)."""
     global _LIB
     if _LIB is not None and path is None:
         return _LIB
@@ -202,6 +205,11 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
         fn.restype = restype
         fn.argtypes = argtypes
     if path is None:
+        from . import build
+        built, here = lib.mmp_source_hash().decode(), build.source_hash()
+        if built != here:
+            raise RuntimeError(f"{p} was built from other sources (digest {built}, the tree's sources {here}): "
+                               "rebuild it (python -c 'import __graft_entry__ as g; g.build()')")
         _LIB = lib
     return lib
 

@@ -8,6 +8,7 @@ hipcc for gfx950 only. libmmprop.so: the C++ drop-in classes of include/mmprop/
 """
 from __future__ import annotations
 
+import hashlib
 import os
 import shutil
 import subprocess
@@ -43,17 +44,33 @@ def _run(cmd):
     return out
 
 
-def _compile_hip(src: str) -> str:
+def source_hash() -> str:
+    """SHA-256 (first 16 hex digits) of the library's sources and its public header, in a fixed
+    order: compiled into libmmprop_hip.so (mmp_source_hash), so that a library built from other
+    sources than the tree's is recognised when it is loaded."""
+    h = hashlib.sha256()
+    names = sorted(f for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp", ".h")))
+    for name in names:
+        h.update(name.encode())
+        with open(os.path.join(CSRC, name), "rb") as f:
+            h.update(f.read())
+    with open(os.path.join(REPO, "include", "mmprop.h"), "rb") as f:
+        h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def _compile_hip(src: str, extra=()) -> str:
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+    _run([HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
     return obj
 
 
 def build_hip_library() -> str:
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
+    stamp = [f'-DMMP_SOURCE_HASH="{source_hash()}"']
     with ThreadPoolExecutor(max_workers=min(4, os.cpu_count() or 1)) as ex:
-        objs = list(ex.map(_compile_hip, HIP_SOURCES))
+        objs = list(ex.map(lambda src: _compile_hip(src, stamp), HIP_SOURCES))
     out = os.path.join(LIBDIR, "libmmprop_hip.so")
     tmp = out + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])

@@ -1193,6 +1193,11 @@ const char* mmp_last_error(void) { return g_last_error.c_str(); }
 
 int mmp_abi_version(void) { return MMPROP_ABI_VERSION; }
 
+#ifndef MMP_SOURCE_HASH
+#define MMP_SOURCE_HASH "unknown"
+#endif
+const char* mmp_source_hash(void) { return MMP_SOURCE_HASH; }
+
 mmp_status mmp_ctx_create(int device, mmp_ctx** out) {
   if (!out) return fail(MMP_ERR_INVALID_ARGUMENT, "out is NULL");
   *out = nullptr;

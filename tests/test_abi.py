@@ -73,3 +73,9 @@ def test_no_device_is_an_error_not_a_fallback(capi_lib):
     from mm_propagation_amd.solver import MultimodePropagationGPU_RamanON
     with pytest.raises(_capi.MmpError):
         MultimodePropagationGPU_RamanON(configs.two_mode_gaussian(nt=64))
+
+
+def test_library_matches_the_sources(capi_lib):
+    """The loaded library carries the digest of the sources it was built from: the tree's."""
+    from mm_propagation_amd import build
+    assert capi_lib.mmp_source_hash().decode() == build.source_hash()
