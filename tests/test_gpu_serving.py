@@ -86,3 +86,41 @@ def test_idle_groups_are_reclaimed_for_other_structures():
         s.computeLawsonRK(r.n_steps)
         assert np.array_equal(r.result, s.getResult()), r.rid
         s.close()
+
+
+def test_autotuned_server_results_agree_with_solves_alone_to_rounding():
+    """With MMP_FLAG_AUTOTUNE every group runs the gather variant tuned for its slot count, which
+    may sum over the pair slots in another order than the solve alone: results agree to rounding."""
+    require_device()
+    reqs = [r for r in mix() if r.params.nt == 4096][:4]
+    note("case serving with autotune, FP64")
+    eng = ServingEngine(slots=2, max_chunk=16, precision=_capi.MMP_PRECISION_FP64, flags=_capi.MMP_FLAG_AUTOTUNE)
+    stats = eng.run(reqs)
+    eng.close()
+    assert stats["requests"] == len(reqs) and stats["nonfinite"] == 0
+    for r in reqs:
+        s = MultimodePropagationGPU_RamanON(r.params, pair_decode=_capi.MMP_PAIRS_PHYSICAL)
+        s.computeLawsonRK(r.n_steps)
+        err = configs.rel_l2_per_mode(r.result, s.getResult())
+        s.close()
+        assert np.all(err <= 1e-12), (r.rid, err)
+
+
+def test_overflowing_request_ends_early_and_frees_its_slot():
+    """A request whose pulses (1-10 MW) overflow within a few steps is parked by the device and
+    ends at the next chunk boundary as non-finite; the request that shares its group still
+    gets the result of its solve alone."""
+    require_device()
+    ok = Request(rid=0, params=configs.grin_fiber(M=3, nt=4096, n_steps=20000, seed=71, symmetric=True), n_steps=40)
+    bad = Request(rid=1, params=configs.grin_fiber(M=3, nt=4096, n_steps=20000, fiber_length=1.0, seed=5,
+                                                   power_range=(1e6, 1e7), symmetric=True), n_steps=400)
+    note("case serving overflowing request")
+    eng = ServingEngine(slots=2, max_chunk=8, precision=_capi.MMP_PRECISION_FP64)
+    stats = eng.run([ok, bad])
+    eng.close()
+    assert stats["nonfinite"] == 1 and bad.nonfinite and bad.result is None
+    assert bad.finished is not None and bad.finished < ok.finished  # at its first chunk boundary, not after 400 steps
+    s = MultimodePropagationGPU_RamanON(ok.params, pair_decode=_capi.MMP_PAIRS_PHYSICAL)
+    s.computeLawsonRK(ok.n_steps)
+    assert np.array_equal(ok.result, s.getResult())
+    s.close()
