@@ -244,11 +244,14 @@ MMP_API mmp_status mmp_solve_static(mmp_ctx* ctx, const mmp_params* params, cons
                                     double* fields_out);
 
 /* Solvers of different shapes advanced together (no reference counterpart: the reference runs one
- * fiber per object). A group holds solvers and batches of any n_modes, nt and precision, all made
- * on ctx with HIP graphs (neither MMP_FLAG_PROFILE nor MMP_FLAG_NO_GRAPH); one graph, captured at
- * mmp_group_create, holds 16 steps of every member, so mmp_group_advance(g, n) steps all of them
- * with one graph launch per 16 steps (and one per remaining step). It is mmp_advance(member, n)
- * for every member, bit for bit. The members keep their own calls; destroy the group before them. */
+ * fiber per object). A group holds solvers and batches of any n_modes, nt and precision, made on
+ * the device of ctx with HIP graphs (neither MMP_FLAG_PROFILE nor MMP_FLAG_NO_GRAPH); one graph,
+ * captured at mmp_group_create, holds 16 steps of every member, so mmp_group_advance(g, n) steps
+ * all of them with one graph launch per 16 steps (and one per remaining step) on ctx's stream.
+ * Members made on ctx run in order; members made on contexts of their own are parallel branches of
+ * the graph (forked from ctx's stream and joined back), so their kernels may overlap on the device.
+ * It is mmp_advance(member, n) for every member, bit for bit, and each member's stream is ordered
+ * after the group's work. The members keep their own calls; destroy the group before them. */
 typedef struct mmp_group mmp_group;
 MMP_API mmp_status mmp_group_create(mmp_ctx* ctx, uint32_t n_solvers, mmp_solver* const* solvers, mmp_group** out);
 MMP_API mmp_status mmp_group_advance(mmp_group* group, uint32_t n_steps);

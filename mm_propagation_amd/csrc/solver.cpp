@@ -143,9 +143,14 @@ struct mmp_solver {
 };
 
 // Solvers of different shapes on one context advanced together: one graph replay steps them all.
+// Members on the group's own context are captured in order on its stream; members on other
+// contexts (other streams of the same device) become parallel branches of the graph, forked from
+// and joined back into the group's stream, so their kernels may run concurrently.
 struct mmp_group {
   mmp_ctx* ctx = nullptr;
   std::vector<mmp_solver*> members;
+  std::vector<hipStream_t> branches;   // distinct member streams other than ctx->stream
+  std::vector<hipEvent_t> events;      // fork, then one join per branch, then one per branch for advance
   hipGraph_t graph = nullptr, graph1 = nullptr;
   hipGraphExec_t exec = nullptr, exec1 = nullptr;
 };
@@ -1122,9 +1127,23 @@ mmp_status capture_group(mmp_group* g, int steps, hipGraph_t* graph_out, hipGrap
   hipStream_t st = g->ctx->stream;
   HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
   mmp_status status = MMP_OK;
+  const size_t nb = g->branches.size();
+  // fork: every branch stream joins the capture behind this event
+  if (nb) {
+    if (hipEventRecord(g->events[0], st) != hipSuccess) status = fail(MMP_ERR_DEVICE, "hipEventRecord");
+    for (size_t b = 0; b < nb && status == MMP_OK; b++)
+      if (hipStreamWaitEvent(g->branches[b], g->events[0], 0) != hipSuccess)
+        status = fail(MMP_ERR_DEVICE, "hipStreamWaitEvent");
+  }
   for (int k = 0; k < steps && status == MMP_OK; k++)
     for (mmp_solver* s : g->members)
       if (status == MMP_OK) status = enqueue_step(s);
+  // join every branch back into the group's stream
+  for (size_t b = 0; b < nb && status == MMP_OK; b++) {
+    if (hipEventRecord(g->events[1 + b], g->branches[b]) != hipSuccess ||
+        hipStreamWaitEvent(st, g->events[1 + b], 0) != hipSuccess)
+      status = fail(MMP_ERR_DEVICE, "hipEventRecord / hipStreamWaitEvent");
+  }
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(st, &graph);
   if (status != MMP_OK) {
@@ -1155,6 +1174,7 @@ void group_release(mmp_group* g) {
   if (g->graph) hipGraphDestroy(g->graph);
   if (g->exec1) hipGraphExecDestroy(g->exec1);
   if (g->graph1) hipGraphDestroy(g->graph1);
+  for (hipEvent_t e : g->events) hipEventDestroy(e);
   delete g;
 }
 
@@ -1280,7 +1300,8 @@ mmp_status mmp_group_create(mmp_ctx* ctx, uint32_t n, mmp_solver* const* solvers
   for (uint32_t i = 0; i < n; i++) {
     const mmp_solver* s = solvers[i];
     if (!s) return fail(MMP_ERR_INVALID_ARGUMENT, "a solver of the group is NULL");
-    if (s->ctx != ctx) return fail(MMP_ERR_INVALID_ARGUMENT, "every solver of a group must be made on the group's context");
+    if (s->ctx->device != ctx->device)
+      return fail(MMP_ERR_INVALID_ARGUMENT, "every solver of a group must be on the group context's device");
     if (!s->graphs) return fail(MMP_ERR_UNSUPPORTED, "a solver of the group launches kernel by kernel (profiling, "
                                                       "MMP_FLAG_NO_GRAPH or no graph support)");
     for (uint32_t j = 0; j < i; j++)
@@ -1290,8 +1311,19 @@ mmp_status mmp_group_create(mmp_ctx* ctx, uint32_t n, mmp_solver* const* solvers
     mmp_group* g = new mmp_group();
     g->ctx = ctx;
     g->members.assign(solvers, solvers + n);
+    for (const mmp_solver* s : g->members)
+      if (s->ctx->stream != ctx->stream &&
+          std::find(g->branches.begin(), g->branches.end(), s->ctx->stream) == g->branches.end())
+        g->branches.push_back(s->ctx->stream);
     hipError_t e = hipSetDevice(ctx->device);
     mmp_status st = e == hipSuccess ? MMP_OK : hip_fail(e, "hipSetDevice");
+    // events: the fork, a join per branch (capture), and one per branch for each advance's start
+    for (size_t k = 0; k < 1 + 2 * g->branches.size() && st == MMP_OK; k++) {
+      hipEvent_t ev = nullptr;
+      e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+      if (e != hipSuccess) st = hip_fail(e, "hipEventCreate");
+      else g->events.push_back(ev);
+    }
     if (st == MMP_OK) st = capture_group(g, kGraphSteps, &g->graph, &g->exec);
     if (st == MMP_OK) st = capture_group(g, 1, &g->graph1, &g->exec1);
     if (st != MMP_OK) {
@@ -1310,11 +1342,21 @@ mmp_status mmp_group_advance(mmp_group* g, uint32_t n) {
   HIP_TRY(hipSetDevice(g->ctx->device), "hipSetDevice");
   if (n == 0) return MMP_OK;
   for (mmp_solver* s : g->members)
-    if (s->chain) MMP_TRY(enqueue_head(s, 0));  // stage 0 of each member's first step
+    if (s->chain) MMP_TRY(enqueue_head(s, 0));  // stage 0 of each member's first step, on its stream
+  hipStream_t st = g->ctx->stream;
+  const size_t nb = g->branches.size();
+  // the graph (on the group's stream) after the members' own work; the members' streams after it
+  for (size_t b = 0; b < nb; b++) {
+    HIP_TRY(hipEventRecord(g->events[1 + nb + b], g->branches[b]), "hipEventRecord");
+    HIP_TRY(hipStreamWaitEvent(st, g->events[1 + nb + b], 0), "hipStreamWaitEvent");
+  }
   uint32_t left = n;
-  for (; left >= (uint32_t)kGraphSteps; left -= kGraphSteps)
-    HIP_TRY(hipGraphLaunch(g->exec, g->ctx->stream), "hipGraphLaunch");
-  for (; left > 0; left--) HIP_TRY(hipGraphLaunch(g->exec1, g->ctx->stream), "hipGraphLaunch");
+  for (; left >= (uint32_t)kGraphSteps; left -= kGraphSteps) HIP_TRY(hipGraphLaunch(g->exec, st), "hipGraphLaunch");
+  for (; left > 0; left--) HIP_TRY(hipGraphLaunch(g->exec1, st), "hipGraphLaunch");
+  if (nb) {
+    HIP_TRY(hipEventRecord(g->events[0], st), "hipEventRecord");
+    for (size_t b = 0; b < nb; b++) HIP_TRY(hipStreamWaitEvent(g->branches[b], g->events[0], 0), "hipStreamWaitEvent");
+  }
   for (mmp_solver* s : g->members) account_steps(s, n);
   return MMP_OK;
 }

@@ -238,3 +238,51 @@ def test_group_of_mixed_shapes_equals_members_stepped_alone():
     for m in members + alone:
         m.close()
     ctx.close()
+
+
+def test_group_with_members_on_their_own_streams():
+    """mmp_group whose members are on contexts (streams) of their own: each is a parallel branch of
+    the group's graph. Group steps, a member's own steps in between and reads on the members'
+    streams give the results of the same calls on the members alone, bit for bit."""
+    from mm_propagation_amd.batch import SolverGroup
+    from mm_propagation_amd.solver import Context
+    require_device()
+    gctx, ca, cb = Context(0), Context(0), Context(0)
+    a_sps = fibers(3, 4096, 3, base=100)
+    b_sps = fibers(6, 8192, 2, base=105)
+    c_sp = configs.two_mode_gaussian(nt=1024)
+    note("case solver group with parallel branches")
+
+    def make():
+        return (MultimodePropagationBatch(a_sps, precision=_capi.MMP_PRECISION_MIXED, ctx=ca),
+                MultimodePropagationBatch(b_sps, ctx=cb),
+                MultimodePropagationGPU_RamanON(c_sp, ctx=gctx))
+
+    def results(ms):
+        return [[ms[0].result(f) for f in range(3)], [ms[1].result(f) for f in range(2)], ms[2].getResult()]
+
+    members = make()
+    g = SolverGroup(members, gctx)
+    g.advance(37)
+    members[0].advance(3)  # on its own stream, after the group's steps
+    g.advance(5)
+    got = results(members)
+    steps = [members[0].info(1)["steps_taken"], members[1].info(0)["steps_taken"], members[2].info()["steps_taken"]]
+    g.close()
+    alone = make()
+    for m in alone:
+        m.advance(37)
+    alone[0].advance(3)
+    for m in alone:
+        m.advance(5)
+    ref = results(alone)
+    for f in range(3):
+        assert np.array_equal(got[0][f], ref[0][f]), f
+    for f in range(2):
+        assert np.array_equal(got[1][f], ref[1][f]), f
+    assert np.array_equal(got[2], ref[2])
+    assert steps == [45, 42, 42]
+    for m in members + alone:
+        m.close()
+    for c in (gctx, ca, cb):
+        c.close()
