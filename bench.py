@@ -114,6 +114,9 @@ def main() -> int:
                     help="MMP_FLAG_AUTOTUNE: each solver times its column-stage variants at creation and keeps the "
                          "fastest (default for the batched workload, off for the others)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
+    ap.add_argument("--matrix-cores", choices=["default", "on", "off"], default="default",
+                    help="mixed precision: the gather's coupling sums on the matrix cores by default at 4-8 modes "
+                         "(on: MMP_FLAG_MATRIX_CORES, wherever that gather exists; off: MMP_FLAG_NO_MATRIX_CORES)")
     ap.add_argument("--separate-columns", action="store_true",
                     help="every column pass a kernel of its own (7 per stage instead of 4)")
     ap.add_argument("--mixed", action="store_true",
@@ -301,7 +304,8 @@ def run_batched(args) -> int:
     for i, sp in enumerate(fibers):
         classes.setdefault((sp.n_modes, sp.nt), []).append(i)
     autotune = True if args.autotune is None else args.autotune
-    bflags = (_capi.MMP_FLAG_AUTOTUNE if autotune else 0) | (_capi.MMP_FLAG_NO_GRAPH if args.no_graph else 0)
+    bflags = ((_capi.MMP_FLAG_AUTOTUNE if autotune else 0) | (_capi.MMP_FLAG_NO_GRAPH if args.no_graph else 0) |
+              {"default": 0, "on": _capi.MMP_FLAG_MATRIX_CORES, "off": _capi.MMP_FLAG_NO_MATRIX_CORES}[args.matrix_cores])
     one_stream = args.batch_streams == "one"
     use_group = args.batch_streams in ("one", "concurrent") and not args.no_graph
     shared = Context(0) if one_stream else None
@@ -315,7 +319,8 @@ def run_batched(args) -> int:
     plans = {f"M{k[0]}_NT{k[1]}": {"fibers": len(idx), "gather_waves": b.info(0)["gather_waves"],
                                    "chunk_lines": b.info(0)["chunk_lines"], "pair_columns": b.info(0)["pair_columns"],
                                    "pair_groups": b.info(0)["pair_groups"],
-                                   "gather_hi_occupancy": b.info(0)["gather_hi_occupancy"]}
+                                   "gather_hi_occupancy": b.info(0)["gather_hi_occupancy"],
+                                   "gather_matrix_cores": b.info(0)["gather_matrix_cores"]}
              for k, idx, _, b in batches}
     contexts = [shared] if one_stream else [ctx for _, _, ctx, _ in batches]
     psi0 = {(bi, f): b.state(f)[0] for bi, (_, idx, _, b) in enumerate(batches) for f in range(len(idx))}
@@ -371,6 +376,8 @@ def run_batched(args) -> int:
             else:
                 finite.add((bi, f))
     finite_fibers = [idx[f] for bi, (_, idx, _, _) in enumerate(batches) for f in range(len(idx)) if (bi, f) in finite]
+    for bi, (key, idx, _, _) in enumerate(batches):
+        plans[f"M{key[0]}_NT{key[1]}"]["finite_after_timed_steps"] = sum(1 for f in range(len(idx)) if (bi, f) in finite)
     units = sum(fibers[i].n_modes * fibers[i].nt for i in finite_fibers)
     value = units * steps / elapsed
 

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MMPROP_ABI_VERSION 5
+#define MMPROP_ABI_VERSION 6
 
 /* Only the entry points below are exported; the library is built with hidden visibility. */
 #if defined(__GNUC__) || defined(__clang__)
@@ -74,7 +74,18 @@ enum {
                                  workgroup, Raman lines per LDS chunk) on this solver's own data and
                                  keep the fastest; the choice is cached per structure for the process.
                                  The solver is left at z = 0 with its initial field. Results stay within
-                                 rounding of the untuned solver (the sum order over slots changes). */
+                                 rounding of the untuned solver (the sum order over slots changes). */,
+  MMP_FLAG_NO_MATRIX_CORES = 1u << 4, /* mixed precision: keep the column-stage gather's coupling sums on
+                                         the vector ALUs. By default they run as FP32 matrix products on
+                                         the matrix cores at 4 to 8 modes (the same arithmetic, another
+                                         order of FP32 rounding), and the tuner tries both. */
+  MMP_FLAG_MATRIX_CORES = 1u << 5     /* mixed precision: the coupling sums on the matrix cores wherever
+                                         that gather exists (4 to 10 modes with parity-compact coupling
+                                         blocks, 4 to 8 without), also where the default keeps them on
+                                         the vector ALUs; the tuner then tries only such variants. Not
+                                         with MMP_FLAG_NO_MATRIX_CORES. The default does not depend on
+                                         the number of fibers, so a batch and its fibers solved alone
+                                         run the same arithmetic. */
 };
 
 /*
@@ -134,6 +145,7 @@ typedef struct mmp_info {
   uint32_t pair_columns;     /* column-stage pair pass: neighbouring columns per workgroup (0: separate passes) */
   uint32_t pair_groups;      /* column-stage pair pass: workgroups sharing one column block's Raman lines */
   uint32_t gather_hi_occupancy; /* 1 when the gather runs its three-workgroups-per-CU variant */
+  uint32_t gather_matrix_cores; /* 1 when the gather's coupling sums run on the matrix cores (FP32) */
   uint32_t halted;           /* 1 when the device parked this fiber because its field stopped being finite
                                 (after the step that made it so: z and steps_taken count that step) */
 } mmp_info;

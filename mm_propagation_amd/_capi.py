@@ -40,6 +40,8 @@ MMP_FLAG_PROFILE = 1 << 0
 MMP_FLAG_NO_GRAPH = 1 << 1
 MMP_FLAG_SEPARATE_COLUMNS = 1 << 2
 MMP_FLAG_AUTOTUNE = 1 << 3
+MMP_FLAG_NO_MATRIX_CORES = 1 << 4
+MMP_FLAG_MATRIX_CORES = 1 << 5
 
 
 class MmpParams(ctypes.Structure):
@@ -85,7 +87,7 @@ class MmpInfo(ctypes.Structure):
         ("n_fibers", c_uint32), ("live", c_uint32),
         ("gather_waves", c_uint32), ("chunk_lines", c_uint32), ("tuned", c_uint32),
         ("pair_columns", c_uint32), ("pair_groups", c_uint32),
-        ("gather_hi_occupancy", c_uint32), ("halted", c_uint32),
+        ("gather_hi_occupancy", c_uint32), ("gather_matrix_cores", c_uint32), ("halted", c_uint32),
     ]
 
 

@@ -166,6 +166,8 @@ struct FusedArgs {
   int waves;                   // wavefronts of the gather workgroup: 4, or 8 (colstage_wide_ok)
   int hi_occupancy;            // with 4 wavefronts: the three-workgroups-per-CU variant where it
                                // exists (FP32, 7 to 10 modes); same arithmetic, same results
+  int matrix_cores;            // with 4 wavefronts: the slot sums on the matrix cores where that
+                               // variant exists (colstage_matrix_cores_ok); FP32 rounding differs
 };
 
 // Limits of one column-stage chunk: Raman lines, slots, and coupling coefficients
@@ -175,6 +177,9 @@ int colstage_max_modes();
 bool colstage_wide_ok(int M, bool mixed);
 // Whether the three-workgroups-per-CU 4-wavefront gather exists (FusedArgs::hi_occupancy).
 bool colstage_hi_occupancy_ok(int M, bool mixed, bool parity);
+// Whether the matrix-core gather exists (FusedArgs::matrix_cores): FP32, 4 to 12 modes with
+// parity-structured coupling blocks, 4 to 9 without.
+bool colstage_matrix_cores_ok(int M, bool mixed, bool parity);
 int colstage_lines_per_chunk(int M);
 int colstage_slots_per_chunk();
 int colstage_coefficients_per_chunk(int M, bool mixed);

@@ -902,10 +902,37 @@ template <int M, class W, bool PAR>
 constexpr bool kColStageHiOcc = std::is_same<W, float>::value && M >= 7 && (M <= 9 || (M == 10 && PAR));
 bool colstage_hi_occupancy_ok(int M, bool mixed, bool parity) { return mixed && M >= 7 && (M <= 9 || (M == 10 && parity)); }
 
+// The slot sums on the matrix cores (MF, FP32): per sample, B[p][k] = sum over slots s of
+// Qd[s][p][k] X_s is a real-by-complex matrix product with the rows (p, k) of one coupling parity
+// class, the slots of that class as the inner dimension and the samples (real and imaginary parts
+// as two products) as the columns; sigma_p = sum_k U_k B[p][k] follows per sample. One
+// v_mfma_f32_32x32x2_f32 takes 32 rows, two slots and 32 samples. Rows are ordered so that the
+// two lane halves of a 32 x 32 result hold the same modes p: the row of (p, kk) is
+// 8 (u / 4) + 4 (kk % 2) + u % 4 with u = p * KHH + kk / 2 (KHH = ceil(QROW / 2), QROW the
+// coefficients per row p), so an accumulator register r of tile t holds u = 16 t + r in both
+// halves, and only k depends on the half. Tiles: ceil(M * KHH / 16); up to two (64 accumulator
+// registers per lane).
+template <int M, bool PAR>
+struct MfmaRows {
+  static constexpr int QROW = PAR ? (M + 1) / 2 : M;
+  static constexpr int KHH = (QROW + 1) / 2;
+  static constexpr int RT = (M * KHH + 15) / 16;
+};
+template <int M, class W, bool PAR>
+constexpr bool kColStageMfma = std::is_same<W, float>::value && M >= 4 && MfmaRows<M, PAR>::RT <= 2;
+bool colstage_matrix_cores_ok(int M, bool mixed, bool parity) {
+  if (!mixed || M < 4 || M > kColStageMaxM) return false;
+  const int qrow = parity ? (M + 1) / 2 : M, khh = (qrow + 1) / 2;
+  return (M * khh + 15) / 16 <= 2;
+}
+typedef float mfma_acc_t __attribute__((ext_vector_type(16)));
+
 // NW wavefronts (4, or 8 when M <= 6); each line transform runs inside one wavefront,
-// in place in its padded LDS row, and wavefronts without a line skip it.
-template <int M, int NW, class W, bool PAR, bool HI = false>
-__global__ void __launch_bounds__(NW * 64, HI ? 3 : 1) k_colstage_raman_gather(Geometry g, const cpx<W>* __restrict__ coltw,
+// in place in its padded LDS row, and wavefronts without a line skip it. With MF (4 wavefronts,
+// FP32) the slot sums run on the matrix cores: wavefront w takes the samples 32 (w % 2) .. + 31
+// and every other pair of its parity class's slots (from pair w / 2).
+template <int M, int NW, class W, bool PAR, bool HI = false, bool MF = false>
+__global__ void __launch_bounds__(NW * 64, HI ? 3 : (MF ? (MfmaRows<M, PAR>::RT == 1 ? 3 : 2) : 1)) k_colstage_raman_gather(Geometry g, const cpx<W>* __restrict__ coltw,
                                                                    const cpx<W>* __restrict__ fourtw,
                                                                    const cpx<W>* __restrict__ U,
                                                                    const cpx<W>* __restrict__ conv,
@@ -930,10 +957,15 @@ __global__ void __launch_bounds__(NW * 64, HI ? 3 : 1) k_colstage_raman_gather(G
   // Raman line rows of a chunk, or the NW partial sums of M modes after the gather
   constexpr int RCAP = colstage_rows(M) * RS > NW * M * L ? colstage_rows(M) * RS : NW * M * L;
   static_assert(M >= 1 && M <= kColStageMaxM, "modes out of range");
+  static_assert(!MF || (NW == 4 && !HI && kColStageMfma<M, W, PAR>), "matrix-core gather: FP32, 4 wavefronts");
   __shared__ cw Ush[UR * RS];   // field rows; sigma rows after the gather
   __shared__ cw Rsh[RCAP];      // Raman line rows of the chunk; partial sums after the gather
   __shared__ W Qsh[kColStageQCapOf<M, W>];  // [slots of the chunk][M][M] coupling blocks
   __shared__ uint32_t Msh[kColStageSCap];  // slot metadata of the chunk
+  // MF: the chunk's slots of each parity class in order (metadata, offset in the chunk), and
+  // their counts
+  __shared__ uint2 Lsh[MF ? 2 * kColStageSCap : 1];
+  __shared__ int Nsh[2];
   const int tid = threadIdx.x;
   const int job = tid / T, t = tid % T;
   const int lane = tid % 64;
@@ -974,9 +1006,27 @@ __global__ void __launch_bounds__(NW * 64, HI ? 3 : 1) k_colstage_raman_gather(G
     if (e < M * L) Ush[(e / L) * RS + e % L] = uin[r];
   }
   const W sc = (W)scale, kv = (W)fp.kv, kr = (W)fp.kr;
-  cw u[M], acc[M];
+  cw u[MF ? 1 : M], acc[M];
 #pragma unroll
   for (int p = 0; p < M; p++) acc[p] = {0, 0};
+  // MF: this lane's sample and lane half, and the coupling offset of its A-operand row in each
+  // tile and parity class (-1: a padding row, zero)
+  const int hh = lane >> 5;
+  const int smp = MF ? 32 * (wave & 1) + (lane & 31) : lane;
+  using MR = MfmaRows<M, PAR>;
+  int aoff[2][MF ? MR::RT : 1];
+  if constexpr (MF) {
+#pragma unroll
+    for (int cc = 0; cc < 2; cc++)
+#pragma unroll
+      for (int t = 0; t < MR::RT; t++) {
+        const int R = 32 * t + (lane & 31);
+        const int uu = 4 * (R >> 3) + (R & 3);
+        const int p = uu / MR::KHH, kk = 2 * (uu % MR::KHH) + ((R >> 2) & 1);
+        const int k = PAR ? 2 * kk + ((p + cc) & 1) : kk;
+        aoff[cc][t] = (p < M && kk < MR::QROW && k < M) ? p * MR::QROW + kk : -1;
+      }
+  }
   for (int c = 0; c < fa.n_chunks; c++) {
     if (c > 0) {
       __syncthreads();  // Rsh, Qsh and Msh of the previous chunk consumed
@@ -995,6 +1045,32 @@ __global__ void __launch_bounds__(NW * 64, HI ? 3 : 1) k_colstage_raman_gather(G
     for (int r = 0; r < QIN; r++) Qsh[tid + r * NT] = qin[r];
     for (int e = QIN * NT + tid; e < (s1 - s0) * MM; e += NT) Qsh[e] = Qd[(size_t)s0 * MM + e];
     if (tid < s1 - s0) Msh[tid] = min_;
+    if constexpr (MF) {
+      // the class lists: wavefronts 0 and 1 hold the chunk's slots 0..63 and 64..127 (at most
+      // kColStageSCap = 128); wavefront 1 also reads the metadata of slots 0..63 for their counts
+      if (tid < 128) {
+        const int ns = s1 - s0, ln = tid % 64;
+        const bool has = tid < ns;
+        const int cls = PAR ? (int)((min_ >> 18) & 1u) : 0;
+        const uint64_t own1 = __ballot(has && cls == 1), own0 = __ballot(has && cls == 0);
+        uint64_t lo1 = own1, lo0 = own0;  // class masks of slots 0..63
+        if (tid >= 64) {
+          const bool has_lo = ln < ns;
+          const uint32_t m = has_lo ? fa.slot_meta[s0 + ln] : 0u;
+          const int cl = PAR ? (int)((m >> 18) & 1u) : 0;
+          lo1 = __ballot(has_lo && cl == 1);
+          lo0 = __ballot(has_lo && cl == 0);
+        }
+        const uint64_t below = ln ? (~0ull >> (64 - ln)) : 0ull;
+        const int pos = cls ? (tid >= 64 ? __popcll(lo1) : 0) + __popcll(own1 & below)
+                            : (tid >= 64 ? __popcll(lo0) : 0) + __popcll(own0 & below);
+        if (has) Lsh[cls * kColStageSCap + pos] = make_uint2(min_, (uint32_t)tid);
+        if (tid == 64) {
+          Nsh[0] = __popcll(lo0) + __popcll(own0);
+          Nsh[1] = __popcll(lo1) + __popcll(own1);
+        }
+      }
+    }
     for (int r0 = l0; r0 < l1; r0 += JOBS) {
       const int q = r0 + job;
       const bool valid = q < l1;
@@ -1014,9 +1090,11 @@ __global__ void __launch_bounds__(NW * 64, HI ? 3 : 1) k_colstage_raman_gather(G
       }
     }
     __syncthreads();
-    if (c == 0) {
+    if constexpr (!MF) {
+      if (c == 0) {
 #pragma unroll
-      for (int k = 0; k < M; k++) u[k] = Ush[k * RS + lane];
+        for (int k = 0; k < M; k++) u[k] = Ush[k * RS + lane];
+      }
     }
     // slot s mixes kv U_a conj(U_b) with its Raman line read as is, conjugated, or by real or
     // imaginary part; everything but the sample is wave-uniform and read from LDS by broadcast.
@@ -1079,12 +1157,105 @@ __global__ void __launch_bounds__(NW * 64, HI ? 3 : 1) k_colstage_raman_gather(G
         }
       }
     };
-    slot_pass(std::integral_constant<int, 0>{});
-    if constexpr (PAR) slot_pass(std::integral_constant<int, 1>{});
+    // the matrix-core form: per parity class, the pairs of its slots in the chunk through the
+    // 32 x 32 x 2 products, then sigma_p += U_k B[p][k] for this lane's sample
+    auto mf_pass = [&](auto cls_c) {
+      constexpr int cc = decltype(cls_c)::value;
+      const int n = Nsh[cc];
+      const int npairs = (n + 1) >> 1;
+      mfma_acc_t ar[MR::RT], ai[MR::RT];
+#pragma unroll
+      for (int t = 0; t < MR::RT; t++) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+          ar[t][r] = 0.f;
+          ai[t][r] = 0.f;
+        }
+      }
+      // the operands of pair pi (this lane's slot 2 pi + hh: X at its sample and its A-operand
+      // coefficients; zeros past the class's last slot), built one pair ahead of its products so
+      // that their LDS reads overlap the matrix-core work of the pair before; the slot's
+      // descriptor (metadata, offset in the chunk) is read one pair before that
+      auto descriptor = [&](int pi) { return Lsh[cc * kColStageSCap + (2 * pi + hh < n ? 2 * pi + hh : 0)]; };
+      auto operands = [&](int pi, uint2 d, W& xre, W& xim, W (&qa)[MR::RT]) {
+        // past the last slot: slot 0's data (finite) times zero, so that no branch splits the reads
+        const W hm = 2 * pi + hh < n ? (W)1 : (W)0;
+        const uint32_t meta = d.x;
+        const int x = (int)d.y;
+        const int pa = meta & 255u, pb = (meta >> 8) & 255u, kind = (meta >> 16) & 3u, line = (int)(meta >> 19);
+        const cw r = Rsh[(line - l0) * RS + smp];
+        const cw ua = Ush[pa * RS + smp], ub = Ush[pb * RS + smp];
+#pragma unroll
+        for (int t = 0; t < MR::RT; t++) {
+          const int o = aoff[cc][t];
+          qa[t] = Qsh[x * MM + (o >= 0 ? o : 0)] * (o >= 0 ? hm : (W)0);
+        }
+        const cw V = cmulc(ua, ub);
+        const W vre = kind == kIm ? r.im : r.re;
+        const W vim = kind == kDirect ? r.im : (kind == kConj ? -r.im : (W)0);
+        xre = (kv * V.re + kr * vre) * hm;
+        xim = (kv * V.im + kr * vim) * hm;
+      };
+      int pi = wave >> 1;
+      W xre, xim, qa[MR::RT];
+      uint2 dn = {0u, 0u};
+      if (n > 0) {
+        operands(pi, descriptor(pi), xre, xim, qa);
+        dn = descriptor(pi + 2);
+      }
+#pragma nounroll
+      for (; pi < npairs; pi += 2) {  // wave-uniform
+        W nre, nim, nqa[MR::RT];
+        const uint2 d2 = dn;
+        dn = descriptor(pi + 4);
+        operands(pi + 2, d2, nre, nim, nqa);
+#pragma unroll
+        for (int t = 0; t < MR::RT; t++) {
+          ar[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(qa[t], xre, ar[t], 0, 0, 0);
+          ai[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(qa[t], xim, ai[t], 0, 0, 0);
+        }
+        xre = nre;
+        xim = nim;
+#pragma unroll
+        for (int t = 0; t < MR::RT; t++) qa[t] = nqa[t];
+      }
+      // accumulator register r of tile t holds u = 16 t + r: mode p = u / KHH, and k from the half
+#pragma unroll
+      for (int t = 0; t < MR::RT; t++) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+          const int uu = 16 * t + r;
+          const int p = uu / MR::KHH;
+          if (p < M) {
+            const int kk0 = 2 * (uu % MR::KHH);
+            const int kA = PAR ? 2 * kk0 + ((p + cc) & 1) : kk0;
+            const int kB = PAR ? kA + 2 : kA + 1;
+            // U_k of this lane's half from its LDS row (a padding row's products are zero: any
+            // field row serves)
+            const int row = kA >= M ? 0 : (kB < M ? kA + (kB - kA) * hh : kA);
+            const cw uk = Ush[row * RS + smp];
+            acc[p] = cadd(acc[p], cmul(uk, cw{ar[t][r], ai[t][r]}));
+          }
+        }
+      }
+      // the sums are taken here, so that the accumulators of one class are free before the next
+      // class's products start (otherwise the compiler sinks them past those and spills)
+#pragma unroll
+      for (int p = 0; p < M; p++) asm volatile("" : "+v"(acc[p].re), "+v"(acc[p].im));
+    };
+    if constexpr (MF) {
+      mf_pass(std::integral_constant<int, 0>{});
+      if constexpr (PAR) mf_pass(std::integral_constant<int, 1>{});
+    } else {
+      slot_pass(std::integral_constant<int, 0>{});
+      if constexpr (PAR) slot_pass(std::integral_constant<int, 1>{});
+    }
   }
   __syncthreads();  // every lane done with Ush and Rsh
+  // NW partial sums of every mode and sample (MF: one per pair parity and lane half)
+  const int part = MF ? 2 * (wave >> 1) + hh : wave;
 #pragma unroll
-  for (int p = 0; p < M; p++) Rsh[(wave * M + p) * L + lane] = acc[p];
+  for (int p = 0; p < M; p++) Rsh[(part * M + p) * L + smp] = acc[p];
   __syncthreads();
   for (int e = tid; e < M * L; e += NT) {
     const int p = e / L, j = e % L;
@@ -1463,6 +1634,12 @@ hipError_t launch_colstage_fields_pairs(const Geometry& g, const Twiddles& tw, c
 }
 
 #define MMP_COLSTAGE_LAUNCH(MV, PARV)                                                                             \
+  if constexpr (kColStageMfma<MV, W, PARV>)                                                                       \
+    if (fa.waves == 4 && fa.matrix_cores) {                                                                       \
+      MMP_LAUNCH((k_colstage_raman_gather<MV, 4, W, PARV, false, true>), dim3(g.N2, F), dim3(256), 0, s, g,      \
+                 TwOf<W>::col(tw), TwOf<W>::four(tw), U, conv, yout, ga, fa, Qd, scale, b);                        \
+      break;                                                                                                      \
+    }                                                                                                             \
   if constexpr (kColStageHiOcc<MV, W, PARV>)                                                                      \
     if (fa.waves == 4 && fa.hi_occupancy) {                                                                       \
       MMP_LAUNCH((k_colstage_raman_gather<MV, 4, W, PARV, true>), dim3(g.N2, F), dim3(256), 0, s, g,             \

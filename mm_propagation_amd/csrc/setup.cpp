@@ -150,6 +150,10 @@ mmp_status prepare_problem(const mmp_params* in, const mmp_options& opt, Problem
     err = "unknown pair_decode";
     return MMP_ERR_INVALID_ARGUMENT;
   }
+  if ((opt.flags & MMP_FLAG_MATRIX_CORES) && (opt.flags & MMP_FLAG_NO_MATRIX_CORES)) {
+    err = "MMP_FLAG_MATRIX_CORES and MMP_FLAG_NO_MATRIX_CORES exclude each other";
+    return MMP_ERR_INVALID_ARGUMENT;
+  }
   if (in->coupling_offsets[0] != 0) { err = "coupling_offsets[0] must be 0"; return MMP_ERR_INVALID_ARGUMENT; }
   for (uint32_t p = 0; p < M; p++)
     if (in->coupling_offsets[p + 1] < in->coupling_offsets[p]) {

@@ -849,6 +849,23 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
     colstage_pairs_plan((int)P.M, s->g.N2, s->n_lines, (int)s->F, s->mixed, s->pair_columns, s->pair_groups);
     // grids of 8 or more workgroups per CU
     s->fused.hi_occupancy = colstage_hi_occupancy_ok((int)P.M, s->mixed, compact) && (long)s->g.N2 * s->F >= 2048;
+    // the coupling sums on the matrix cores (FP32) by default at 4 to 8 modes, whatever the
+    // number of fibers (a batch runs the arithmetic of its fibers alone). Measured on one
+    // MI355X, gather per launch against the vector gather: 7-8 modes 1.03-1.23x (one 32-row
+    // tile, three workgroups per CU); 4-6 modes 1.06-1.31x on grids of 2048 workgroups and more,
+    // 0.84-0.90x on small ones (against the eight-wavefront vector gather); 9-10 modes 1.12-1.19x
+    // on small grids but 0.84-0.92x on large ones, where two tiles hold the matrix-core gather to
+    // two workgroups per CU against three for the vector gather.
+    {
+      const int M = (int)P.M;
+      const bool want = (opt.flags & MMP_FLAG_MATRIX_CORES) || (M >= 4 && M <= 8);
+      s->fused.matrix_cores =
+          !(opt.flags & MMP_FLAG_NO_MATRIX_CORES) && want && colstage_matrix_cores_ok(M, s->mixed, compact) ? 1 : 0;
+    }
+    if (s->fused.matrix_cores) {
+      s->fused.waves = 4;
+      s->fused.hi_occupancy = 0;
+    }
     // lines in column order before the Raman row pass too: its LDS staging costs latency that
     // only large grids hide (measured: batched M = 10, NT = 16384 x 36 fibers row pass 140 ->
     // 111 us; one grin6 fiber 7.0 -> 7.6 us)
@@ -915,22 +932,25 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
 // Raman lines travel in column order into their row pass. Only the gather variants change the
 // order of a sum (over the pair slots); the other choices leave every result bit-identical.
 struct ColPlan {
-  int waves, lines, hi, columns, groups, conv_cols;
+  int waves, lines, hi, columns, groups, conv_cols, mf;
   bool operator==(const ColPlan& o) const {
     return waves == o.waves && lines == o.lines && hi == o.hi && columns == o.columns && groups == o.groups &&
-           conv_cols == o.conv_cols;
+           conv_cols == o.conv_cols && mf == o.mf;
   }
 };
 
 ColPlan current_plan(const mmp_solver* s) {
   return ColPlan{s->fused.waves, s->chunk_lines, s->fused.hi_occupancy, s->pair_columns, s->pair_groups,
-                 s->conv_cols};
+                 s->conv_cols, s->fused.matrix_cores};
 }
 
 bool plan_valid(const mmp_solver* s, const ColPlan& c) {
   const int M = (int)s->P.M;
   if (c.waves != 4 && !(c.waves == 8 && colstage_wide_ok(M, s->mixed))) return false;
   if (c.hi && !(c.waves == 4 && colstage_hi_occupancy_ok(M, s->mixed, s->fused.parity != 0))) return false;
+  const bool mf_ok = colstage_matrix_cores_ok(M, s->mixed, s->fused.parity != 0);
+  if (c.mf && (c.waves != 4 || c.hi || (s->opt.flags & MMP_FLAG_NO_MATRIX_CORES) || !mf_ok)) return false;
+  if (!c.mf && mf_ok && (s->opt.flags & MMP_FLAG_MATRIX_CORES)) return false;
   if (c.columns != 1 && c.columns != 2 && c.columns != 4) return false;
   if (s->g.N2 % c.columns || colstage_pairs_lds_bytes(M, c.columns, s->mixed) > kColStagePairsLds) return false;
   if (c.groups < 1 || c.groups > s->n_lines || (c.groups & (c.groups - 1))) return false;
@@ -951,6 +971,7 @@ mmp_status apply_colstage_plan(mmp_solver* s, const ColPlan& c) {
   s->fused.max_chunk_slots = P.fused_max_slots;
   s->fused.waves = c.waves;
   s->fused.hi_occupancy = c.hi;
+  s->fused.matrix_cores = c.mf;
   s->chunk_lines = c.lines;
   s->pair_columns = c.columns;
   s->pair_groups = c.groups;
@@ -963,7 +984,7 @@ mmp_status advance_impl(mmp_solver* s, uint32_t n);
 
 // Autotuner: the column-stage plan is chosen in three rounds, each keeping the best of the rounds
 // before: the gather variants (wavefronts, lines per chunk that give distinct chunk plans, the
-// three-workgroups-per-CU instance), then the pair pass's columns per workgroup and line groups,
+// three-workgroups-per-CU instance, the coupling sums on the matrix cores), then the pair pass's columns per workgroup and line groups,
 // then the line order into the Raman row pass. Every candidate runs one warm-up graph and two
 // timed graphs of steps on this solver's own fibers (HIP events); the fastest is kept and
 // remembered per structure (modes, samples, stages, slots, lines, parity, precision, fibers) for
@@ -1018,13 +1039,15 @@ mmp_status tune_rounds(mmp_solver* s, hipEvent_t e0, hipEvent_t e1, ColPlan& bes
   const ColPlan base1 = best;
   for (int waves : {4, 8})
     for (int lines : lines_opts)
-      for (int hi : {0, 1}) {
-        ColPlan c = base1;
-        c.waves = waves;
-        c.lines = lines;
-        c.hi = hi;
-        MMP_TRY(consider(c));
-      }
+      for (int hi : {0, 1})
+        for (int mf : {0, 1}) {
+          ColPlan c = base1;
+          c.waves = waves;
+          c.lines = lines;
+          c.hi = hi;
+          c.mf = mf;
+          MMP_TRY(consider(c));
+        }
   // round 2: the pair pass's columns per workgroup and line groups
   const ColPlan base2 = best;
   for (int columns : {4, 2, 1})
@@ -1604,7 +1627,9 @@ mmp_status mmp_fiber_info(mmp_solver* s, uint32_t f, mmp_info* o) {
   o->tuned = s->tuned ? 1u : 0u;
   o->pair_columns = s->use_fused ? (uint32_t)s->pair_columns : 0u;
   o->pair_groups = s->use_fused ? (uint32_t)s->pair_groups : 0u;
-  o->gather_hi_occupancy = s->use_fused && s->fused.waves == 4 && s->fused.hi_occupancy ? 1u : 0u;
+  o->gather_hi_occupancy =
+      s->use_fused && s->fused.waves == 4 && s->fused.hi_occupancy && !s->fused.matrix_cores ? 1u : 0u;
+  o->gather_matrix_cores = s->use_fused && s->fused.waves == 4 && s->fused.matrix_cores ? 1u : 0u;
   o->halted = s->halted[f];
   return MMP_OK;
 }

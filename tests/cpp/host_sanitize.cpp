@@ -162,6 +162,8 @@ void invalid_inputs() {
   expect(mmp_plan(&pr.p, nullptr, nullptr) == MMP_ERR_INVALID_ARGUMENT, "NULL info refused");
   const mmp_options bad_opt{7, 0, 0, 0};
   expect(mmp_plan(&pr.p, &bad_opt, &info) == MMP_ERR_INVALID_ARGUMENT, "unknown precision refused");
+  const mmp_options both_mc{MMP_PRECISION_MIXED, 0, MMP_FLAG_MATRIX_CORES | MMP_FLAG_NO_MATRIX_CORES, 0};
+  expect(mmp_plan(&pr.p, &both_mc, &info) == MMP_ERR_INVALID_ARGUMENT, "contradicting matrix-core flags refused");
   // no device here: creation fails loudly instead of computing elsewhere
   mmp_ctx* ctx = nullptr;
   const mmp_status st = mmp_ctx_create(0, &ctx);

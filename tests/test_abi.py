@@ -33,7 +33,7 @@ def test_every_declared_symbol_is_exported(capi_lib):
 
 
 def test_abi_version_and_status_strings(capi_lib):
-    assert capi_lib.mmp_abi_version() == 5
+    assert capi_lib.mmp_abi_version() == 6
     for code, name in _capi.STATUS_NAMES.items():
         assert capi_lib.mmp_status_string(code).decode() == name
 

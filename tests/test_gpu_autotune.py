@@ -35,7 +35,8 @@ def test_tuned_solver_starts_as_created_and_matches_the_oracle():
     # the second solver of this structure takes the cached choice: the same arithmetic, bit for bit
     b = MultimodePropagationGPU_RamanON(sp, pair_decode=1, flags=_capi.MMP_FLAG_AUTOTUNE)
     ib = b.info()
-    keys = ("gather_waves", "chunk_lines", "gather_hi_occupancy", "pair_columns", "pair_groups")
+    keys = ("gather_waves", "chunk_lines", "gather_hi_occupancy", "gather_matrix_cores", "pair_columns",
+            "pair_groups")
     assert [ib[k] for k in keys] == [info[k] for k in keys]
     b.computeLawsonRK(20)
     assert np.array_equal(b.getResult(), ra)

@@ -49,24 +49,33 @@ def test_batch_equals_single_solves(M, nt, precision):
         s.close()
 
 
-@pytest.mark.parametrize("M", [7, 10])
-def test_large_grid_variants_equal_single_solves(M):
-    """16 fibers at NT = 8192 make 2048 gather workgroups: the batch runs the three-workgroups-
-    per-CU gather, a four-column pair pass with one line group and the Raman lines in column
-    order before their row pass; a fiber alone the two-per-CU gather, four or eight line groups
-    and the lines in row order. The arithmetic is the same, so the results are too."""
+@pytest.mark.parametrize("M,flags,variant", [(7, 0, "matrix"), (7, _capi.MMP_FLAG_NO_MATRIX_CORES, "vector3"),
+                                             (10, _capi.MMP_FLAG_NO_MATRIX_CORES, "vector3")])
+def test_large_grid_variants_equal_single_solves(M, flags, variant):
+    """16 fibers at NT = 8192 make 2048 gather workgroups: the batch runs a four-column pair pass
+    with one line group, the Raman lines in column order before their row pass and the gather's
+    large-grid variant (7 modes: the coupling sums on the matrix cores, or with
+    MMP_FLAG_NO_MATRIX_CORES the three-workgroups-per-CU vector gather; 10 modes: the latter); a
+    fiber alone four or eight line groups, the lines in row order and its small-grid gather
+    (matrix cores at 7 modes, two-per-CU vector gather without them; at 10 modes the flag keeps
+    the fiber alone off the matrix cores, which it would take on its small grid). The arithmetic of each
+    gather is the same in the batch and alone, so the results are too."""
     require_device()
     F = 16
     sps = fibers(M, 8192, F, base=40)
-    note(f"case large grid M={M} NT=8192 fibers={F}")
-    b = MultimodePropagationBatch(sps, pair_decode=1, precision=_capi.MMP_PRECISION_MIXED)
+    note(f"case large grid M={M} NT=8192 fibers={F} flags={flags}")
+    b = MultimodePropagationBatch(sps, pair_decode=1, precision=_capi.MMP_PRECISION_MIXED, flags=flags)
     info = b.info(0)
-    assert (info["pair_columns"], info["gather_waves"], info["gather_hi_occupancy"]) == (4, 4, 1)
+    assert (info["pair_columns"], info["gather_waves"]) == (4, 4)
+    assert (info["gather_hi_occupancy"], info["gather_matrix_cores"]) == ((0, 1) if variant == "matrix" else (1, 0))
     b.computeLawsonRK(24)
     got = {f: b.result(f) for f in (0, 9, F - 1)}
     b.close()
     for f, r in got.items():
-        assert np.array_equal(r, single(sps[f], 24, _capi.MMP_PRECISION_MIXED)), f
+        s = MultimodePropagationGPU_RamanON(sps[f], pair_decode=1, precision=_capi.MMP_PRECISION_MIXED, flags=flags)
+        s.computeLawsonRK(24)
+        assert np.array_equal(r, s.getResult()), f
+        s.close()
 
 
 def test_batch_against_oracle():
