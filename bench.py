@@ -274,6 +274,18 @@ def colstage_gather_flops(M: int, NT: int, P: int, L: int, parity: bool) -> floa
     return float(NT) * per_sample
 
 
+def colstage_gather_mfma_flops(M: int, NT: int, P: int, L: int, parity: bool, chunks: int) -> float:
+    """Arithmetic of the matrix-core form of that launch, in floating-point operations: the same
+    line DFTs (30 per sample and line, plus the 6-flop twiddle), slot mixes (12 per slot) and
+    forward DFT of sigma (36 per mode), the coupling products B[p][k] += Q X_s on the matrix cores
+    (4 per slot, mode pair and sample, the zero padding of the 32-row tiles not counted), and
+    sigma_p += U_k B[p][k] (8 per mode pair and sample, once per chunk and parity class)."""
+    K = (M + 1) // 2 if parity else M
+    classes = 2 if parity else 1
+    per_sample = 36 * L + 12 * P + 4 * P * M * K + 8 * M * K * chunks * classes + 36 * M
+    return float(NT) * per_sample
+
+
 def coupling_parity(sp) -> bool:
     """Whether each pair slot (l, m) of the coupling couples only modes p, k of one parity of
     p + k (zero values aside): the structure under which the gather keeps ceil(M/2) modes per slot."""
@@ -492,18 +504,26 @@ def run_batched(args) -> int:
     gbps = k["bytes_per_launch"] / avg_s / 1e9
     if k["name"] == "col_raman_gather":
         M, NT = key
-        flops = colstage_gather_flops(M, NT, int(info["n_pairs"]), int(info["n_conv_lines"]),
-                                      coupling_parity(fibers[idx[0]])) * len(idx)
+        par = coupling_parity(fibers[idx[0]])
+        matrix = bool(info["gather_matrix_cores"])
+        if matrix:
+            flops = colstage_gather_mfma_flops(M, NT, int(info["n_pairs"]), int(info["n_conv_lines"]), par,
+                                               int(info["gather_chunks"])) * len(idx)
+        else:
+            flops = colstage_gather_flops(M, NT, int(info["n_pairs"]), int(info["n_conv_lines"]), par) * len(idx)
         tflops = flops / avg_s / 1e12
+        # FP32: the matrix cores' dense FP32 peak equals the packed vector peak (157.3 TFLOP/s)
         peak = VALU_PEAK_TFLOPS["f64" if args.fp64_batch else "f32"]
+        arith = "mfma" if matrix else "valu"
         if tflops / peak >= gbps / HBM_PEAK_GBS:
-            roofline = {"bound": "valu", "kernel": k["name"], "achieved": round(tflops, 3), "peak": peak,
+            roofline = {"bound": arith, "kernel": k["name"], "achieved": round(tflops, 3), "peak": peak,
                         "unit": "TFLOP/s", "frac": round(tflops / peak, 5), "traffic": None,
                         "hbm_gbps": round(gbps, 1), "hbm_frac": round(gbps / HBM_PEAK_GBS, 4)}
         else:
             roofline = {"bound": "hbm", "kernel": k["name"], "achieved": round(gbps, 2), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBS, 5), "traffic": None,
-                        "valu_tflops": round(tflops, 3), "valu_frac": round(tflops / peak, 4)}
+                        f"{arith}_tflops": round(tflops, 3), f"{arith}_frac": round(tflops / peak, 4)}
+        roofline["matrix_cores"] = matrix
         roofline.update({"flops_per_launch": flops, "bytes_per_launch": k["bytes_per_launch"]})
     else:
         roofline = {"bound": "hbm", "kernel": k["name"], "achieved": round(gbps, 2), "peak": HBM_PEAK_GBS,
@@ -515,8 +535,9 @@ def run_batched(args) -> int:
         roofline.update({"traffic": traffic, "traffic_source": "profiles/hbm_traffic_batched.md (rocprofv3 FETCH_SIZE "
                                                                "x2.0 + WRITE_SIZE x1.0, copy-calibrated)"})
     roofline.update({"class": cls, "avg_launch_us": round(avg_s * 1e6, 3),
-                     "peak_note": "FP32 vector peak 157.3 TFLOP/s counts packed FP32 instructions, which this build "
-                                  "does not emit (78.6 TFLOP/s without them, as FP64)"})
+                     "peak_note": "FP32 peak 157.3 TFLOP/s: the matrix cores' dense FP32 rate, and the vector rate "
+                                  "with packed FP32 instructions, which this build does not emit (78.6 TFLOP/s "
+                                  "without them, as FP64)"})
     out = {
         "metric": METRIC, "value": value, "unit": "M·NT·steps/s", "n_gpus": 1, "steps": steps,
         "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / steps, "higher_is_better": True, "scaling": "weak",
@@ -597,6 +618,14 @@ def run_serving(args) -> int:
     eng.close()
     makespan = stats["makespan_s"]
     finite_reqs = [r for r in reqs if not r.nonfinite]
+    # capacity: the same requests all submitted at once (the server's rate when the arrivals do
+    # not hold it back), counted over the same finite requests
+    from mm_propagation_amd.serving import Request
+    burst = [Request(rid=r.rid, params=r.params, n_steps=r.n_steps, arrival=0.0, raman=r.raman) for r in reqs]
+    eng = ServingEngine(slots=args.slots)
+    burst_stats = eng.run(burst)
+    eng.close()
+    burst_finite = {r.rid for r in burst if not r.nonfinite}
     # throughput counts the requests whose field stays finite (the others are parked by the device
     # when they overflow and end at the next chunk boundary)
     units = sum(r.units for r in finite_reqs)
@@ -672,7 +701,12 @@ def run_serving(args) -> int:
         "peak_device_bytes": stats["peak_device_bytes"],
         "plain_fp64_sequential": {"value": units / plain_time, "seconds": plain_time,
                                   "ratio": value / (units / plain_time),
-                                  "note": "the finite requests solved one after another by the plain FP64 solver"},
+                                  "note": "the finite requests solved one after another by the plain FP64 solver, "
+                                          "with no wait for their arrivals"},
+        "capacity_all_at_once": {"value": units / burst_stats["makespan_s"], "makespan_s": burst_stats["makespan_s"],
+                                 "ratio_to_plain_fp64_sequential": plain_time / burst_stats["makespan_s"],
+                                 "same_finite_requests": burst_finite == {r.rid for r in finite_reqs},
+                                 "note": "the same requests submitted together at t = 0"},
         "accuracy_vs_fp64_oracle": acc,
         "roofline": {"bound": "hbm", "kernel": "whole run (all kernels)", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),

@@ -146,6 +146,7 @@ typedef struct mmp_info {
   uint32_t pair_groups;      /* column-stage pair pass: workgroups sharing one column block's Raman lines */
   uint32_t gather_hi_occupancy; /* 1 when the gather runs its three-workgroups-per-CU variant */
   uint32_t gather_matrix_cores; /* 1 when the gather's coupling sums run on the matrix cores (FP32) */
+  uint32_t gather_chunks;    /* column-stage gather: LDS chunks of Raman lines and slots per launch */
   uint32_t halted;           /* 1 when the device parked this fiber because its field stopped being finite
                                 (after the step that made it so: z and steps_taken count that step) */
 } mmp_info;

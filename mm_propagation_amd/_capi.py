@@ -87,7 +87,8 @@ class MmpInfo(ctypes.Structure):
         ("n_fibers", c_uint32), ("live", c_uint32),
         ("gather_waves", c_uint32), ("chunk_lines", c_uint32), ("tuned", c_uint32),
         ("pair_columns", c_uint32), ("pair_groups", c_uint32),
-        ("gather_hi_occupancy", c_uint32), ("gather_matrix_cores", c_uint32), ("halted", c_uint32),
+        ("gather_hi_occupancy", c_uint32), ("gather_matrix_cores", c_uint32),
+        ("gather_chunks", c_uint32), ("halted", c_uint32),
     ]
 
 

@@ -1630,6 +1630,7 @@ mmp_status mmp_fiber_info(mmp_solver* s, uint32_t f, mmp_info* o) {
   o->gather_hi_occupancy =
       s->use_fused && s->fused.waves == 4 && s->fused.hi_occupancy && !s->fused.matrix_cores ? 1u : 0u;
   o->gather_matrix_cores = s->use_fused && s->fused.waves == 4 && s->fused.matrix_cores ? 1u : 0u;
+  o->gather_chunks = s->use_fused ? (uint32_t)s->fused.n_chunks : 0u;
   o->halted = s->halted[f];
   return MMP_OK;
 }
