@@ -177,8 +177,8 @@ int colstage_max_modes();
 bool colstage_wide_ok(int M, bool mixed);
 // Whether the three-workgroups-per-CU 4-wavefront gather exists (FusedArgs::hi_occupancy).
 bool colstage_hi_occupancy_ok(int M, bool mixed, bool parity);
-// Whether the matrix-core gather exists (FusedArgs::matrix_cores): FP32, 4 to 12 modes with
-// parity-structured coupling blocks, 4 to 9 without.
+// Whether the matrix-core gather exists (FusedArgs::matrix_cores): FP32, 4 to 16 modes with
+// parity-structured coupling blocks, 4 to 10 without.
 bool colstage_matrix_cores_ok(int M, bool mixed, bool parity);
 int colstage_lines_per_chunk(int M);
 int colstage_slots_per_chunk();

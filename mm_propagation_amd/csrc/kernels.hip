@@ -910,27 +910,32 @@ bool colstage_hi_occupancy_ok(int M, bool mixed, bool parity) { return mixed && 
 // two lane halves of a 32 x 32 result hold the same modes p: the row of (p, kk) is
 // 8 (u / 4) + 4 (kk % 2) + u % 4 with u = p * KHH + kk / 2 (KHH = ceil(QROW / 2), QROW the
 // coefficients per row p), so an accumulator register r of tile t holds u = 16 t + r in both
-// halves, and only k depends on the half. Tiles: ceil(M * KHH / 16); up to two (64 accumulator
-// registers per lane).
+// halves, and only k depends on the half. Tiles: ceil(M * KHH / 16), up to four. With one or two
+// tiles each wavefront holds all of them (32 or 64 accumulator registers per lane) and takes every
+// other pair of slots; with three or four the wavefront pairs split the tiles (two each) and every
+// wavefront takes every pair, forming its slots' X twice over the workgroup.
 template <int M, bool PAR>
 struct MfmaRows {
   static constexpr int QROW = PAR ? (M + 1) / 2 : M;
   static constexpr int KHH = (QROW + 1) / 2;
   static constexpr int RT = (M * KHH + 15) / 16;
+  static constexpr bool TSPLIT = RT > 2;      // the wavefront pairs split the tiles
+  static constexpr int TPW = TSPLIT ? 2 : RT;  // tiles per wavefront
 };
 template <int M, class W, bool PAR>
-constexpr bool kColStageMfma = std::is_same<W, float>::value && M >= 4 && MfmaRows<M, PAR>::RT <= 2;
+constexpr bool kColStageMfma = std::is_same<W, float>::value && M >= 4 && MfmaRows<M, PAR>::RT <= 4;
 bool colstage_matrix_cores_ok(int M, bool mixed, bool parity) {
   if (!mixed || M < 4 || M > kColStageMaxM) return false;
   const int qrow = parity ? (M + 1) / 2 : M, khh = (qrow + 1) / 2;
-  return (M * khh + 15) / 16 <= 2;
+  return (M * khh + 15) / 16 <= 4;
 }
 typedef float mfma_acc_t __attribute__((ext_vector_type(16)));
 
 // NW wavefronts (4, or 8 when M <= 6); each line transform runs inside one wavefront,
 // in place in its padded LDS row, and wavefronts without a line skip it. With MF (4 wavefronts,
 // FP32) the slot sums run on the matrix cores: wavefront w takes the samples 32 (w % 2) .. + 31
-// and every other pair of its parity class's slots (from pair w / 2).
+// and, with up to two 32-row tiles, every other pair of its parity class's slots (from pair
+// w / 2); with three or four tiles, tiles 2 (w / 2) and 2 (w / 2) + 1 over every pair.
 template <int M, int NW, class W, bool PAR, bool HI = false, bool MF = false>
 __global__ void __launch_bounds__(NW * 64, HI ? 3 : (MF ? (MfmaRows<M, PAR>::RT == 1 ? 3 : 2) : 1)) k_colstage_raman_gather(Geometry g, const cpx<W>* __restrict__ coltw,
                                                                    const cpx<W>* __restrict__ fourtw,
@@ -1014,17 +1019,19 @@ __global__ void __launch_bounds__(NW * 64, HI ? 3 : (MF ? (MfmaRows<M, PAR>::RT 
   const int hh = lane >> 5;
   const int smp = MF ? 32 * (wave & 1) + (lane & 31) : lane;
   using MR = MfmaRows<M, PAR>;
-  int aoff[2][MF ? MR::RT : 1];
+  const int tg = MR::TSPLIT ? (wave >> 1) : 0;  // this wavefront's tile group (tiles 2 tg, 2 tg + 1)
+  int aoff[2][MF ? MR::TPW : 1];
   if constexpr (MF) {
 #pragma unroll
     for (int cc = 0; cc < 2; cc++)
 #pragma unroll
-      for (int t = 0; t < MR::RT; t++) {
+      for (int j = 0; j < MR::TPW; j++) {
+        const int t = 2 * tg + j;
         const int R = 32 * t + (lane & 31);
         const int uu = 4 * (R >> 3) + (R & 3);
         const int p = uu / MR::KHH, kk = 2 * (uu % MR::KHH) + ((R >> 2) & 1);
         const int k = PAR ? 2 * kk + ((p + cc) & 1) : kk;
-        aoff[cc][t] = (p < M && kk < MR::QROW && k < M) ? p * MR::QROW + kk : -1;
+        aoff[cc][j] = (t < MR::RT && p < M && kk < MR::QROW && k < M) ? p * MR::QROW + kk : -1;
       }
   }
   for (int c = 0; c < fa.n_chunks; c++) {
@@ -1163,9 +1170,11 @@ __global__ void __launch_bounds__(NW * 64, HI ? 3 : (MF ? (MfmaRows<M, PAR>::RT 
       constexpr int cc = decltype(cls_c)::value;
       const int n = Nsh[cc];
       const int npairs = (n + 1) >> 1;
-      mfma_acc_t ar[MR::RT], ai[MR::RT];
+      constexpr int TPW = MR::TPW;
+      constexpr int PSTEP = MR::TSPLIT ? 1 : 2;  // pairs between a wavefront's consecutive pairs
+      mfma_acc_t ar[TPW], ai[TPW];
 #pragma unroll
-      for (int t = 0; t < MR::RT; t++) {
+      for (int t = 0; t < TPW; t++) {
 #pragma unroll
         for (int r = 0; r < 16; r++) {
           ar[t][r] = 0.f;
@@ -1177,7 +1186,7 @@ __global__ void __launch_bounds__(NW * 64, HI ? 3 : (MF ? (MfmaRows<M, PAR>::RT 
       // that their LDS reads overlap the matrix-core work of the pair before; the slot's
       // descriptor (metadata, offset in the chunk) is read one pair before that
       auto descriptor = [&](int pi) { return Lsh[cc * kColStageSCap + (2 * pi + hh < n ? 2 * pi + hh : 0)]; };
-      auto operands = [&](int pi, uint2 d, W& xre, W& xim, W (&qa)[MR::RT]) {
+      auto operands = [&](int pi, uint2 d, W& xre, W& xim, W (&qa)[TPW]) {
         // past the last slot: slot 0's data (finite) times zero, so that no branch splits the reads
         const W hm = 2 * pi + hh < n ? (W)1 : (W)0;
         const uint32_t meta = d.x;
@@ -1186,7 +1195,7 @@ __global__ void __launch_bounds__(NW * 64, HI ? 3 : (MF ? (MfmaRows<M, PAR>::RT 
         const cw r = Rsh[(line - l0) * RS + smp];
         const cw ua = Ush[pa * RS + smp], ub = Ush[pb * RS + smp];
 #pragma unroll
-        for (int t = 0; t < MR::RT; t++) {
+        for (int t = 0; t < TPW; t++) {
           const int o = aoff[cc][t];
           qa[t] = Qsh[x * MM + (o >= 0 ? o : 0)] * (o >= 0 ? hm : (W)0);
         }
@@ -1196,47 +1205,60 @@ __global__ void __launch_bounds__(NW * 64, HI ? 3 : (MF ? (MfmaRows<M, PAR>::RT 
         xre = (kv * V.re + kr * vre) * hm;
         xim = (kv * V.im + kr * vim) * hm;
       };
-      int pi = wave >> 1;
-      W xre, xim, qa[MR::RT];
+      int pi = MR::TSPLIT ? 0 : (wave >> 1);
+      W xre, xim, qa[TPW];
       uint2 dn = {0u, 0u};
       if (n > 0) {
         operands(pi, descriptor(pi), xre, xim, qa);
-        dn = descriptor(pi + 2);
+        dn = descriptor(pi + PSTEP);
       }
 #pragma nounroll
-      for (; pi < npairs; pi += 2) {  // wave-uniform
-        W nre, nim, nqa[MR::RT];
+      for (; pi < npairs; pi += PSTEP) {  // wave-uniform
+        W nre, nim, nqa[TPW];
         const uint2 d2 = dn;
-        dn = descriptor(pi + 4);
-        operands(pi + 2, d2, nre, nim, nqa);
+        dn = descriptor(pi + 2 * PSTEP);
+        operands(pi + PSTEP, d2, nre, nim, nqa);
 #pragma unroll
-        for (int t = 0; t < MR::RT; t++) {
+        for (int t = 0; t < TPW; t++) {
+          if (MR::TSPLIT && 2 * tg + t >= MR::RT) continue;  // wave-uniform: the group's second tile absent
           ar[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(qa[t], xre, ar[t], 0, 0, 0);
           ai[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(qa[t], xim, ai[t], 0, 0, 0);
         }
         xre = nre;
         xim = nim;
 #pragma unroll
-        for (int t = 0; t < MR::RT; t++) qa[t] = nqa[t];
+        for (int t = 0; t < TPW; t++) qa[t] = nqa[t];
       }
-      // accumulator register r of tile t holds u = 16 t + r: mode p = u / KHH, and k from the half
+      // accumulator register r of tile t holds u = 16 t + r: mode p = u / KHH, and k from the
+      // half; the tile group is a compile-time constant of each epilogue instance
+      auto epilogue = [&](auto tg_c) {
+        constexpr int TG = decltype(tg_c)::value;
 #pragma unroll
-      for (int t = 0; t < MR::RT; t++) {
+        for (int j = 0; j < TPW; j++) {
+          const int t = 2 * TG + j;
+          if (t >= MR::RT) break;
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-          const int uu = 16 * t + r;
-          const int p = uu / MR::KHH;
-          if (p < M) {
-            const int kk0 = 2 * (uu % MR::KHH);
-            const int kA = PAR ? 2 * kk0 + ((p + cc) & 1) : kk0;
-            const int kB = PAR ? kA + 2 : kA + 1;
-            // U_k of this lane's half from its LDS row (a padding row's products are zero: any
-            // field row serves)
-            const int row = kA >= M ? 0 : (kB < M ? kA + (kB - kA) * hh : kA);
-            const cw uk = Ush[row * RS + smp];
-            acc[p] = cadd(acc[p], cmul(uk, cw{ar[t][r], ai[t][r]}));
+          for (int r = 0; r < 16; r++) {
+            const int uu = 16 * t + r;
+            const int p = uu / MR::KHH;
+            if (p < M) {
+              const int kk0 = 2 * (uu % MR::KHH);
+              const int kA = PAR ? 2 * kk0 + ((p + cc) & 1) : kk0;
+              const int kB = PAR ? kA + 2 : kA + 1;
+              // U_k of this lane's half from its LDS row (a padding row's products are zero: any
+              // field row serves)
+              const int row = kA >= M ? 0 : (kB < M ? kA + (kB - kA) * hh : kA);
+              const cw uk = Ush[row * RS + smp];
+              acc[p] = cadd(acc[p], cmul(uk, cw{ar[j][r], ai[j][r]}));
+            }
           }
         }
+      };
+      if constexpr (!MR::TSPLIT) {
+        epilogue(std::integral_constant<int, 0>{});
+      } else {
+        if (tg == 0) epilogue(std::integral_constant<int, 0>{});
+        else epilogue(std::integral_constant<int, 1>{});
       }
       // the sums are taken here, so that the accumulators of one class are free before the next
       // class's products start (otherwise the compiler sinks them past those and spills)
@@ -1252,7 +1274,7 @@ __global__ void __launch_bounds__(NW * 64, HI ? 3 : (MF ? (MfmaRows<M, PAR>::RT 
     }
   }
   __syncthreads();  // every lane done with Ush and Rsh
-  // NW partial sums of every mode and sample (MF: one per pair parity and lane half)
+  // NW partial sums of every mode and sample (MF: one per pair parity, or tile group, and lane half)
   const int part = MF ? 2 * (wave >> 1) + hh : wave;
 #pragma unroll
   for (int p = 0; p < M; p++) Rsh[(part * M + p) * L + smp] = acc[p];

@@ -5,8 +5,8 @@ columns), then sigma_p = sum_k U_k B[p][k] per sample. Against the FP64 oracle a
 mixed-precision bar of tests/test_gpu_mixed.py (1e-6, or the dynamics' amplification of an FP64
 rounding-level perturbation scaled to FP32), and against the vector-ALU gather of the same
 problem (MMP_FLAG_NO_MATRIX_CORES), which differs from it by FP32 rounding only. The default takes
-the matrix cores at 4 to 8 modes whatever the number of fibers; MMP_FLAG_MATRIX_CORES takes them
-at 9 and 10 modes too."""
+the matrix cores at 4 to 8 and 13 to 16 modes whatever the number of fibers;
+MMP_FLAG_MATRIX_CORES takes them at 9 to 12 modes too."""
 import numpy as np
 import pytest
 
@@ -35,11 +35,13 @@ def solve(sp, steps, flags=0):
 
 
 @pytest.mark.parametrize("M,nt,broken", [(7, 4096, False), (8, 8192, False), (8, 4096, True), (9, 4096, False),
-                                         (10, 16384, False)])
+                                         (10, 16384, False), (10, 4096, True), (12, 4096, False), (15, 8192, False),
+                                         (16, 4096, False)])
 def test_matrix_core_gather_against_oracle_and_vector_gather(M, nt, broken):
-    """7 to 10 modes with parity-compact coupling blocks (one or two 32-row tiles per class, odd
-    M with padding rows), and 8 modes with the parity structure broken (full blocks, one class,
-    two tiles)."""
+    """7 to 16 modes with parity-compact coupling blocks (one to four 32-row tiles per class: up to
+    two per wavefront with the wavefronts splitting the pairs, three or four with the wavefronts
+    splitting the tiles; odd M with padding rows), and 8 and 10 modes with the parity structure
+    broken (full blocks, one class, two and four tiles)."""
     require_device()
     sp = configs.grin_fiber(M=M, nt=nt, symmetric=True, power_range=(1.0, 100.0))
     if broken:
@@ -49,7 +51,7 @@ def test_matrix_core_gather_against_oracle_and_vector_gather(M, nt, broken):
     ra, ia = solve(sp, steps, MATRIX)
     rb, ib = solve(sp, steps, VALU)
     _, idef = solve(sp, 1)
-    assert idef["gather_matrix_cores"] == (1 if M <= 8 else 0), idef
+    assert idef["gather_matrix_cores"] == (1 if M <= 8 or M >= 13 else 0), idef
     assert ia["gather_matrix_cores"] == 1 and ia["gather_waves"] == 4, ia
     assert ib["gather_matrix_cores"] == 0, ib
     o = ob.Oracle(sp, pair_decode=1, n_threads=16)
@@ -82,11 +84,12 @@ def test_matrix_core_batch_equals_single_solves():
 
 
 @pytest.mark.parametrize("M,nt,fibers,expected", [(10, 8192, 16, (1, 0)), (10, 8192, 1, (0, 0)), (6, 16384, 8, (0, 1)),
-                                                  (6, 16384, 1, (0, 1)), (8, 16384, 8, (0, 1)), (4, 4096, 2, (0, 1))])
+                                                  (6, 16384, 1, (0, 1)), (8, 16384, 8, (0, 1)), (4, 4096, 2, (0, 1)),
+                                                  (12, 4096, 2, (0, 0)), (15, 4096, 2, (0, 1))])
 def test_default_gather_variant(M, nt, fibers, expected):
-    """(gather_hi_occupancy, gather_matrix_cores) chosen at creation: 4-8 modes take the matrix
-    cores, 9-10 modes the vector gather (three workgroups per CU on grids of 2048 workgroups and
-    more), whatever the number of fibers."""
+    """(gather_hi_occupancy, gather_matrix_cores) chosen at creation: 4-8 and 13-16 modes take the
+    matrix cores, 9-12 modes the vector gather (three workgroups per CU on grids of 2048
+    workgroups and more at 9-10 modes), whatever the number of fibers."""
     require_device()
     sps = [configs.grin_fiber(M=M, nt=nt, seed=i, symmetric=True) for i in range(fibers)]
     b = MultimodePropagationBatch(sps, pair_decode=1, precision=MIXED)
@@ -95,13 +98,16 @@ def test_default_gather_variant(M, nt, fibers, expected):
     assert (info["gather_hi_occupancy"], info["gather_matrix_cores"]) == expected, info
 
 
-@pytest.mark.parametrize("M,precision", [(10, _capi.MMP_PRECISION_FP64), (3, MIXED), (11, MIXED), (12, MIXED)])
-def test_no_matrix_cores_where_the_variant_does_not_exist(M, precision):
-    """FP64, fewer than 4 modes, or more than two 32-row tiles per class (11 and 12 modes: the
-    accumulators would not fit beside the rest of the kernel): MMP_FLAG_MATRIX_CORES changes
+@pytest.mark.parametrize("M,precision,broken", [(10, _capi.MMP_PRECISION_FP64, False), (3, MIXED, False),
+                                                (11, MIXED, True), (20, MIXED, False)])
+def test_no_matrix_cores_where_the_variant_does_not_exist(M, precision, broken):
+    """FP64, fewer than 4 modes, more than four 32-row tiles per class (11 modes with full
+    coupling blocks), or more than 16 modes (the separate passes): MMP_FLAG_MATRIX_CORES changes
     nothing there. With MMP_FLAG_NO_MATRIX_CORES as well it is refused."""
     require_device()
-    sp = configs.grin_fiber(M=M, nt=4096, symmetric=True)
+    sp = configs.grin_fiber(M=M, nt=4096 if M <= 16 else 256, symmetric=True)
+    if broken:
+        sp = break_parity(sp)
     s = MultimodePropagationGPU_RamanON(sp, pair_decode=1, precision=precision, flags=MATRIX)
     assert s.info()["gather_matrix_cores"] == 0
     s.close()
