@@ -983,10 +983,11 @@ mmp_status apply_colstage_plan(mmp_solver* s, const ColPlan& c) {
 
 mmp_status advance_impl(mmp_solver* s, uint32_t n);
 
-// Autotuner: the column-stage plan is chosen in three rounds, each keeping the best of the rounds
+// Autotuner: the column-stage plan is chosen in three passes, each keeping the best of the passes
 // before: the gather variants (wavefronts, lines per chunk that give distinct chunk plans, the
-// three-workgroups-per-CU instance, the coupling sums on the matrix cores), then the pair pass's columns per workgroup and line groups,
-// then the line order into the Raman row pass. Every candidate runs one warm-up graph and two
+// three-workgroups-per-CU instance, the coupling sums on the matrix cores), then the pair
+// pass's columns per workgroup and line groups, then the line order into the Raman row pass.
+// Every candidate runs one warm-up graph and two
 // timed graphs of steps on this solver's own fibers (HIP events); the fastest is kept and
 // remembered per structure (modes, samples, stages, slots, lines, parity, precision, fibers) for
 // later solvers of the process. The fibers' spectra are saved first and restored after; z, steps,
@@ -1010,7 +1011,7 @@ mmp_status time_plan(mmp_solver* s, const ColPlan& c, hipEvent_t e0, hipEvent_t 
   return MMP_OK;
 }
 
-mmp_status tune_rounds(mmp_solver* s, hipEvent_t e0, hipEvent_t e1, ColPlan& best) {
+mmp_status tune_passes(mmp_solver* s, hipEvent_t e0, hipEvent_t e1, ColPlan& best) {
   const Problem& P = s->P;
   const int M = (int)P.M;
   best = current_plan(s);
@@ -1026,7 +1027,7 @@ mmp_status tune_rounds(mmp_solver* s, hipEvent_t e0, hipEvent_t e1, ColPlan& bes
     }
     return MMP_OK;
   };
-  // round 1: gather variants (lines per chunk only where they give distinct chunk plans)
+  // (1) gather variants (lines per chunk only where they give distinct chunk plans)
   std::vector<std::vector<uint32_t>> seen;
   std::vector<int> lines_opts;
   for (int lines : {colstage_lines_per_chunk(M), 24, 16}) {
@@ -1049,7 +1050,7 @@ mmp_status tune_rounds(mmp_solver* s, hipEvent_t e0, hipEvent_t e1, ColPlan& bes
           c.mf = mf;
           MMP_TRY(consider(c));
         }
-  // round 2: the pair pass's columns per workgroup and line groups
+  // (2) the pair pass's columns per workgroup and line groups
   const ColPlan base2 = best;
   for (int columns : {4, 2, 1})
     for (int groups = 1; groups <= 16 && (groups == 1 || groups * 4 <= s->n_lines); groups *= 2) {
@@ -1058,7 +1059,7 @@ mmp_status tune_rounds(mmp_solver* s, hipEvent_t e0, hipEvent_t e1, ColPlan& bes
       c.groups = groups;
       MMP_TRY(consider(c));
     }
-  // round 3: the Raman lines' order into their row pass
+  // (3) the Raman lines' order into their row pass
   ColPlan c = best;
   c.conv_cols = 1 - c.conv_cols;
   MMP_TRY(consider(c));
@@ -1088,7 +1089,7 @@ mmp_status autotune(mmp_solver* s) {
   if (st == MMP_OK && hipMemcpyAsync(saved, s->psi_hat, state_bytes, hipMemcpyDeviceToDevice, s->ctx->stream) != hipSuccess)
     st = fail(MMP_ERR_DEVICE, "hipMemcpyAsync D2D");
   ColPlan best = current_plan(s);
-  if (st == MMP_OK) st = tune_rounds(s, e0, e1, best);
+  if (st == MMP_OK) st = tune_passes(s, e0, e1, best);
   if (st == MMP_OK) st = apply_colstage_plan(s, best);
   if (st == MMP_OK && hipMemcpyAsync(s->psi_hat, saved, state_bytes, hipMemcpyDeviceToDevice, s->ctx->stream) != hipSuccess)
     st = fail(MMP_ERR_DEVICE, "hipMemcpyAsync D2D");

@@ -2,7 +2,7 @@
 Kerr sum in FP32; spectral state, Lawson phases, stage combination and energy in FP64.
 Compared with the FP64 oracle on the same inputs, per-mode rel-L2 after the same steps.
 
-Bar: MIXED_SHORT = 1e-6 (the north star's FP32 bar), or, where it is larger, the growth the
+Bar: MIXED_SHORT = 1e-6 (the FP32 target of the mixed-precision path), or, where it is larger, the growth the
 dynamics give an FP64 rounding-level perturbation over the same steps (the oracle with its
 decimation-in-frequency FFT against decimation in time) scaled from FP64 to FP32 unit roundoff
 (2^29) times AMPLIFIED_FACTOR. These workloads take large nonlinear phase steps (10-1000 W per

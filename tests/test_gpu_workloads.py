@@ -6,7 +6,7 @@ NT = 65536 requests.
 
 Bars (per-mode rel-L2 against the oracle after the same steps):
   FP64_SHORT  = 1e-12: the plain FP64 path; short runs observe ~1e-15.
-  MIXED_BAR   = 1e-6:  mixed precision, the north star's FP32 bar.
+  MIXED_BAR   = 1e-6:  mixed precision, the FP32 target of the mixed-precision path.
   Mixed precision on a fiber whose dynamics amplify rounding quickly: the workload's pulses reach
   10 kW with as few as 10000 steps per metre, and on such a fiber an FP64 rounding-level
   perturbation (the oracle with its decimation-in-frequency FFT instead of decimation in time)
