@@ -1019,14 +1019,14 @@ __global__ void __launch_bounds__(NW * 64, HI ? 3 : (MF ? (MfmaRows<M, PAR>::RT 
   const int hh = lane >> 5;
   const int smp = MF ? 32 * (wave & 1) + (lane & 31) : lane;
   using MR = MfmaRows<M, PAR>;
-  const int tg = MR::TSPLIT ? (wave >> 1) : 0;  // this wavefront's tile group (tiles 2 tg, 2 tg + 1)
+  const int tg = MR::TSPLIT ? (wave >> 1) : 0;  // this wavefront's tile group (tiles TPW tg .. TPW tg + TPW - 1)
   int aoff[2][MF ? MR::TPW : 1];
   if constexpr (MF) {
 #pragma unroll
     for (int cc = 0; cc < 2; cc++)
 #pragma unroll
       for (int j = 0; j < MR::TPW; j++) {
-        const int t = 2 * tg + j;
+        const int t = MR::TPW * tg + j;
         const int R = 32 * t + (lane & 31);
         const int uu = 4 * (R >> 3) + (R & 3);
         const int p = uu / MR::KHH, kk = 2 * (uu % MR::KHH) + ((R >> 2) & 1);
@@ -1220,7 +1220,7 @@ __global__ void __launch_bounds__(NW * 64, HI ? 3 : (MF ? (MfmaRows<M, PAR>::RT 
         operands(pi + PSTEP, d2, nre, nim, nqa);
 #pragma unroll
         for (int t = 0; t < TPW; t++) {
-          if (MR::TSPLIT && 2 * tg + t >= MR::RT) continue;  // wave-uniform: the group's second tile absent
+          if (MR::TSPLIT && TPW * tg + t >= MR::RT) continue;  // wave-uniform: a tile of the group absent
           ar[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(qa[t], xre, ar[t], 0, 0, 0);
           ai[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(qa[t], xim, ai[t], 0, 0, 0);
         }
@@ -1235,7 +1235,7 @@ __global__ void __launch_bounds__(NW * 64, HI ? 3 : (MF ? (MfmaRows<M, PAR>::RT 
         constexpr int TG = decltype(tg_c)::value;
 #pragma unroll
         for (int j = 0; j < TPW; j++) {
-          const int t = 2 * TG + j;
+          const int t = TPW * TG + j;
           if (t >= MR::RT) break;
 #pragma unroll
           for (int r = 0; r < 16; r++) {
