@@ -121,14 +121,18 @@ def main() -> int:
                     help="every column pass a kernel of its own (7 per stage instead of 4)")
     ap.add_argument("--mixed", action="store_true",
                     help="mixed precision (FP32 transforms and sums, FP64 state); the bench line is FP64 by default")
-    ap.add_argument("--workload", choices=["grin6", "gaussian2", "grin10_mixed", "batched", "serving"],
+    ap.add_argument("--workload", choices=["grin6", "gaussian2", "grin10_mixed", "batched", "serving", "gather"],
                     default="batched",
                     help="batched (default): 256 independent GRIN fibers "
                          "(M in {3, 6, 10}, NT in {4096, 8192, 16384}) in batches per (M, NT), mixed precision; "
                          "grin6: the 6-mode GRIN fiber in FP64; gaussian2: the 2-mode Gaussian in FP64; "
                          "grin10_mixed: the 10-mode GRIN fiber at NT=65536 in mixed precision; serving: Poisson "
                          "arrivals of solves (M in {1, 3, 6, 10, 15}, NT 2^12..2^16, 500..20000 steps) through "
-                         "the continuous-batching engine")
+                         "the continuous-batching engine; gather: one batch of --fibers GRIN fibers of --modes "
+                         "modes at --nt samples (1-100 W), mixed precision, the column-stage gather on the vector "
+                         "ALUs and on the matrix cores, dispatch-recorded time per launch of each")
+    ap.add_argument("--modes", type=int, default=10, help="gather workload: modes")
+    ap.add_argument("--nt", type=int, default=16384, help="gather workload: samples")
     ap.add_argument("--fibers", type=int, default=256, help="batched workload: number of fibers")
     ap.add_argument("--fp64-batch", action="store_true", help="batched workload in FP64 (default: mixed)")
     ap.add_argument("--batch-streams", choices=["concurrent", "one", "per-class"], default="concurrent",
@@ -151,6 +155,8 @@ def main() -> int:
         return run_batched(args)
     if args.workload == "serving":
         return run_serving(args)
+    if args.workload == "gather":
+        return run_gather(args)
 
     from mm_propagation_amd import _capi, configs
     from mm_propagation_amd.solver import Context, MultimodePropagationGPU_RamanON
@@ -583,6 +589,53 @@ def run_batched(args) -> int:
         c.close()
     if args.copy_calibration_mib > 0:
         copy_calibration(args.copy_calibration_mib)
+    return 0
+
+
+def run_gather(args) -> int:
+    """The column-stage gather of one batch (`--fibers` GRIN fibers, `--modes` modes, `--nt`
+    samples, 1-100 W so that every fiber stays finite, mixed precision, untuned) in its two forms:
+    the coupling sums on the vector ALUs (MMP_FLAG_NO_MATRIX_CORES) and on the matrix cores
+    (MMP_FLAG_MATRIX_CORES, where that form exists). Eager launches with dispatch-recorded events:
+    4 warm-up steps, then exactly --steps steps (default 16) timed per kernel."""
+    from mm_propagation_amd import _capi, configs
+    from mm_propagation_amd.batch import MultimodePropagationBatch
+
+    F = args.fibers if args.fibers != 256 else 16
+    steps = args.steps if args.steps is not None else 16
+    sps = [configs.grin_fiber(M=args.modes, nt=args.nt, n_steps=20000, fiber_length=0.5, seed=200 + i,
+                              symmetric=True, power_range=(1.0, 100.0)) for i in range(F)]
+    forms = {}
+    for name, flag in (("vector", _capi.MMP_FLAG_NO_MATRIX_CORES), ("matrix", _capi.MMP_FLAG_MATRIX_CORES)):
+        b = MultimodePropagationBatch(sps, pair_decode=_capi.MMP_PAIRS_PHYSICAL, precision=_capi.MMP_PRECISION_MIXED,
+                                      flags=flag | _capi.MMP_FLAG_PROFILE)
+        info = b.info(0)
+        if name == "matrix" and not info["gather_matrix_cores"]:
+            b.close()
+            continue
+        b.advance(4)
+        b.kernel_times()
+        b.advance(steps)
+        kt = b.kernel_times()
+        b.close()
+        g = next(k for k in kt if k["name"] == "col_raman_gather")
+        avg_s = g["total_ms"] / 1e3 / g["launches"]
+        flops = (colstage_gather_mfma_flops(args.modes, args.nt, int(info["n_pairs"]), int(info["n_conv_lines"]), True,
+                                            int(info["gather_chunks"])) if name == "matrix" else
+                 colstage_gather_flops(args.modes, args.nt, int(info["n_pairs"]), int(info["n_conv_lines"]), True)) * F
+        forms[name] = {"avg_launch_us": round(avg_s * 1e6, 2), "tflops": round(flops / avg_s / 1e12, 2),
+                       "frac_of_157.3": round(flops / avg_s / 1e12 / VALU_PEAK_TFLOPS["f32"], 4),
+                       "hbm_gbps": round(g["bytes_per_launch"] / avg_s / 1e9, 1),
+                       "step_kernel_us": round(sum(k["total_ms"] for k in kt) * 1e3 / steps, 1),
+                       "gather_waves": info["gather_waves"], "gather_hi_occupancy": info["gather_hi_occupancy"]}
+    best = min(forms.values(), key=lambda f: f["avg_launch_us"])
+    units = args.modes * args.nt * F
+    out = {"metric": "column-stage gather launches per second x M·NT (one RK stage of every fiber)",
+           "value": units / (best["avg_launch_us"] * 1e-6), "unit": "M·NT/s", "n_gpus": 1, "steps": steps,
+           "warmup": 4, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"gather: {F} GRIN fibers, M = {args.modes}, NT = {args.nt}, 1-100 W, mixed, untuned"},
+           "forms": forms}
+    print(json.dumps(out), flush=True)
     return 0
 
 
