@@ -81,12 +81,14 @@ def single_workloads():
 
 def cpu_baseline(sp, steps: int, decode: int, name: str):
     """Time the FP64 C restatement of the reference CPU path on a bounded sample: the first
-    `steps` steps of the solve from z = 0 (computeLawsonRK(steps), operator set-up included).
-    Returns the baseline record and the oracle's field at z = steps * h."""
+    `steps` steps of the solve from z = 0 (computeLawsonRK(steps), operator set-up included),
+    with the reference's decomposition (OpenMP over modes, the Raman pair loop serial,
+    src/MultimodePropagation_CPU.cpp:196-205). Returns the baseline record and the oracle's
+    field at z = steps * h."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import oracle_binding as ob
     threads = min(16, os.cpu_count() or 1)
-    o = ob.Oracle(sp, pair_decode=decode, n_threads=threads, parallel_pairs=True)
+    o = ob.Oracle(sp, pair_decode=decode, n_threads=threads, parallel_pairs=False)
     t0 = time.perf_counter()
     o.compute_lawson_rk(steps)
     dt = time.perf_counter() - t0
@@ -95,7 +97,8 @@ def cpu_baseline(sp, steps: int, decode: int, name: str):
     rec = {"value": sp.n_modes * sp.nt * steps / dt, "unit": "M·NT·steps/s", "cores": threads,
            "kind": "port",
            "sample": f"{name} M={sp.n_modes} NT={sp.nt}: first {steps} of {sp.n_steps} RK4 steps in {dt:.2f} s, "
-                     f"OpenMP {threads} threads over modes and Raman pairs, radix-2 FFT"}
+                     f"OpenMP {threads} threads over modes, the Raman pair loop serial as in the reference, "
+                     "radix-2 FFT (no FFTW here)"}
     return rec, fields
 
 
