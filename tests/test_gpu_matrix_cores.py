@@ -34,14 +34,15 @@ def solve(sp, steps, flags=0):
     return r, info
 
 
-@pytest.mark.parametrize("M,nt,broken", [(7, 4096, False), (8, 8192, False), (8, 4096, True), (9, 4096, False),
+@pytest.mark.parametrize("M,nt,broken", [(4, 8192, True), (5, 4096, False), (6, 4096, True), (7, 4096, False),
+                                         (8, 8192, False), (8, 4096, True), (9, 4096, False),
                                          (10, 16384, False), (10, 4096, True), (12, 4096, False), (15, 8192, False),
                                          (16, 4096, False)])
 def test_matrix_core_gather_against_oracle_and_vector_gather(M, nt, broken):
-    """7 to 16 modes with parity-compact coupling blocks (one to four 32-row tiles per class: up to
+    """5 to 16 modes with parity-compact coupling blocks (one to four 32-row tiles per class: up to
     two per wavefront with the wavefronts splitting the pairs, three or four with the wavefronts
-    splitting the tiles; odd M with padding rows), and 8 and 10 modes with the parity structure
-    broken (full blocks, one class, two and four tiles)."""
+    splitting the tiles; odd M with padding rows), and 4, 6, 8 and 10 modes with the parity
+    structure broken (full blocks, one class, one to four tiles)."""
     require_device()
     sp = configs.grin_fiber(M=M, nt=nt, symmetric=True, power_range=(1.0, 100.0))
     if broken:
