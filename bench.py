@@ -397,6 +397,30 @@ def run_batched(args) -> int:
             else:
                 finite.add((bi, f))
     finite_fibers = [idx[f] for bi, (_, idx, _, _) in enumerate(batches) for f in range(len(idx)) if (bi, f) in finite]
+
+    # the same warm-up and timed steps with the classes in order on one stream (one graph, no
+    # branches; a second set of batches, the tuner's cached plans): its wall time per step is what
+    # the classes' kernel times add up to, and the difference to ms_per_step is the branches' overlap
+    in_order_ms = None
+    if group is not None and args.batch_streams == "concurrent":
+        sctx = Context(0)
+        sb = [MultimodePropagationBatch([fibers[i] for i in idx], precision=prec, ctx=sctx, flags=bflags)
+              for _, idx, _, _ in batches]
+        sg = SolverGroup(sb, sctx)
+        if args.warmup:
+            sg.advance(args.warmup)
+            for bi, (b2, (_, idx, _, _)) in enumerate(zip(sb, batches)):
+                for f in range(len(idx)):
+                    b2.set_state(f, psi0[(bi, f)])
+            sctx.synchronize()
+        t1 = time.perf_counter()
+        sg.advance(steps)
+        sctx.synchronize()
+        in_order_ms = (time.perf_counter() - t1) * 1e3 / steps
+        sg.close()
+        for b2 in sb:
+            b2.close()
+        sctx.close()
     for bi, (key, idx, _, _) in enumerate(batches):
         plans[f"M{key[0]}_NT{key[1]}"]["finite_after_timed_steps"] = sum(1 for f in range(len(idx)) if (bi, f) in finite)
     units = sum(fibers[i].n_modes * fibers[i].nt for i in finite_fibers)
@@ -578,6 +602,9 @@ def run_batched(args) -> int:
         "kernel_sum_note": "each class's kernels timed alone with dispatch events, summed over classes; where "
                            "classes run on streams of their own their kernels overlap on the device, and "
                            "ms_per_step may then be below this sum",
+        "ms_per_step_classes_in_order": round(in_order_ms, 4) if in_order_ms is not None else None,
+        "ms_per_step_classes_in_order_note": "the same warm-up and timed steps of the same fibers with the nine "
+                                             "classes in order on one stream (one graph without branches)",
         "kernel_us_per_step_by_class": per_class,
         "kernel_us_per_step_by_class_and_kernel": per_class_kernel,
         "roofline": roofline, "cpu_baseline": cpu,
