@@ -1154,20 +1154,23 @@ mmp_status capture_group(mmp_group* g, int steps, hipGraph_t* graph_out, hipGrap
   mmp_status status = MMP_OK;
   const size_t nb = g->branches.size();
   // fork: every branch stream joins the capture behind this event
+  size_t forked = 0;
   if (nb) {
     if (hipEventRecord(g->events[0], st) != hipSuccess) status = fail(MMP_ERR_DEVICE, "hipEventRecord");
-    for (size_t b = 0; b < nb && status == MMP_OK; b++)
-      if (hipStreamWaitEvent(g->branches[b], g->events[0], 0) != hipSuccess)
-        status = fail(MMP_ERR_DEVICE, "hipStreamWaitEvent");
+    for (size_t b = 0; b < nb && status == MMP_OK; b++) {
+      if (hipStreamWaitEvent(g->branches[b], g->events[0], 0) == hipSuccess) forked++;
+      else status = fail(MMP_ERR_DEVICE, "hipStreamWaitEvent");
+    }
   }
   for (int k = 0; k < steps && status == MMP_OK; k++)
     for (mmp_solver* s : g->members)
       if (status == MMP_OK) status = enqueue_step(s);
-  // join every branch back into the group's stream
-  for (size_t b = 0; b < nb && status == MMP_OK; b++) {
+  // join every branch that joined the capture back into the group's stream, also after a failure
+  // (a capture ends only with its branches joined, and the branch streams leave capture mode then)
+  for (size_t b = 0; b < forked; b++) {
     if (hipEventRecord(g->events[1 + b], g->branches[b]) != hipSuccess ||
         hipStreamWaitEvent(st, g->events[1 + b], 0) != hipSuccess)
-      status = fail(MMP_ERR_DEVICE, "hipEventRecord / hipStreamWaitEvent");
+      if (status == MMP_OK) status = fail(MMP_ERR_DEVICE, "hipEventRecord / hipStreamWaitEvent");
   }
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(st, &graph);
