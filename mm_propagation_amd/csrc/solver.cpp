@@ -843,8 +843,11 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
     A(upload(s, d_cl, P.fused_line.data(), P.fused_line.size() * 4));
     A(upload(s, d_cs, P.fused_slot.data(), P.fused_slot.size() * 4));
     A(upload(s, d_meta, slot_meta.data(), P.P * 4));
+    // eight gather wavefronts from 4 to 6 modes (two per SIMD hide the slot loop's LDS latency);
+    // below 4 modes a column has at most 6 Raman lines and 9 slots, which four wavefronts cover
+    // (measured, 29 fibers of 3 modes at NT = 16384: 71 us with eight, 38 us with four)
     s->fused = FusedArgs{d_cl,   d_cs, d_meta, P.fused_chunks, P.fused_max_lines, P.fused_max_slots, compact ? 1 : 0,
-                         P.M <= 6 ? 8 : 4};
+                         P.M >= 4 && P.M <= 6 ? 8 : 4};
     s->chunk_lines = colstage_lines_per_chunk((int)P.M);
     colstage_pairs_plan((int)P.M, s->g.N2, s->n_lines, (int)s->F, s->mixed, s->pair_columns, s->pair_groups);
     // grids of 8 or more workgroups per CU
