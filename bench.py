@@ -118,7 +118,7 @@ def main() -> int:
                          "fastest (default for the batched workload, off for the others)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
     ap.add_argument("--matrix-cores", choices=["default", "on", "off"], default="default",
-                    help="mixed precision: the gather's coupling sums on the matrix cores by default at 4-8 and 13-16 modes "
+                    help="mixed precision: the gather's coupling sums on the matrix cores by default at 4-10 and 13-16 modes "
                          "(on: MMP_FLAG_MATRIX_CORES, wherever that gather exists; off: MMP_FLAG_NO_MATRIX_CORES)")
     ap.add_argument("--separate-columns", action="store_true",
                     help="every column pass a kernel of its own (7 per stage instead of 4)")

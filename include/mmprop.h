@@ -77,13 +77,13 @@ enum {
                                  rounding of the untuned solver (the sum order over slots changes). */,
   MMP_FLAG_NO_MATRIX_CORES = 1u << 4, /* mixed precision: keep the column-stage gather's coupling sums on
                                          the vector ALUs. By default they run as FP32 matrix products on
-                                         the matrix cores at 4 to 8 and 13 to 16 modes (the same
+                                         the matrix cores at 4 to 10 and 13 to 16 modes (the same
                                          arithmetic, another order of FP32 rounding), and the tuner tries
                                          both. */
   MMP_FLAG_MATRIX_CORES = 1u << 5     /* mixed precision: the coupling sums on the matrix cores wherever
                                          that gather exists (4 to 16 modes with parity-compact coupling
                                          blocks, 4 to 10 without), also where the default keeps them on
-                                         the vector ALUs (9 to 12 modes); the tuner then tries only such
+                                         the vector ALUs (11 and 12 modes); the tuner then tries only such
                                          variants. Not with MMP_FLAG_NO_MATRIX_CORES. The default does not
                                          depend on the number of fibers, so a batch and its fibers solved
                                          alone run the same arithmetic. */

@@ -852,17 +852,18 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
     colstage_pairs_plan((int)P.M, s->g.N2, s->n_lines, (int)s->F, s->mixed, s->pair_columns, s->pair_groups);
     // grids of 8 or more workgroups per CU
     s->fused.hi_occupancy = colstage_hi_occupancy_ok((int)P.M, s->mixed, compact) && (long)s->g.N2 * s->F >= 2048;
-    // the coupling sums on the matrix cores (FP32) by default at 4 to 8 and 13 to 16 modes,
+    // the coupling sums on the matrix cores (FP32) by default at 4 to 10 and 13 to 16 modes,
     // whatever the number of fibers (a batch runs the arithmetic of its fibers alone). Measured on
     // one MI355X, gather per launch against the vector gather: 7-8 modes 1.03-1.23x (one 32-row
     // tile, three workgroups per CU); 4-6 modes 1.06-1.31x on grids of 2048 workgroups and more,
     // 0.84-0.90x on small ones (against the eight-wavefront vector gather); 9-10 modes 1.04-1.19x
-    // on small grids but 0.84-0.92x on large ones, where two tiles hold the matrix-core gather to
-    // two workgroups per CU against three for the vector gather; 11-12 modes 0.89-1.05x; 13-16
-    // modes 1.03-1.25x on every grid (three or four tiles, the wavefront pairs splitting them).
+    // on the grids of one fiber or a few, 0.84-0.92x on large batches, where two tiles hold the
+    // matrix-core gather to two workgroups per CU against three for the vector gather (the tuner
+    // picks per batch); 11-12 modes 0.89-1.05x; 13-16 modes 1.03-1.25x on every grid (three or
+    // four tiles, the wavefront pairs splitting them).
     {
       const int M = (int)P.M;
-      const bool want = (opt.flags & MMP_FLAG_MATRIX_CORES) || (M >= 4 && M <= 8) || M >= 13;
+      const bool want = (opt.flags & MMP_FLAG_MATRIX_CORES) || (M >= 4 && M <= 10) || M >= 13;
       s->fused.matrix_cores =
           !(opt.flags & MMP_FLAG_NO_MATRIX_CORES) && want && colstage_matrix_cores_ok(M, s->mixed, compact) ? 1 : 0;
     }

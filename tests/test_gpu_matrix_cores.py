@@ -5,8 +5,8 @@ columns), then sigma_p = sum_k U_k B[p][k] per sample. Against the FP64 oracle a
 mixed-precision bar of tests/test_gpu_mixed.py (1e-6, or the dynamics' amplification of an FP64
 rounding-level perturbation scaled to FP32), and against the vector-ALU gather of the same
 problem (MMP_FLAG_NO_MATRIX_CORES), which differs from it by FP32 rounding only. The default takes
-the matrix cores at 4 to 8 and 13 to 16 modes whatever the number of fibers;
-MMP_FLAG_MATRIX_CORES takes them at 9 to 12 modes too."""
+the matrix cores at 4 to 10 and 13 to 16 modes whatever the number of fibers;
+MMP_FLAG_MATRIX_CORES takes them at 11 and 12 modes too."""
 import numpy as np
 import pytest
 
@@ -52,7 +52,7 @@ def test_matrix_core_gather_against_oracle_and_vector_gather(M, nt, broken):
     ra, ia = solve(sp, steps, MATRIX)
     rb, ib = solve(sp, steps, VALU)
     _, idef = solve(sp, 1)
-    assert idef["gather_matrix_cores"] == (1 if M <= 8 or M >= 13 else 0), idef
+    assert idef["gather_matrix_cores"] == (1 if M <= 10 or M >= 13 else 0), idef
     assert ia["gather_matrix_cores"] == 1 and ia["gather_waves"] == 4, ia
     assert ib["gather_matrix_cores"] == 0, ib
     o = ob.Oracle(sp, pair_decode=1, n_threads=16)
@@ -84,13 +84,12 @@ def test_matrix_core_batch_equals_single_solves():
         assert np.array_equal(got[f], r), f
 
 
-@pytest.mark.parametrize("M,nt,fibers,expected", [(10, 8192, 16, (1, 0)), (10, 8192, 1, (0, 0)), (6, 16384, 8, (0, 1)),
+@pytest.mark.parametrize("M,nt,fibers,expected", [(10, 8192, 16, (0, 1)), (10, 8192, 1, (0, 1)), (6, 16384, 8, (0, 1)),
                                                   (6, 16384, 1, (0, 1)), (8, 16384, 8, (0, 1)), (4, 4096, 2, (0, 1)),
                                                   (12, 4096, 2, (0, 0)), (15, 4096, 2, (0, 1))])
 def test_default_gather_variant(M, nt, fibers, expected):
-    """(gather_hi_occupancy, gather_matrix_cores) chosen at creation: 4-8 and 13-16 modes take the
-    matrix cores, 9-12 modes the vector gather (three workgroups per CU on grids of 2048
-    workgroups and more at 9-10 modes), whatever the number of fibers."""
+    """(gather_hi_occupancy, gather_matrix_cores) chosen at creation: 4-10 and 13-16 modes take the
+    matrix cores, 11 and 12 modes the vector gather, whatever the number of fibers."""
     require_device()
     sps = [configs.grin_fiber(M=M, nt=nt, seed=i, symmetric=True) for i in range(fibers)]
     b = MultimodePropagationBatch(sps, pair_decode=1, precision=MIXED)
