@@ -531,6 +531,13 @@ def run_batched(args) -> int:
     kt = pb.kernel_times()
     info = pb.info(0)
     pb.close()
+    # every pass of that class launch: algorithmic bytes over the dispatch-recorded time (the
+    # transforms' HBM rates beside the dominant kernel's)
+    passes = {p["name"]: {"avg_us": round(p["total_ms"] * 1e3 / max(p["launches"], 1), 3),
+                          "gbps": round(p["bytes_per_launch"] / (p["total_ms"] / 1e3 / max(p["launches"], 1)) / 1e9, 1),
+                          "hbm_frac": round(p["bytes_per_launch"] / (p["total_ms"] / 1e3 / max(p["launches"], 1))
+                                            / 1e9 / HBM_PEAK_GBS, 4)}
+              for p in kt if p["launches"] > 0}
     dom = (max(kt, key=lambda k: k["total_ms"]), key, fin, info)
     k, key, idx, info = dom
     avg_s = k["total_ms"] / 1e3 / max(k["launches"], 1)
@@ -567,7 +574,7 @@ def run_batched(args) -> int:
     if traffic:
         roofline.update({"traffic": traffic, "traffic_source": "profiles/hbm_traffic_batched.md (rocprofv3 FETCH_SIZE "
                                                                "x2.0 + WRITE_SIZE x1.0, copy-calibrated)"})
-    roofline.update({"class": cls, "avg_launch_us": round(avg_s * 1e6, 3),
+    roofline.update({"class": cls, "avg_launch_us": round(avg_s * 1e6, 3), "passes": passes,
                      "peak_note": "FP32 peak 157.3 TFLOP/s: the matrix cores' dense FP32 rate, and the vector rate "
                                   "with packed FP32 instructions, which this build does not emit (78.6 TFLOP/s "
                                   "without them, as FP64)"})
