@@ -295,3 +295,42 @@ def test_group_with_members_on_their_own_streams():
         m.close()
     for c in (gctx, ca, cb):
         c.close()
+
+
+def test_group_of_branches_only_with_parked_fibers():
+    """Every member on a context of its own (the group's stream only forks and joins) and fibers
+    that overflow on the way (parked by the device): group steps equal the members' own steps,
+    bit for bit, parking included."""
+    from mm_propagation_amd.batch import SolverGroup
+    from mm_propagation_amd.solver import Context
+    require_device()
+    gctx, ca, cb = Context(0), Context(0), Context(0)
+    a_sps = fibers(3, 4096, 3, base=130)
+    a_sps[1] = configs.grin_fiber(M=3, nt=4096, n_steps=20000, fiber_length=1.0, seed=5, power_range=(1e6, 1e7),
+                                  symmetric=True)  # overflows within a few steps
+    b_sps = fibers(6, 4096, 2, base=135)
+    note("case solver group of branches only, with a parked fiber")
+
+    def make():
+        return (MultimodePropagationBatch(a_sps, precision=_capi.MMP_PRECISION_MIXED, ctx=ca),
+                MultimodePropagationBatch(b_sps, ctx=cb))
+
+    members = make()
+    g = SolverGroup(members, gctx)
+    g.advance(0)
+    g.advance(21)
+    got = [[members[0].info(f)["halted"] for f in range(3)],
+           [members[0].result(f) for f in (0, 2)], [members[1].result(f) for f in range(2)]]
+    g.close()
+    alone = make()
+    for m in alone:
+        m.advance(21)
+    assert got[0] == [alone[0].info(f)["halted"] for f in range(3)] and got[0][1] == 1
+    for r, f in zip(got[1], (0, 2)):
+        assert np.array_equal(r, alone[0].result(f)), f
+    for f in range(2):
+        assert np.array_equal(got[2][f], alone[1].result(f)), f
+    for m in members + alone:
+        m.close()
+    for c in (gctx, ca, cb):
+        c.close()
