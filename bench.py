@@ -260,7 +260,7 @@ def main() -> int:
         "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if mixed else "f64", "data": "synthetic",
         "config": {"workload": desc + (" (run in mixed precision)" if args.mixed else ""),
-                   "global_batch": 1, "seq_len": NT, "parallelism": "single GPU"},
+                   "fibers": 1, "nt": NT, "devices": "one GPU"},
         "accuracy": accuracy, "roofline": roofline, "cpu_baseline": cpu, "gather_plan": tuned,
     }
     print(json.dumps(out), flush=True)
@@ -594,7 +594,7 @@ def run_batched(args) -> int:
                                   if group is not None or args.batch_streams == "per-class"
                                   else "one stream, eager launches")
                                + (", autotuned" if autotune else ""),
-                   "global_batch": args.fibers, "seq_len": "4096-16384", "parallelism": "single GPU"},
+                   "fibers": args.fibers, "nt": "4096-16384", "devices": "one GPU"},
         "fibers_finite": len(finite),
         "fibers_parked": {"count": len(halted_at), "steps_before_parking_median":
                           float(np.median(halted_at)) if halted_at else None},
@@ -782,7 +782,7 @@ def run_serving(args) -> int:
                                "M in {1,3,6,10,15}, NT in {4096..65536}, n_steps log-uniform [500, 20000], "
                                "L U(0.2,1) m, T0 U(0.05,0.3) ps, P log-uniform [10 W, 10 kW], symmetric GRIN "
                                f"coupling, RK4, Raman on, mixed precision, {args.slots} slots per batch group",
-                   "global_batch": args.requests, "seq_len": "4096-65536", "parallelism": "single GPU"},
+                   "requests": args.requests, "nt": "4096-65536", "devices": "one GPU"},
         "latency_s": {"p50": stats["p50_latency_s"], "p99": stats["p99_latency_s"],
                       "max": max(r.latency for r in reqs)},
         "makespan_s": makespan,
