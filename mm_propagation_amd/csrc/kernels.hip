@@ -90,7 +90,9 @@ __device__ __forceinline__ cd cmul_exact(cd a, cd b) {
           __dadd_rn(__dmul_rn(a.re, b.im), __dmul_rn(a.im, b.re))};
 }
 
-template <int L, class W>
+// RE: Lawson factors computed from Im L and the fiber's z (HeadArgs::recompute_e) instead of
+// read from the tables; they are formed where they are used, so no table registers stay live.
+template <int L, class W, bool RE>
 __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_row_head(Geometry g, const cpx<W>* __restrict__ rowtw, HeadArgs a, Batch b) {
   using C = RowCfg<L>;
   using cw = cpx<W>;
@@ -112,7 +114,8 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_row_head(Geometry g, con
   cd* __restrict__ stash = a.stash ? a.stash + fib * b.ss : nullptr;
   cd* __restrict__ e_prev = a.e_prev ? a.e_prev + fib * b.se : nullptr;
   const cd* __restrict__ e_cur = a.e_cur ? a.e_cur + fib * b.se : nullptr;
-  const cd* __restrict__ e1p = a.e1 + fib * b.sf;
+  const cd* __restrict__ e1p = a.e1 ? a.e1 + fib * b.sf : nullptr;
+  const double* __restrict__ imLp = a.imL ? a.imL + fib * b.sf : nullptr;
   const cw* __restrict__ ysig = static_cast<const cw*>(a.ysig) + fib * b.sf;
   cw* __restrict__ yout = static_cast<cw*>(a.yout) + fib * b.sf;
   LineTw<L, W> lt;
@@ -120,19 +123,35 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_row_head(Geometry g, con
 
   // issue every load this lane needs before the first barrier
   cw v[8];
-  cd x[8], accb[8], ep[8], ec[8], e1[8];
+  cd x[8], accb[8];
+  cd ep[RE ? 1 : 8], ec[RE ? 1 : 8], e1[RE ? 1 : 8];  // Lawson tables (not RE)
+  double lam[RE ? 8 : 1];                              // Im L (RE)
   const cd zero = {0., 0.};
+  const bool want_cur = !a.final_stage || a.chain_next;
 #pragma unroll
   for (int i = 0; i < 8; i++) {
     const size_t idx = base + i * C::T;
+    if constexpr (RE) lam[i] = valid ? imLp[idx] : 0.;
     v[i] = (valid && a.has_prev) ? ysig[idx] : cw{0, 0};
-    ep[i] = (valid && a.has_prev) ? e_prev[idx] : zero;
+    if constexpr (!RE) ep[i] = (valid && a.has_prev) ? e_prev[idx] : zero;
     const cd psi = (valid && (a.prev_is_first || !a.final_stage)) ? psi_hat[idx] : zero;
     const cd accin = (valid && a.has_prev && !a.prev_is_first) ? accp[idx] : zero;
     accb[i] = a.prev_is_first ? psi : accin;  // the step result so far
     x[i] = psi;                               // stage i's argument (not the final stage)
-    e1[i] = (valid && a.update_prev) ? e1p[idx] : zero;
-    ec[i] = (valid && (!a.final_stage || a.chain_next)) ? e_cur[idx] : zero;
+    if constexpr (!RE) {
+      e1[i] = (valid && a.update_prev) ? e1p[idx] : zero;
+      ec[i] = (valid && want_cur) ? e_cur[idx] : zero;
+    }
+  }
+  // RE: E_{i-1} = exp(i Im L (z + c_{i-1} h)), E_i = exp(i Im L (z + c_i h)) with z the step's
+  // start; the close head's block 0 advances z in this kernel, so it reads the start from zs
+  // (recorded by stage head 1) and its chained next stage 0 is at z + h
+  double tp = 0., tc = 0.;
+  if constexpr (RE) {
+    const double z0 = a.final_stage ? b.st[fib].zs : b.st[fib].z;  // uniform
+    tp = z0 + a.c_prev * fp.h;
+    tc = (a.final_stage ? z0 + fp.h : z0) + a.c_cur * fp.h;
+    if (a.set_zs && blockIdx.x == 0 && threadIdx.x == 0) b.st[fib].zs = z0;
   }
   if (a.final_stage && a.advance_z && blockIdx.x == 0 && threadIdx.x == 0) {
     b.st[fib].z += fp.h;  // z += h (repeated FP64 addition)
@@ -154,9 +173,11 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_row_head(Geometry g, con
     }
   }
   // the Lawson table of stage i-1 moves on by E1 (it is not read again in this step)
-  if (a.has_prev && a.update_prev && valid) {
+  if constexpr (!RE) {
+    if (a.has_prev && a.update_prev && valid) {
 #pragma unroll
-    for (int i = 0; i < 8; i++) store_stream(e_prev + base + i * C::T, cmul_exact(ep[i], e1[i]));
+      for (int i = 0; i < 8; i++) store_stream(e_prev + base + i * C::T, cmul_exact(ep[i], e1[i]));
+    }
   }
   if (a.has_prev) {
     const double bh = hc[0];
@@ -166,7 +187,10 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_row_head(Geometry g, con
 #pragma unroll
     for (int i = 0; i < 8; i++) {
       const size_t idx = base + i * C::T;
-      const cd y = cmulc(widen(v[i]), ep[i]);     // sigma_hat * conj(E_{i-1})
+      cd epi;
+      if constexpr (RE) epi = phase(lam[i], tp);
+      else epi = ep[i];
+      const cd y = cmulc(widen(v[i]), epi);         // sigma_hat * conj(E_{i-1})
       const cd nh = {-y.im * fp.sgn, y.re * fp.sgn};  // * (i sgn) = N_{i-1}
       cd acc = accb[i];
       if (a.b_prev != 0.) acc = cadd(acc, cscale(nh, bh));
@@ -188,7 +212,12 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_row_head(Geometry g, con
   }
   if (a.final_stage && !a.chain_next) return;
 #pragma unroll
-  for (int i = 0; i < 8; i++) v[i] = narrow<W>(cscale(cmul(ec[i], x[i]), a.scale_cur));
+  for (int i = 0; i < 8; i++) {
+    cd eci;
+    if constexpr (RE) eci = phase(lam[i], tc);
+    else eci = ec[i];
+    v[i] = narrow<W>(cscale(cmul(eci, x[i]), a.scale_cur));
+  }
   line_fft<L, 1, 1, 1>(v, sm, lt, t);
   if (valid) {
 #pragma unroll
@@ -1461,7 +1490,13 @@ static hipError_t row_head_impl(const Geometry& g, const Twiddles& tw, const Hea
   using C = RowCfg<L>;
   const int rows = g.M * g.N1;
   dim3 grid((rows + C::G - 1) / C::G, F);
-  MMP_LAUNCH((k_row_head<L, W>), grid, dim3(C::THREADS), 0, s, g, TwOf<W>::row(tw), a, b);
+  // the table-free form only for FP32 work arrays (mixed precision)
+  if (a.recompute_e && !std::is_same<W, float>::value) return hipErrorInvalidValue;
+  if (a.recompute_e)
+    MMP_LAUNCH((k_row_head<L, W, std::is_same<W, float>::value>), grid, dim3(C::THREADS), 0, s, g,
+               TwOf<W>::row(tw), a, b);
+  else
+    MMP_LAUNCH((k_row_head<L, W, false>), grid, dim3(C::THREADS), 0, s, g, TwOf<W>::row(tw), a, b);
   return hipGetLastError();
 }
 template <class W>

@@ -251,22 +251,24 @@ mmp_status enqueue_head_w(mmp_solver* s, int i) {
   const double cb = sizeof(cpx<W>), cs = 16.0;
   const double MN = (double)P.M * P.N;
   // algorithmic bytes: every array the head reads or writes, once
+  // (Lawson tables only when they are not computed from Im L, which is then read once, 8 bytes)
+  const int tab = h.recompute_e ? 0 : 1;
   int state_arrays = 0, work_arrays = 0;
   if (h.has_prev) {
     work_arrays += 1;                                  // sigma
-    state_arrays += 2;                                 // E_{i-1}, acc or psi_hat written
+    state_arrays += 1 + tab;                           // E_{i-1}, acc or psi_hat written
     if (!h.prev_is_first) state_arrays += 1;           // acc read
-    if (h.update_prev) state_arrays += 2;              // E1 read, E_{i-1} written
+    if (h.update_prev) state_arrays += 2 * tab;        // E1 read, E_{i-1} written
     if (h.stash_prev) state_arrays += 1;               // N_{i-1} stashed
   }
   if (h.prev_is_first || !h.final_stage) state_arrays += 1;  // psi_hat read
   if (!h.final_stage || h.chain_next) {
-    state_arrays += 1;                                 // E_i (E_0 of the next step when chained)
+    state_arrays += tab;                               // E_i (E_0 of the next step when chained)
     work_arrays += 1;                                  // row-inverse output
     for (int e = 0; e < h.n_a; e++)
       if (h.a_idx[e] != h.prev_stage) state_arrays += 1;  // stashed N_j
   }
-  const double hb = MN * (cs * state_arrays + cb * work_arrays);
+  const double hb = MN * (cs * state_arrays + cb * work_arrays + (h.recompute_e ? 8.0 : 0.0));
   return run_kernel(s, i == P.rk.s ? "row_close_step" : "row_stage_head", hb * s->F,
                     [&] { return launch_row_head<W>(s->g, s->tw, h, s->batch, (int)s->F, s->ctx->stream); });
 }
@@ -403,12 +405,13 @@ mmp_status refresh_halted(mmp_solver* s);
 
 mmp_status reset_z(mmp_solver* s) {
   MMP_TRY(refresh_halted(s));
-  std::vector<StepState> zero(s->F, StepState{0., 0u, 0u});
+  std::vector<StepState> zero(s->F, StepState{0., 0u, 0u, 0.});
   for (uint32_t f = 0; f < s->F; f++) zero[f].halted = s->halted[f];  // a non-finite field stays parked
   MMP_TRY(upload(s, s->st, zero.data(), zero.size() * sizeof(StepState)));
-  HIP_TRY(hipMemcpyAsync(s->etab, s->etab0, (size_t)s->F * s->ntab * s->P.M * s->P.N * sizeof(cd),
-                         hipMemcpyDeviceToDevice, s->ctx->stream),
-          "hipMemcpyAsync D2D");
+  if (s->etab)
+    HIP_TRY(hipMemcpyAsync(s->etab, s->etab0, (size_t)s->F * s->ntab * s->P.M * s->P.N * sizeof(cd),
+                           hipMemcpyDeviceToDevice, s->ctx->stream),
+            "hipMemcpyAsync D2D");
   HIP_TRY(hipStreamSynchronize(s->ctx->stream), "hipStreamSynchronize");  // `zero` is a host staging copy
   std::fill(s->z_host.begin(), s->z_host.end(), 0.);
   std::fill(s->steps.begin(), s->steps.end(), 0u);
@@ -434,11 +437,12 @@ mmp_status refresh_halted(mmp_solver* s) {
 
 // z = 0 and the initial Lawson tables for fiber f only.
 mmp_status reset_fiber(mmp_solver* s, uint32_t f) {
-  static const StepState zero{0., 0u, 0u};
+  static const StepState zero{0., 0u, 0u, 0.};
   MMP_TRY(upload(s, s->st + f, &zero, sizeof zero));
-  HIP_TRY(hipMemcpyAsync(s->etab + f * s->batch.se, s->etab0 + f * s->batch.se, s->batch.se * sizeof(cd),
-                         hipMemcpyDeviceToDevice, s->ctx->stream),
-          "hipMemcpyAsync D2D");
+  if (s->etab)
+    HIP_TRY(hipMemcpyAsync(s->etab + f * s->batch.se, s->etab0 + f * s->batch.se, s->batch.se * sizeof(cd),
+                           hipMemcpyDeviceToDevice, s->ctx->stream),
+            "hipMemcpyAsync D2D");
   s->z_host[f] = 0.;
   s->steps[f] = 0;
   return MMP_OK;
@@ -481,7 +485,10 @@ void release(mmp_solver* s) {
   delete s;
 }
 
-// Device bytes create_impl allocates for problem P (kept in step with its dalloc calls).
+// Lawson tables E_u (one per distinct c_u, advanced by E1 each step) exist in FP64 only: the
+// mixed-precision stage heads form E_u from Im L and the fiber's z where they use it.
+bool lawson_tables(const Problem& P, bool mixed) { return !(mixed && P.rk.s >= 2); }
+
 // Device bytes create_impl allocates for F fibers of problem P (kept in step with its dalloc calls).
 size_t planned_device_bytes(const Problem& P, const mmp_options& opt, size_t F) {
   auto c = [](size_t n) { return n ? n : (size_t)1; };
@@ -491,7 +498,7 @@ size_t planned_device_bytes(const Problem& P, const mmp_options& opt, size_t F) 
   for (int j = 0; j < P.rk.s; j++) any_stash = any_stash || P.stash_needed[j];
   size_t b = 16 * MN * (2 * F + 2 * Fw) + 8 * MN * F + 16 * (P.N2 + P.N1 + P.N) + sizeof(StepState) * F +
              sizeof(FiberParams) * F + 8 * kHeadCoef * ((size_t)P.rk.s + 1) * F + 8 * c(P.M);
-  b += 16 * MN * F * (2 * P.c_unique.size() + 1);
+  if (lawson_tables(P, mixed)) b += 16 * MN * F * (2 * P.c_unique.size() + 1);
   if (any_stash) b += 16 * (size_t)P.rk.s * MN * F;
   if (P.raman) b += (mixed ? 8 : 16) * c(L) * P.N * F + 16 * (size_t)P.N * F + 8 * (size_t)P.N;
   b += 4 * c(P.coff.size()) + 4 * c(P.nnz) + 8 * c(P.nnz) + 4 * 4 * c(P.P) + 4 * 3 * c(L);
@@ -578,9 +585,11 @@ mmp_status load_fiber(mmp_solver* s, uint32_t f, Problem& Pf) {
   const Batch& b = s->batch;
   const size_t MN = (size_t)Pf.M * Pf.N;
   MMP_TRY(upload(s, s->imL + f * b.sf, Pf.imL_fourstep.data(), MN * 8));
-  MMP_TRY(upload(s, s->e1 + f * b.sf, Pf.e1.data(), MN * 16));
-  MMP_TRY(upload(s, s->etab0 + f * b.se, Pf.etab0.data(), Pf.etab0.size() * sizeof(double)));
-  MMP_TRY(upload(s, s->etab + f * b.se, Pf.etab0.data(), Pf.etab0.size() * sizeof(double)));
+  if (s->etab) {
+    MMP_TRY(upload(s, s->e1 + f * b.sf, Pf.e1.data(), MN * 16));
+    MMP_TRY(upload(s, s->etab0 + f * b.se, Pf.etab0.data(), Pf.etab0.size() * sizeof(double)));
+    MMP_TRY(upload(s, s->etab + f * b.se, Pf.etab0.data(), Pf.etab0.size() * sizeof(double)));
+  }
   std::vector<double> qd;
   std::vector<float> qd32;
   if (s->qdense_w) {
@@ -602,7 +611,7 @@ mmp_status load_fiber(mmp_solver* s, uint32_t f, Problem& Pf) {
     for (int e = 0; e < a.n_a; e++) coef[i * kHeadCoef + 1 + e] = a.a_coef[e] * Pf.delta_z;
   }
   MMP_TRY(upload(s, s->d_coef + f * b.nheads * kHeadCoef, coef.data(), coef.size() * sizeof(double)));
-  const StepState zero{0., 0u, 0u};
+  const StepState zero{0., 0u, 0u, 0.};
   MMP_TRY(upload(s, s->st + f, &zero, sizeof zero));
   if (Pf.raman) {
     cd* hs = s->hspec + f * b.sh;
@@ -712,9 +721,11 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
   A(dalloc(s, &s->ybuf, Fw * MN));
   A(dalloc(s, &s->U, Fw * MN));
   A(dalloc(s, &s->imL, F * MN));
-  A(dalloc(s, &s->etab, F * s->batch.se));
-  A(dalloc(s, &s->etab0, F * s->batch.se));
-  A(dalloc(s, &s->e1, F * MN));
+  if (lawson_tables(P, s->mixed)) {
+    A(dalloc(s, &s->etab, F * s->batch.se));
+    A(dalloc(s, &s->etab0, F * s->batch.se));
+    A(dalloc(s, &s->e1, F * MN));
+  }
   A(dalloc(s, &rowtw, P.N2));
   A(dalloc(s, &coltw, P.N1));
   A(dalloc(s, &fourtw, P.N));
@@ -888,15 +899,24 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
     a.acc = s->acc;
     a.stash = s->stash;
     a.yout = s->ybuf_w;
-    a.e1 = s->e1;
+    a.e1 = s->e1;  // null without tables
     a.scale_cur = 1.0;
     a.has_prev = i > 0;
     a.prev_stage = i - 1;
     a.prev_is_first = (i == 1);
     a.final_stage = (i == P.rk.s);
+    // mixed precision: Lawson factors from Im L and the FP64 z (no tables exist); the close
+    // head takes the step's start z from StepState::zs, which head 1 records
+    if (!lawson_tables(P, s->mixed)) {
+      a.recompute_e = 1;
+      a.imL = s->imL;
+      a.set_zs = (i == 1);
+      if (i > 0) a.c_prev = P.rk.c[i - 1];
+      a.c_cur = (i < P.rk.s) ? P.rk.c[i] : P.rk.c[0];
+    }
     if (i > 0) {
       const int u = P.stage_table[i - 1];
-      a.e_prev = s->etab + (size_t)u * MN;
+      a.e_prev = s->etab ? s->etab + (size_t)u * MN : nullptr;
       a.update_prev = (P.table_last_head[u] == i);
       a.b_prev = P.rk.b[i - 1];
       a.stash_prev = P.stash_needed[i - 1];
@@ -905,11 +925,11 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
       a.advance_z = 1;
       if (s->chain) {
         a.chain_next = 1;
-        a.e_cur = s->etab + (size_t)P.stage_table[0] * MN;
+        a.e_cur = s->etab ? s->etab + (size_t)P.stage_table[0] * MN : nullptr;
       }
     }
     if (i < P.rk.s) {
-      a.e_cur = s->etab + (size_t)P.stage_table[i] * MN;
+      a.e_cur = s->etab ? s->etab + (size_t)P.stage_table[i] * MN : nullptr;
       for (int j = 0; j < i; j++)
         if (P.rk.a[i][j] != 0.) {
           a.a_idx[a.n_a] = j;
@@ -1103,7 +1123,7 @@ mmp_status autotune(mmp_solver* s) {
     if (!s->halted[f]) continue;
     const uint32_t v = live_before[f];
     st = upload(s, reinterpret_cast<char*>(s->d_fp + f) + offsetof(FiberParams, live), &v, sizeof v);
-    static const StepState zero{0., 0u, 0u};  // clears StepState::halted before reset_z reads it
+    static const StepState zero{0., 0u, 0u, 0.};  // clears StepState::halted before reset_z reads it
     if (st == MMP_OK) st = upload(s, s->st + f, &zero, sizeof zero);
     s->halted[f] = 0;
     s->live[f] = live_before[f];

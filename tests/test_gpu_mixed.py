@@ -129,3 +129,27 @@ def test_mixed_separate_column_passes(M, nt):
     err, _, _ = run_mixed(sp, 50, decode=1, flags=_capi.MMP_FLAG_SEPARATE_COLUMNS)
     print("rel-L2", err)
     assert np.all(err <= bar(sp, 50, decode=1)), err
+
+
+def test_mixed_lawson_factors_follow_z():
+    """Mixed precision forms the Lawson factors from Im L and the fiber's z instead of advancing
+    tables: a solve split into advance() calls (16-step and 1-step graphs, the first stage head
+    issued by each call) equals the same solve in one call bit for bit, and it holds the oracle's
+    bar after the same steps."""
+    require_device()
+    sp = configs.grin_fiber(M=6, nt=4096, symmetric=True)
+    steps = 37
+    err, whole, _ = run_mixed(sp, steps, decode=1)
+    print("rel-L2", err)
+    assert np.all(err <= bar(sp, steps, decode=1)), err
+    note("case mixed split advance 5 + 16 + 16")
+    s = MultimodePropagationGPU_RamanON(sp, pair_decode=1, precision=MIXED)
+    s.computeLawsonRK(5)
+    _, z5 = s.get_state()
+    s.advance(16)
+    s.advance(16)
+    split = s.getResult()
+    _, z = s.get_state()
+    s.close()
+    assert np.array_equal(split, whole)
+    assert z == pytest.approx(z5 * steps / 5, rel=1e-12)
