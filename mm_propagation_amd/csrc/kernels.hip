@@ -146,11 +146,15 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_row_head(Geometry g, con
   // RE: E_{i-1} = exp(i Im L (z + c_{i-1} h)), E_i = exp(i Im L (z + c_i h)) with z the step's
   // start; the close head's block 0 advances z in this kernel, so it reads the start from zs
   // (recorded by stage head 1) and its chained next stage 0 is at z + h
+  // (both at one z, as in a stage that repeats c or a close with c_{s-1} = 1: E_{i-1} serves
+  // both, applied to the stage product as soon as that is formed)
   double tp = 0., tc = 0.;
+  bool same = false;
   if constexpr (RE) {
     const double z0 = a.final_stage ? b.st[fib].zs : b.st[fib].z;  // uniform
     tp = z0 + a.c_prev * fp.h;
     tc = (a.final_stage ? z0 + fp.h : z0) + a.c_cur * fp.h;
+    same = a.has_prev && want_cur && tp == tc;
     if (a.set_zs && blockIdx.x == 0 && threadIdx.x == 0) b.st[fib].zs = z0;
   }
   if (a.final_stage && a.advance_z && blockIdx.x == 0 && threadIdx.x == 0) {
@@ -202,6 +206,9 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_row_head(Geometry g, con
       }
       if (a.final_stage) x[i] = acc;  // chain_next: stage 0 of the next step from the new psi_hat
       else if (prev_term) x[i] = cadd(x[i], cscale(nh, cprev));
+      if constexpr (RE) {
+        if (same) x[i] = cmul(epi, x[i]);  // E_i x_i
+      }
     }
     // a state that is no longer finite stays so: the fiber is parked (every lane that sees it
     // stores the same values), its later kernels skip it and its z and step count stop here
@@ -213,10 +220,10 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_row_head(Geometry g, con
   if (a.final_stage && !a.chain_next) return;
 #pragma unroll
   for (int i = 0; i < 8; i++) {
-    cd eci;
-    if constexpr (RE) eci = phase(lam[i], tc);
-    else eci = ec[i];
-    v[i] = narrow<W>(cscale(cmul(eci, x[i]), a.scale_cur));
+    cd ex;  // E_i x_i
+    if constexpr (RE) ex = same ? x[i] : cmul(phase(lam[i], tc), x[i]);
+    else ex = cmul(ec[i], x[i]);
+    v[i] = narrow<W>(cscale(ex, a.scale_cur));
   }
   line_fft<L, 1, 1, 1>(v, sm, lt, t);
   if (valid) {
