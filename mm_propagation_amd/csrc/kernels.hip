@@ -64,8 +64,15 @@ __device__ __forceinline__ int xcd_block(int bid, int nb) {
 // share lines with this one).
 template <class T>
 __device__ __forceinline__ void store_stream(cpx<T>* p, cpx<T> v) {
-  __builtin_nontemporal_store(v.re, &p->re);
-  __builtin_nontemporal_store(v.im, &p->im);
+  if constexpr (sizeof(T) == 4) {
+    // one 8-byte store per FP32 value (p is 8-byte aligned: cpx<float> is)
+    const unsigned long long bits =
+        (unsigned long long)__float_as_uint(v.re) | ((unsigned long long)__float_as_uint(v.im) << 32);
+    __builtin_nontemporal_store(bits, reinterpret_cast<unsigned long long*>(p));
+  } else {
+    __builtin_nontemporal_store(v.re, &p->re);
+    __builtin_nontemporal_store(v.im, &p->im);
+  }
 }
 // The same in FP64 only: FP32 stores of the column kernels measured slower streamed.
 template <class T>
