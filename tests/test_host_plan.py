@@ -122,3 +122,14 @@ def test_validation_statuses():
     sp.raman_response = None
     assert plan.plan(sp, raman=False)["n_conv_lines"] == 0
     assert plan.plan(base)["n_conv_lines"] == 2
+
+
+def test_mixed_precision_plan_holds_no_lawson_tables():
+    """Mixed precision forms the Lawson factors from Im L and z: its footprint lacks the FP64
+    form's three RK4 tables, their restart copies and E1 (7 x 16 B per sample), on top of the
+    FP32 work arrays' saving."""
+    M, nt = 6, 16384
+    sp = configs.grin_fiber(M=M, nt=nt, symmetric=True)
+    fp64 = plan.plan(sp, pair_decode=1)["device_bytes"]
+    mixed = plan.plan(sp, pair_decode=1, precision=MIXED)["device_bytes"]
+    assert fp64 - mixed >= 7 * 16 * M * nt
