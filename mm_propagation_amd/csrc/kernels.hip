@@ -553,7 +553,15 @@ __global__ void __launch_bounds__((RamanColsCfg<L, W>::THREADS)) k_row_raman_col
   // the parts of a cache line that several of them store meet in one L2 (measured: rows of 1024
   // 23.7 -> 21.7 us); FP64 runs of 8 rows already fill 128-byte lines, and there the plain
   // order measured slightly faster (rows of 256: 5.09 vs 5.21 us)
-  const int row0 = (std::is_same<W, float>::value ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x) * C::R;
+  // Where those FP32 runs fill whole 128-byte lines (R = 16) the blocks take the lines fastest
+  // instead, so the blocks on one XCD share row groups and read only those rows of the Raman
+  // spectrum (every XCD reading all of it fetched the spectrum 8 times from HBM)
+  const int blk = std::is_same<W, float>::value ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  int row0 = blk * C::R;
+  if constexpr (std::is_same<W, float>::value && C::R * sizeof(cw) >= 128) {
+    const int nl = (int)gridDim.x * C::R / g.N1;  // lines of the launch
+    if (gridDim.x % 8 == 0) row0 = (blk % nl) * g.N1 + (blk / nl) * C::R;
+  }
   const int line = row0 / g.N1, n10 = row0 % g.N1, n1 = n10 + r;
   const size_t cbase = (size_t)line * g.N + n10;  // the R rows' first sample in column order
   const size_t hbase = (size_t)n1 * g.N2;
