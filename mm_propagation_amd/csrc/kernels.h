@@ -67,7 +67,7 @@ struct StepState {
   double z;
   uint32_t steps;
   uint32_t halted;
-  double zs;  // z at the start of the step, for the step-closing head (written by stage head 1)
+  double n0;  // steps at the start of the step, for the step-closing head (written by stage head 1)
 };
 
 // Per-fiber scalars of a batch of fibers that share M, NT, the tableau and the pair structure.
@@ -128,11 +128,11 @@ struct HeadArgs {
   int a_idx[kMaxStages];    // j (N_{i-1} comes from registers, earlier N_j from stash)
   double a_coef[kMaxStages];  // a_ij (the fiber's a_ij h come from Batch::coef)
   // Lawson factors computed from the FP64 phase instead of read from the tables (mixed precision):
-  // E_u = exp(i Im L (z + c_u h)) with z the fiber's z at the start of the step (StepState::z,
-  // or StepState::zs in the step-closing head, whose own block 0 advances z), the chained next
-  // stage 0 at z + h. No table is read or written.
+  // E_u = exp(i Im L (n + c_u) h) with n the fiber's step count at the start of the step
+  // (StepState::steps, or StepState::n0 in the step-closing head, whose own block 0 advances
+  // the count), the chained next stage 0 at n + 1. No table is read or written.
   int recompute_e;
-  int set_zs;          // stage head 1: StepState::zs = z
+  int set_n0;          // stage head 1: StepState::n0 = steps
   double c_prev;       // c_{i-1}
   double c_cur;        // c_i (c_0 for the chained next stage 0)
   const double* imL;   // Im L in four-step order ([F][M][N], stride Batch::sf)

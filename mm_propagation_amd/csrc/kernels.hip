@@ -150,19 +150,20 @@ __global__ void __launch_bounds__(RowCfg<L>::THREADS) k_row_head(Geometry g, con
       ec[i] = (valid && want_cur) ? e_cur[idx] : zero;
     }
   }
-  // RE: E_{i-1} = exp(i Im L (z + c_{i-1} h)), E_i = exp(i Im L (z + c_i h)) with z the step's
-  // start; the close head's block 0 advances z in this kernel, so it reads the start from zs
-  // (recorded by stage head 1) and its chained next stage 0 is at z + h
-  // (both at one z, as in a stage that repeats c or a close with c_{s-1} = 1: E_{i-1} serves
-  // both, applied to the stage product as soon as that is formed)
+  // RE: E_{i-1} = exp(i Im L (n + c_{i-1}) h), E_i = exp(i Im L (n + c_i) h) with n the steps
+  // taken before this one (one rounding of the distance, where z sums h step by step); the
+  // close head's block 0 advances the count in this kernel, so it reads n from n0 (recorded by
+  // stage head 1) and its chained next stage 0 is at n + 1
+  // (both at one distance, as in a stage that repeats c or a close with c_{s-1} = 1: E_{i-1}
+  // serves both, applied to the stage product as soon as that is formed)
   double tp = 0., tc = 0.;
   bool same = false;
   if constexpr (RE) {
-    const double z0 = a.final_stage ? b.st[fib].zs : b.st[fib].z;  // uniform
-    tp = z0 + a.c_prev * fp.h;
-    tc = (a.final_stage ? z0 + fp.h : z0) + a.c_cur * fp.h;
+    const double n = a.final_stage ? b.st[fib].n0 : (double)b.st[fib].steps;  // uniform
+    tp = (n + a.c_prev) * fp.h;
+    tc = ((a.final_stage ? n + 1.0 : n) + a.c_cur) * fp.h;
     same = a.has_prev && want_cur && tp == tc;
-    if (a.set_zs && blockIdx.x == 0 && threadIdx.x == 0) b.st[fib].zs = z0;
+    if (a.set_n0 && blockIdx.x == 0 && threadIdx.x == 0) b.st[fib].n0 = n;
   }
   if (a.final_stage && a.advance_z && blockIdx.x == 0 && threadIdx.x == 0) {
     b.st[fib].z += fp.h;  // z += h (repeated FP64 addition)

@@ -905,12 +905,12 @@ mmp_status create_impl(mmp_ctx* ctx, uint32_t n_fibers, const mmp_params* params
     a.prev_stage = i - 1;
     a.prev_is_first = (i == 1);
     a.final_stage = (i == P.rk.s);
-    // mixed precision: Lawson factors from Im L and the FP64 z (no tables exist); the close
-    // head takes the step's start z from StepState::zs, which head 1 records
+    // mixed precision: Lawson factors from Im L and the step count (no tables exist); the close
+    // head takes the count at the step's start from StepState::n0, which head 1 records
     if (!lawson_tables(P, s->mixed)) {
       a.recompute_e = 1;
       a.imL = s->imL;
-      a.set_zs = (i == 1);
+      a.set_n0 = (i == 1);
       if (i > 0) a.c_prev = P.rk.c[i - 1];
       a.c_cur = (i < P.rk.s) ? P.rk.c[i] : P.rk.c[0];
     }

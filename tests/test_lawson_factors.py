@@ -9,10 +9,10 @@ FP32_UNIT = 2.0 ** -24
 
 
 def test_phase_factors_stay_within_rounding_of_advanced_tables():
-    """exp(i Im L (z + c_i h)), with z summed step by step in FP64 as the device keeps it,
-    against E_i(0) * E1^n after n steps of a 20000-step GRIN solve: the two differ only by the
-    tables' rounding and z's accumulated addition, which grow linearly with the steps and stay
-    far below the FP32 stages' unit roundoff."""
+    """exp(i Im L (n + c_i) h), the mixed-precision heads' factor after n steps, against
+    E_i(0) * E1^n after n steps of a 20000-step GRIN solve: the two differ only by the tables'
+    rounding, which grows linearly with the steps and stays far below the FP32 stages' unit
+    roundoff. (A phase from z summed step by step in FP64 would sit 8.6e-10 away at the end.)"""
     steps = 20000
     sp = configs.grin_fiber(M=3, nt=4096, n_steps=steps, symmetric=True)
     arr = plan.plan_arrays(sp, pair_decode=1)
@@ -23,16 +23,14 @@ def test_phase_factors_stay_within_rounding_of_advanced_tables():
     c = np.array([0.0, 0.5, 0.5, 1.0])  # RK4
     # the prepared tables are the phases at z = 0
     assert np.abs(tables - np.exp(1j * im_l[None] * (c[:, None, None] * h))).max() < 1e-15
-    z = 0.0
     worst = []
     for n in range(1, steps + 1):
         tables *= e1
-        z += h
         if n % 5000 == 0:
-            direct = np.exp(1j * im_l[None] * (z + c[:, None, None] * h))
+            direct = np.exp(1j * im_l[None] * ((n + c[:, None, None]) * h))
             worst.append(float(np.abs(tables - direct).max()))
     print("max |E_table - E_phase| every 5000 steps:", worst)
-    assert worst[-1] <= 2e-9
-    assert worst[-1] < FP32_UNIT / 10
-    # linear growth (a wrong sign or c would show as an O(1) difference at once)
-    assert worst[0] < worst[-1] and worst[0] <= 1e-9
+    assert worst[-1] <= 1e-11
+    assert worst[-1] < FP32_UNIT / 1000
+    # growth with the steps (a wrong sign, c or step count would show as an O(1) difference)
+    assert worst[0] < worst[-1]
