@@ -132,10 +132,10 @@ def test_mixed_separate_column_passes(M, nt):
 
 
 def test_mixed_lawson_factors_follow_z():
-    """Mixed precision forms the Lawson factors from Im L and the fiber's z instead of advancing
-    tables: a solve split into advance() calls (16-step and 1-step graphs, the first stage head
-    issued by each call) equals the same solve in one call bit for bit, and it holds the oracle's
-    bar after the same steps."""
+    """Mixed precision forms the Lawson factors from Im L and the fiber's step count instead of
+    advancing tables: a solve split into advance() calls (16-step and 1-step graphs, the first
+    stage head issued by each call) equals the same solve in one call bit for bit, and it holds
+    the oracle's bar after the same steps."""
     require_device()
     sp = configs.grin_fiber(M=6, nt=4096, symmetric=True)
     steps = 37
